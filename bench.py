@@ -1,0 +1,213 @@
+#!/usr/bin/env python3
+"""Swendsen-Wang throughput of the MI355X engine (site-updates/s).
+
+Default workload: the batched uniform scan -- L=32, p=0.05, 40 temperatures
+in linspace(1.0, 3.0) x 32 disorder realizations = 1280 lattices, one
+workgroup per lattice with everything in LDS, 2000 thermalisation + 20000
+measured sweeps (10 bins) with the moments accumulated on the device.  A step
+is one sweep of every lattice of the batch; the default times the whole
+22000-sweep run.  Disorder is synthetic (seeded numpy draws, as the reference
+generates it), spins start hot.
+
+Printed as one JSON line, with
+  roofline      achieved algorithmic HBM bytes of the dominant kernel
+                (4 B per site-update: spin read + write, J_h + J_v read) per
+                average launch duration, vs MI355X's 8 TB/s;
+  plain         the one-lattice-at-a-time tiled path at the same L;
+  cpu_baseline  the oracle's C restatement of the reference sweep on all host
+                cores (one lattice per thread), on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps 22000] [--warmup 200]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "rbim-fk-percolation_amd"))
+
+PEAK_HBM_GBS = 8000.0
+ALGO_BYTES_PER_SITE_UPDATE = 4
+
+
+def scan_specs(L, p, temperatures, R, seed):
+    from rbim.engine import LatticeSpec, engine_seed
+    from rbim.disorder import generate_disorder
+    from rbim.simulation import scan_lattices
+
+    disorder_seeds, run_seeds = np.random.SeedSequence(seed).spawn(2)
+    rng = np.random.default_rng(disorder_seeds)
+    realizations = [generate_disorder(L, p, rng) for _ in range(R)]
+    return scan_lattices(L, realizations, temperatures, engine_seed(run_seeds))
+
+
+def run_schedule(K):
+    """(n_therm, n_bins, sweeps_per_bin) with n_therm + 10 * spb == K, ~1/11 therm."""
+    n_bins = 10
+    spb = max(1, (K - K // 11) // n_bins)
+    return K - n_bins * spb, n_bins, spb
+
+
+def gpu_batched(specs, K, W, engine, sweeps_per_launch=0, threads=0):
+    from rbim.engine import Batch
+
+    n_therm, n_bins, spb = run_schedule(K)
+    if W > 0:
+        wn, wb, ws = run_schedule(W)
+        with Batch(specs, n_therm=wn, n_bins=wb, sweeps_per_bin=ws, engine=engine,
+                   sweeps_per_launch=sweeps_per_launch, threads=threads) as warm:
+            warm.run_timed(W)
+    with Batch(specs, n_therm=n_therm, n_bins=n_bins, sweeps_per_bin=spb, engine=engine,
+               sweeps_per_launch=sweeps_per_launch, threads=threads) as batch:
+        t0 = time.perf_counter()
+        ms = batch.run_timed(K)
+        wall = time.perf_counter() - t0
+        info = batch.info()
+        acc = batch.moments()
+    sites = sum(s.L * s.L for s in specs)
+    return {
+        "ms": ms,
+        "wall_s": wall,
+        "site_updates": sites * K,
+        "launches": int(info.kernel_launches),
+        "threads": int(info.threads),
+        "sweeps_per_launch": int(info.sweeps_per_launch),
+        "checksum": int(acc[:, :, 17].sum()),
+    }
+
+
+def gpu_plain(spec, K):
+    from rbim.engine import Batch
+
+    n_therm, n_bins, spb = run_schedule(K)
+    with Batch([spec], n_therm=n_therm, n_bins=n_bins, sweeps_per_bin=spb, engine=1) as warm:
+        warm.run_timed(min(K, 64))
+    with Batch([spec], n_therm=n_therm, n_bins=n_bins, sweeps_per_bin=spb, engine=1) as batch:
+        ms = batch.run_timed(K)
+    return spec.L * spec.L * K / (ms / 1e3)
+
+
+def cpu_baseline(specs, sweeps, per_thread):
+    """Oracle C restatement, one lattice per thread on every host core."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import sw_oracle
+    from rbim.clusters import activation_threshold
+
+    sw_oracle.load_c()
+    cores = len(os.sched_getaffinity(0))
+    n = cores * per_thread
+    chosen = [specs[(i * 37) % len(specs)] for i in range(n)]
+
+    def job(spec):
+        s0 = sw_oracle.c_init_spins(spec.L, spec.seed, spec.lattice_id)
+        sw_oracle.c_run(spec.L, s0, spec.J_h, spec.J_v, activation_threshold(spec.beta), spec.seed,
+                        spec.lattice_id, 0, sweeps, sweeps // 11, 1, sweeps - sweeps // 11)
+        return spec.L * spec.L * sweeps
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=cores) as pool:
+        total = sum(pool.map(job, chosen))
+    wall = time.perf_counter() - t0
+    return {
+        "value": total / wall,
+        "unit": "site-updates/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{n} of the workload's lattices (L={chosen[0].L}) x {sweeps} sweeps, "
+                  f"one lattice per thread, {wall:.1f} s wall",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=22000, help="timed sweeps of every lattice")
+    ap.add_argument("--warmup", type=int, default=200, help="untimed sweeps on an identical batch")
+    ap.add_argument("--L", type=int, default=32)
+    ap.add_argument("--p", type=float, default=0.05)
+    ap.add_argument("--temps", type=int, default=40)
+    ap.add_argument("--realizations", type=int, default=32)
+    ap.add_argument("--engine", type=int, default=0, help="0 auto, 1 plain (tiled), 2 batched (LDS)")
+    ap.add_argument("--sweeps-per-launch", type=int, default=0)
+    ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--plain-steps", type=int, default=400)
+    ap.add_argument("--cpu-sweeps", type=int, default=2500)
+    ap.add_argument("--cpu-per-thread", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: every box in this build has one MI355X "
+              f"(asked for --gpus {args.gpus}); independent lattices shard across GPUs "
+              f"with no communication, so the N-GPU figure is N x the 1-GPU one.", file=sys.stderr)
+        sys.exit(2)
+
+    temperatures = np.linspace(1.0, 3.0, args.temps)
+    specs = scan_specs(args.L, args.p, temperatures, args.realizations, seed=0)
+    res = gpu_batched(specs, args.steps, args.warmup, args.engine, args.sweeps_per_launch, args.threads)
+    value = res["site_updates"] / (res["ms"] / 1e3)
+    ms_per_step = res["ms"] / args.steps
+    n_lat = len(specs)
+    sites = args.L * args.L * n_lat
+    # dominant kernel: every launch covers all lattices for sweeps_per_launch sweeps
+    launches = max(1, res["launches"])
+    avg_launch_ms = res["ms"] / launches
+    bytes_per_launch = ALGO_BYTES_PER_SITE_UPDATE * sites * args.steps / launches
+    achieved = bytes_per_launch / (avg_launch_ms / 1e3) / 1e9
+    plain = gpu_plain(specs[len(specs) // 2], args.plain_steps)
+    line = {
+        "metric": "SW site-updates/s on 1 MI355X, batched vs plain, at L=32 & L=256 (bit-exact)",
+        "value": value,
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8/int32 (FP64 moments)",
+        "data": "synthetic: seeded +-J disorder (numpy, as the reference draws it), hot start",
+        "config": {
+            "workload": f"batched uniform scan: L={args.L}, p={args.p}, {args.temps} T in "
+                        f"linspace(1,3) x {args.realizations} realizations = {n_lat} lattices, "
+                        f"{args.steps} sweeps ({run_schedule(args.steps)[0]} therm + "
+                        f"{args.steps - run_schedule(args.steps)[0]} measured)",
+            "L": args.L,
+            "lattices": n_lat,
+            "engine": {0: "auto", 1: "plain", 2: "batched"}[args.engine],
+            "threads_per_lattice": res["threads"],
+            "sweeps_per_launch": res["sweeps_per_launch"],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBS,
+            "traffic": None,
+            "kernel": "lds_sweep_kernel",
+            "avg_launch_ms": avg_launch_ms,
+            "launches": launches,
+        },
+        "plain": {"value": plain, "unit": "site-updates/s", "L": args.L,
+                  "sample": f"one lattice, {args.plain_steps} sweeps, tiled kernels in hipGraphs"},
+        "batched_vs_plain": value / plain,
+    }
+    if not args.no_cpu:
+        cpu = cpu_baseline(specs, args.cpu_sweeps, args.cpu_per_thread)
+        line["cpu_baseline"] = cpu
+        line["batched_vs_cpu"] = value / cpu["value"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,198 @@
+/*
+ * rbim_hip.h -- C ABI of librbim_hip.so, the MI355X (gfx950) Swendsen-Wang
+ * engine for the 2D +-J random-bond Ising model.
+ *
+ * The reference (Amirhossein-Manouchehri/rbim-fk-percolation) has no FFI: its
+ * operator boundary is a set of Python functions, and its only compiled code is
+ * the numba-jitted union-find.  Each entry point below replaces one of those
+ * seams; the Python package `rbim` (rbim-fk-percolation_amd/rbim) binds them
+ * with ctypes and keeps the reference's Python signatures on top.
+ *
+ * Conventions
+ *   - Every function returns an int status: RBIM_OK (0) or a negative
+ *     RBIM_E_* code; rbim_last_error() then holds a message.  No C++
+ *     exception crosses the ABI.
+ *   - Host buffers are owned by the caller and must be C-contiguous with the
+ *     exact element types named.  Device memory is owned by the library and
+ *     hidden behind rbim_ctx / rbim_batch handles.
+ *   - A handle is used by one host thread at a time.
+ *   - Geometry follows src/rbim/lattice.py:1-24 of the reference: site
+ *     n = x*L + y (x slow); bond 2n joins n to (x+1, y), bond 2n+1 joins n to
+ *     (x, y+1); J_h[x, y] sits on bond 2n and J_v[x, y] on bond 2n+1.
+ *   - Random draws come from Philox4x32-10 keyed by the 64-bit lattice seed,
+ *     counter (site>>1, sweep, lattice_id, 0); results never depend on launch
+ *     shape, engine or batch composition.
+ */
+#ifndef RBIM_HIP_H
+#define RBIM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RBIM_ABI_VERSION 1
+
+/* Moment slots per (lattice, bin); sums over the measured sweeps of a bin.
+ * 128-bit sums are stored as (lo, hi) uint64 pairs.                       */
+#define RBIM_ACC_SLOTS 20
+enum rbim_acc_slot {
+  RBIM_ACC_E = 0,          /* int64  sum E                     */
+  RBIM_ACC_E2_LO = 1,      /* uint128 sum E^2 (lo, hi)          */
+  RBIM_ACC_E2_HI = 2,
+  RBIM_ACC_M = 3,          /* int64  sum M                     */
+  RBIM_ACC_ABSM = 4,       /* uint64 sum |M|                   */
+  RBIM_ACC_M2_LO = 5,      /* uint128 sum M^2                  */
+  RBIM_ACC_M2_HI = 6,
+  RBIM_ACC_M4_LO = 7,      /* uint128 sum M^4                  */
+  RBIM_ACC_M4_HI = 8,
+  RBIM_ACC_SMAX = 9,       /* uint64 sum S_max                 */
+  RBIM_ACC_ABSMC = 10,     /* uint64 sum |M_C| of the largest  */
+  RBIM_ACC_S2 = 11,        /* uint64 sum S_2                   */
+  RBIM_ACC_RATIO = 12,     /* double sum |M_C|/S_max (bits)    */
+  RBIM_ACC_NCL = 13,       /* uint64 sum n_clusters            */
+  RBIM_ACC_WX = 14,        /* uint64 count wrap_x              */
+  RBIM_ACC_WY = 15,        /* uint64 count wrap_y              */
+  RBIM_ACC_WANY = 16,      /* uint64 count wrap_x || wrap_y    */
+  RBIM_ACC_COUNT = 17      /* uint64 measured sweeps in the bin */
+};
+
+enum rbim_status {
+  RBIM_OK = 0,
+  RBIM_E_INVAL = -1, /* bad argument (the reference raises ValueError)   */
+  RBIM_E_HIP = -2,   /* HIP runtime error                                */
+  RBIM_E_OOM = -3,   /* device allocation failed                         */
+  RBIM_E_ADMIT = -4, /* batch does not fit the device memory budget      */
+  RBIM_E_STATE = -5  /* handle in the wrong state (e.g. run past the end) */
+};
+
+enum rbim_engine {
+  RBIM_ENGINE_AUTO = 0,    /* LDS-resident kernel when L <= 128, else tiled  */
+  RBIM_ENGINE_PLAIN = 1,   /* tiled multi-kernel path, sweep by sweep        */
+  RBIM_ENGINE_BATCHED = 2  /* LDS-resident persistent kernel (L <= 128)      */
+};
+
+enum rbim_start { RBIM_START_HOT = 0, RBIM_START_COLD = 1 };
+
+typedef struct rbim_ctx rbim_ctx;
+typedef struct rbim_batch rbim_batch;
+
+/* Observables of one sweep (src/rbim/swendsen_wang.py:24-53 SweepResult).
+ * "largest" / "second" order clusters by size, ties by the smaller minimum
+ * site index. */
+typedef struct {
+  int64_t energy, magnetization, n_clusters;
+  int64_t largest_size, largest_abs_mag, second_size, second_abs_mag;
+  int32_t wrap_x, wrap_y;
+} rbim_sweep_stats;
+
+/* A batch of independent lattices advanced together.  Lattice i has L[i]
+ * sites per side; its couplings are the L[i]^2 entries of Jh / Jv starting at
+ * sum_{j<i} L[j]^2 (x-major, +-1 int8).  Replaces the task list that
+ * run_temperature_scan farms out to processes (src/rbim/simulation.py:384-399). */
+typedef struct {
+  int32_t n_lattices;
+  const int32_t* L;
+  const int8_t* Jh;
+  const int8_t* Jv;
+  const uint32_t* thr31;      /* round((1 - exp(-2/T)) * 2^31) per lattice       */
+  const uint64_t* seed;       /* Philox key per lattice                           */
+  const uint32_t* lattice_id; /* Philox counter word per lattice                  */
+  const int8_t* spins0;       /* initial spins (concatenated) or NULL -> start    */
+  int32_t start;              /* rbim_start, used when spins0 == NULL             */
+  int32_t n_bins;             /* >= 1                                             */
+  int64_t sweeps_per_bin;     /* >= 1                                             */
+  int64_t n_therm;            /* >= 0                                             */
+  int32_t engine;             /* rbim_engine                                      */
+  int32_t sweeps_per_launch;  /* LDS kernel sweeps per launch / tiled sweeps per
+                                 graph; 0 -> default                              */
+  int32_t threads;            /* LDS kernel workgroup size (64..1024); 0 -> default */
+} rbim_batch_desc;
+
+typedef struct {
+  int32_t n_groups;           /* lattices are grouped by (L, engine)           */
+  int32_t lds_groups, tiled_groups;
+  int32_t threads;            /* LDS kernel workgroup size of the first group  */
+  int32_t sweeps_per_launch;
+  int64_t device_bytes;       /* device memory held by the batch               */
+  int64_t sweeps_done;        /* sweeps issued so far (all lattices in step)   */
+  int64_t total_sweeps;       /* n_therm + n_bins * sweeps_per_bin             */
+  int64_t kernel_launches;    /* launches issued so far                        */
+} rbim_batch_info;
+
+/* ------------------------------------------------------------ context */
+int rbim_abi_version(void);
+/* Opens device `device`; fails loudly (RBIM_E_HIP) when no GPU is present. */
+int rbim_open(int device, rbim_ctx** out);
+int rbim_close(rbim_ctx* ctx);
+/* Message of the last failure on ctx (ctx == NULL: last failure anywhere). */
+const char* rbim_last_error(const rbim_ctx* ctx);
+int rbim_device_memory(rbim_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
+
+/* ------------------------------------------- one-lattice entry points */
+/* Replaces label_and_wrap / _label_and_wrap (src/rbim/clusters.py:144-159,
+ * src/rbim/_unionfind.py:91-136).  active: 2N bytes in bond order.
+ * labels (N, or NULL): 0..n_clusters-1 numbered in increasing min-site order,
+ * as the reference numbers them.  canon (N, or NULL): min site index of each
+ * site's cluster.  wrap_xy[2]: exact winding flags.  engine: PLAIN (tiled
+ * kernels, any L >= 3) or BATCHED (LDS kernel, L <= 128). */
+int rbim_label_and_wrap(rbim_ctx* ctx, int32_t L, const uint8_t* active, int32_t engine,
+                        int32_t* labels, int32_t* canon, int32_t* n_clusters,
+                        int32_t* wrap_xy);
+
+/* Replaces activate_bonds (src/rbim/clusters.py:48-79) with the engine's
+ * Philox stream.  active: 2N bytes out; flipbits (N, or NULL): flip bit of the
+ * cluster whose min site is n. */
+int rbim_activate(rbim_ctx* ctx, int32_t L, const int8_t* spins, const int8_t* Jh,
+                  const int8_t* Jv, uint32_t thr31, uint64_t seed, uint32_t lattice_id,
+                  uint32_t sweep, uint8_t* active, uint8_t* flipbits);
+
+/* One SW update with given occupation and flip bits (src/rbim/swendsen_wang.py:
+ * 137-172 after activate_bonds): label, flip clusters whose min site m has
+ * flip_by_minsite[m] != 0, then measure.  spins (N, int8 +-1) updated in place;
+ * labels (N, or NULL) as in rbim_label_and_wrap. */
+int rbim_sweep_given_bits(rbim_ctx* ctx, int32_t L, int8_t* spins, const int8_t* Jh,
+                          const int8_t* Jv, const uint8_t* active,
+                          const uint8_t* flip_by_minsite, int32_t engine, int32_t* labels,
+                          rbim_sweep_stats* out);
+
+/* One full SW update of one lattice (sw_sweep, src/rbim/swendsen_wang.py:103-172)
+ * with Philox draws for sweep index `sweep`.  out is written when measure != 0. */
+int rbim_sweep(rbim_ctx* ctx, int32_t L, int8_t* spins, const int8_t* Jh, const int8_t* Jv,
+               uint32_t thr31, uint64_t seed, uint32_t lattice_id, uint32_t sweep,
+               int32_t measure, int32_t engine, rbim_sweep_stats* out);
+
+/* cluster_statistics (src/rbim/clusters.py:98-118): sizes and signed cluster
+ * magnetizations from labels 0..n_clusters-1. */
+int rbim_cluster_statistics(rbim_ctx* ctx, int64_t n_sites, const int32_t* labels,
+                            const int8_t* spins, int32_t n_clusters, int64_t* sizes,
+                            int64_t* mags);
+
+/* energy / magnetization (src/rbim/swendsen_wang.py:81-89). */
+int rbim_energy_magnetization(rbim_ctx* ctx, int32_t L, const int8_t* spins, const int8_t* Jh,
+                              const int8_t* Jv, int64_t* energy, int64_t* magnetization);
+
+/* -------------------------------------------------------------- batch */
+/* Device bytes a batch would hold; the serving loop admits by this figure. */
+int rbim_batch_footprint(const rbim_batch_desc* desc, uint64_t* bytes);
+int rbim_batch_create(rbim_ctx* ctx, const rbim_batch_desc* desc, rbim_batch** out);
+/* Queue n_sweeps sweeps of every lattice on the batch's stream (asynchronous).
+ * Fails with RBIM_E_STATE past n_therm + n_bins*sweeps_per_bin. */
+int rbim_batch_run(rbim_batch* b, int64_t n_sweeps);
+/* Like rbim_batch_run, then waits; *elapsed_ms is the time between HIP events
+ * recorded on the batch's stream around the queued work. */
+int rbim_batch_run_timed(rbim_batch* b, int64_t n_sweeps, double* elapsed_ms);
+int rbim_batch_query(rbim_batch* b, int32_t* done);
+int rbim_batch_sync(rbim_batch* b);
+/* Moment sums, [n_lattices][n_bins][RBIM_ACC_SLOTS] in batch order. */
+int rbim_batch_moments(rbim_batch* b, uint64_t* out);
+/* Current spins, concatenated in batch order. */
+int rbim_batch_spins(rbim_batch* b, int8_t* out);
+int rbim_batch_info_get(rbim_batch* b, rbim_batch_info* out);
+int rbim_batch_destroy(rbim_batch* b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RBIM_HIP_H */

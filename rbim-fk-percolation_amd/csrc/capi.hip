@@ -1,0 +1,724 @@
+// C ABI of librbim_hip.so (declared in include/rbim_hip.h): contexts, the
+// one-lattice entry points, and batches of lattices grouped by (L, engine).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+
+struct rbim_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+};
+
+namespace {
+
+std::mutex g_err_mu;
+std::string g_err;
+
+int set_err(rbim_ctx* ctx, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (ctx) ctx->err = buf;
+  std::lock_guard<std::mutex> lk(g_err_mu);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                   \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      return set_err((ctx), e_ == hipErrorOutOfMemory ? RBIM_E_OOM : RBIM_E_HIP,             \
+                     "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__,        \
+                     __LINE__);                                                              \
+  } while (0)
+
+uint32_t magic_for(int L) {
+  return (uint32_t)(((1ull << 32) + (uint64_t)L - 1) / (uint64_t)L);
+}
+
+// Device allocations of one group, released together.
+struct DevArena {
+  std::vector<void*> ptrs;
+  size_t bytes = 0;
+  hipError_t alloc(void** p, size_t n) {
+    if (n == 0) n = 16;
+    hipError_t e = hipMalloc(p, n);
+    if (e == hipSuccess) {
+      ptrs.push_back(*p);
+      bytes += n;
+    }
+    return e;
+  }
+  void release() {
+    for (void* p : ptrs) (void)hipFree(p);
+    ptrs.clear();
+    bytes = 0;
+  }
+};
+
+struct Group {
+  int L = 0, N = 0, n_lat = 0;
+  bool lds = false;
+  int threads = 256;
+  int per_launch = 1;  // LDS sweeps per launch, or tiled sweeps per graph
+  std::vector<int> idx;
+  LatticeSet set{};
+  TiledScratch tile{};
+  DevArena arena;
+  hipGraphExec_t graph = nullptr;
+};
+
+size_t group_bytes(int L, int n_lat, bool lds, int n_bins) {
+  const size_t N = (size_t)L * L;
+  size_t b = N * n_lat * 2 + (size_t)n_lat * (4 + 8 + 4 + 8 + sizeof(rbim_sweep_stats)) +
+             (size_t)n_lat * n_bins * RBIM_ACC_SLOTS * 8;
+  if (!lds) {
+    const size_t tx = (L + RBIM_TILE - 1) / RBIM_TILE;
+    b += N * n_lat * (1 + 8 + 8 + 8) + (size_t)n_lat * tx * tx * 5 * 8 + (size_t)n_lat * 4;
+  }
+  return b;
+}
+
+int default_threads(int L) {
+  const int N = L * L;
+  if (N <= 256) return 64;
+  if (N <= 1024) return 256;
+  if (N <= 4096) return 512;
+  return 1024;
+}
+
+int default_lds_sweeps(int L) {
+  const int N = L * L;
+  return N <= 1024 ? 2000 : N <= 4096 ? 500 : 200;
+}
+
+// Allocates and fills the device side of a group.  jh/jv/spins0 are the
+// group's host slices (concatenated); thr/seed/lid per lattice.
+int group_alloc(rbim_ctx* ctx, Group& g, const std::vector<uint8_t>& jbits,
+                const std::vector<uint32_t>& thr, const std::vector<uint64_t>& seed,
+                const std::vector<uint32_t>& lid, const int8_t* spins0, int start,
+                int64_t n_therm, int64_t spb, int64_t n_bins, int64_t g0) {
+  const size_t NN = (size_t)g.N * g.n_lat;
+  LatticeSet& s = g.set;
+  s.L = g.L;
+  s.N = g.N;
+  s.n_lat = g.n_lat;
+  s.magic = magic_for(g.L);
+  s.n_therm = n_therm;
+  s.spb = spb;
+  s.n_bins = n_bins;
+  s.total = n_therm + n_bins * spb;
+  void* p;
+  HIP_TRY(ctx, g.arena.alloc(&p, NN)); s.spins = (int8_t*)p;
+  HIP_TRY(ctx, g.arena.alloc(&p, NN)); s.jbits = (const uint8_t*)p;
+  HIP_TRY(ctx, hipMemcpyAsync(p, jbits.data(), NN, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, g.arena.alloc(&p, 4 * g.n_lat)); s.thr31 = (const uint32_t*)p;
+  HIP_TRY(ctx, hipMemcpyAsync(p, thr.data(), 4 * g.n_lat, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, g.arena.alloc(&p, 8 * g.n_lat)); s.seed = (const uint64_t*)p;
+  HIP_TRY(ctx, hipMemcpyAsync(p, seed.data(), 8 * g.n_lat, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, g.arena.alloc(&p, 4 * g.n_lat)); s.lat_id = (const uint32_t*)p;
+  HIP_TRY(ctx, hipMemcpyAsync(p, lid.data(), 4 * g.n_lat, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, g.arena.alloc(&p, 8 * g.n_lat)); s.sweep_ctr = (int64_t*)p;
+  {
+    std::vector<int64_t> c(g.n_lat, g0);
+    HIP_TRY(ctx, hipMemcpyAsync(p, c.data(), 8 * g.n_lat, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  const size_t accb = (size_t)g.n_lat * n_bins * RBIM_ACC_SLOTS * 8;
+  HIP_TRY(ctx, g.arena.alloc(&p, accb)); s.acc = (uint64_t*)p;
+  HIP_TRY(ctx, hipMemsetAsync(p, 0, accb, ctx->stream));
+  HIP_TRY(ctx, g.arena.alloc(&p, sizeof(rbim_sweep_stats) * g.n_lat)); s.last = (rbim_sweep_stats*)p;
+  HIP_TRY(ctx, hipMemsetAsync(p, 0, sizeof(rbim_sweep_stats) * g.n_lat, ctx->stream));
+  if (!g.lds) {
+    TiledScratch& t = g.tile;
+    t.tiles_x = t.tiles_y = (g.L + RBIM_TILE - 1) / RBIM_TILE;
+    HIP_TRY(ctx, g.arena.alloc(&p, NN)); t.occ = (uint8_t*)p;
+    HIP_TRY(ctx, g.arena.alloc(&p, NN * 8)); t.uf = (uint64_t*)p;
+    HIP_TRY(ctx, g.arena.alloc(&p, NN * 8)); t.lst = (uint64_t*)p;
+    HIP_TRY(ctx, g.arena.alloc(&p, NN * 8)); t.st = (uint64_t*)p;
+    const size_t pb = (size_t)g.n_lat * t.tiles_x * t.tiles_y * 5 * 8;
+    HIP_TRY(ctx, g.arena.alloc(&p, pb)); t.part = (int64_t*)p;
+    HIP_TRY(ctx, hipMemsetAsync(p, 0, pb, ctx->stream));
+    HIP_TRY(ctx, g.arena.alloc(&p, 4 * g.n_lat)); t.wrapf = (uint32_t*)p;
+    HIP_TRY(ctx, hipMemsetAsync(p, 0, 4 * g.n_lat, ctx->stream));
+  }
+  if (spins0) {
+    HIP_TRY(ctx, hipMemcpyAsync(s.spins, spins0, NN, hipMemcpyHostToDevice, ctx->stream));
+  } else {
+    HIP_TRY(ctx, launch_init_spins(s, start, ctx->stream));
+  }
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return RBIM_OK;
+}
+
+// Queue `n` sweeps of one group on `stream`; counts kernel launches.
+int group_run(rbim_ctx* ctx, Group& g, int64_t n, hipStream_t stream, int64_t* launches) {
+  if (g.lds) {
+    while (n > 0) {
+      const int k = (int)std::min<int64_t>(n, g.per_launch);
+      HIP_TRY(ctx, launch_lds_sweeps(g.set, g.threads, k, stream));
+      ++*launches;
+      n -= k;
+    }
+    return RBIM_OK;
+  }
+  const int S = std::max(1, g.per_launch);
+  if (n >= S && S > 1) {
+    if (!g.graph) {
+      hipGraph_t graph;
+      HIP_TRY(ctx, hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      for (int i = 0; i < S; ++i) {
+        hipError_t e = launch_tiled_sweep(g.set, g.tile, stream);
+        if (e != hipSuccess) {
+          hipGraph_t junk;
+          (void)hipStreamEndCapture(stream, &junk);
+          return set_err(ctx, RBIM_E_HIP, "tiled sweep capture failed: %s", hipGetErrorString(e));
+        }
+      }
+      HIP_TRY(ctx, hipStreamEndCapture(stream, &graph));
+      HIP_TRY(ctx, hipGraphInstantiate(&g.graph, graph, nullptr, nullptr, 0));
+      HIP_TRY(ctx, hipGraphDestroy(graph));
+    }
+    while (n >= S) {
+      HIP_TRY(ctx, hipGraphLaunch(g.graph, stream));
+      *launches += (int64_t)S * tiled_kernels_per_sweep();
+      n -= S;
+    }
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    HIP_TRY(ctx, launch_tiled_sweep(g.set, g.tile, stream));
+    *launches += tiled_kernels_per_sweep();
+  }
+  return RBIM_OK;
+}
+
+void group_free(Group& g) {
+  if (g.graph) (void)hipGraphExecDestroy(g.graph);
+  g.graph = nullptr;
+  g.arena.release();
+}
+
+bool valid_L(int L) { return L >= 3 && L <= 16384; }
+
+// One lattice, one sweep (index `sweep`), optionally with given bits.
+int run_single(rbim_ctx* ctx, int L, int engine, int8_t* spins, const int8_t* Jh, const int8_t* Jv,
+               const uint8_t* occ_pairs, const uint8_t* flips, uint32_t thr, uint64_t seed,
+               uint32_t lid, uint32_t sweep, int measure, int32_t* canon_host,
+               rbim_sweep_stats* stats, uint8_t* occ_host) {
+  if (!valid_L(L)) return set_err(ctx, RBIM_E_INVAL, "L must be at least 3 (got %d)", L);
+  const bool lds = engine == RBIM_ENGINE_BATCHED;
+  if (engine != RBIM_ENGINE_PLAIN && engine != RBIM_ENGINE_BATCHED && engine != RBIM_ENGINE_AUTO)
+    return set_err(ctx, RBIM_E_INVAL, "unknown engine %d", engine);
+  if (lds && L > RBIM_LDS_MAX_L)
+    return set_err(ctx, RBIM_E_INVAL, "the LDS engine holds L <= %d (got %d)", RBIM_LDS_MAX_L, L);
+  const int N = L * L;
+  Group g;
+  g.L = L;
+  g.N = N;
+  g.n_lat = 1;
+  g.lds = lds;
+  g.threads = default_threads(L);
+  g.per_launch = 1;
+  std::vector<uint8_t> jb(N);
+  for (int n = 0; n < N; ++n)
+    jb[n] = (uint8_t)((Jh && Jh[n] < 0 ? 1 : 0) | (Jv && Jv[n] < 0 ? 2 : 0));
+  const int64_t n_therm = measure ? (int64_t)sweep : (int64_t)sweep + 1;
+  int rc = group_alloc(ctx, g, jb, {thr}, {seed}, {lid}, spins, RBIM_START_COLD, n_therm, 1, 1,
+                       (int64_t)sweep);
+  if (rc != RBIM_OK) { group_free(g); return rc; }
+  DevArena extra;
+  void* p;
+  uint8_t* d_occ = nullptr;
+  uint8_t* d_flip = nullptr;
+  int32_t* d_canon = nullptr;
+  auto fail = [&](int code) { extra.release(); group_free(g); return code; };
+#define TRY2(expr)                                                                            \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return fail(set_err(ctx, RBIM_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)));  \
+  } while (0)
+  if (occ_pairs) {
+    TRY2(extra.alloc(&p, N)); d_occ = (uint8_t*)p;
+    TRY2(hipMemcpyAsync(d_occ, occ_pairs, N, hipMemcpyHostToDevice, ctx->stream));
+    TRY2(extra.alloc(&p, N)); d_flip = (uint8_t*)p;
+    TRY2(hipMemcpyAsync(d_flip, flips, N, hipMemcpyHostToDevice, ctx->stream));
+    g.set.given_occ = d_occ;
+    g.set.given_flip = d_flip;
+  }
+  if (canon_host) {
+    TRY2(extra.alloc(&p, 4 * (size_t)N)); d_canon = (int32_t*)p;
+    g.set.canon_out = d_canon;
+  }
+  int64_t launches = 0;
+  rc = group_run(ctx, g, 1, ctx->stream, &launches);
+  if (rc != RBIM_OK) return fail(rc);
+  TRY2(hipMemcpyAsync(spins, g.set.spins, N, hipMemcpyDeviceToHost, ctx->stream));
+  if (canon_host) TRY2(hipMemcpyAsync(canon_host, d_canon, 4 * (size_t)N, hipMemcpyDeviceToHost, ctx->stream));
+  if (stats) TRY2(hipMemcpyAsync(stats, g.set.last, sizeof(rbim_sweep_stats), hipMemcpyDeviceToHost, ctx->stream));
+  if (occ_host) TRY2(hipMemcpyAsync(occ_host, g.tile.occ, N, hipMemcpyDeviceToHost, ctx->stream));
+  TRY2(hipStreamSynchronize(ctx->stream));
+#undef TRY2
+  extra.release();
+  group_free(g);
+  return RBIM_OK;
+}
+
+// active[2N] in bond order -> per-site bits (bit0 horizontal, bit1 vertical)
+std::vector<uint8_t> pack_active(const uint8_t* active, int N) {
+  std::vector<uint8_t> o(N);
+  for (int n = 0; n < N; ++n) o[n] = (uint8_t)((active[2 * n] ? 1 : 0) | (active[2 * n + 1] ? 2 : 0));
+  return o;
+}
+
+// labels (min-site order) from canonical labels, on the device.
+int ranks_from_canon(rbim_ctx* ctx, const int32_t* canon, int N, int32_t* labels, int32_t* ncl) {
+  DevArena a;
+  void* p;
+  HIP_TRY(ctx, a.alloc(&p, 4 * (size_t)N * 3 + 16));
+  int32_t* d_canon = (int32_t*)p;
+  int32_t* d_lab = d_canon + N;
+  int32_t* d_rank = d_lab + N;
+  int32_t* d_n = d_rank + N;
+  hipError_t e = hipMemcpyAsync(d_canon, canon, 4 * (size_t)N, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = launch_label_rank(d_canon, N, d_lab, d_rank, d_n, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(labels, d_lab, 4 * (size_t)N, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess && ncl) e = hipMemcpyAsync(ncl, d_n, 4, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  a.release();
+  if (e != hipSuccess) return set_err(ctx, RBIM_E_HIP, "label ranking failed: %s", hipGetErrorString(e));
+  return RBIM_OK;
+}
+
+}  // namespace
+
+// =================================================================== batch
+struct rbim_batch {
+  rbim_ctx* ctx = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<Group> groups;
+  int n_lat = 0;
+  std::vector<int64_t> off;  // site offset of lattice i (batch order)
+  std::vector<int> Ls;
+  int64_t n_therm = 0, spb = 1, n_bins = 1, total = 0, issued = 0, launches = 0;
+};
+
+extern "C" {
+
+int rbim_abi_version(void) { return RBIM_ABI_VERSION; }
+
+const char* rbim_last_error(const rbim_ctx* ctx) {
+  if (ctx) return ctx->err.c_str();
+  std::lock_guard<std::mutex> lk(g_err_mu);
+  static thread_local std::string copy;
+  copy = g_err;
+  return copy.c_str();
+}
+
+int rbim_open(int device, rbim_ctx** out) {
+  if (!out) return set_err(nullptr, RBIM_E_INVAL, "out is NULL");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= 0)
+    return set_err(nullptr, RBIM_E_HIP, "no HIP device available (%s)",
+                   e != hipSuccess ? hipGetErrorString(e) : "device count is 0");
+  if (device < 0 || device >= n)
+    return set_err(nullptr, RBIM_E_INVAL, "device %d out of range (have %d)", device, n);
+  rbim_ctx* c = new rbim_ctx();
+  c->device = device;
+  e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return set_err(nullptr, RBIM_E_HIP, "context creation failed: %s", hipGetErrorString(e));
+  }
+  *out = c;
+  return RBIM_OK;
+}
+
+int rbim_close(rbim_ctx* ctx) {
+  if (!ctx) return RBIM_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return RBIM_OK;
+}
+
+int rbim_device_memory(rbim_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes) {
+  if (!ctx) return set_err(nullptr, RBIM_E_INVAL, "ctx is NULL");
+  size_t f = 0, t = 0;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return RBIM_OK;
+}
+
+int rbim_label_and_wrap(rbim_ctx* ctx, int32_t L, const uint8_t* active, int32_t engine,
+                        int32_t* labels, int32_t* canon, int32_t* n_clusters, int32_t* wrap_xy) {
+  if (!ctx || !active) return set_err(ctx, RBIM_E_INVAL, "NULL argument");
+  if (!valid_L(L)) return set_err(ctx, RBIM_E_INVAL, "L must be at least 3 (got %d)", L);
+  const int N = L * L;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  std::vector<uint8_t> occ = pack_active(active, N);
+  std::vector<uint8_t> flips(N, 0);
+  std::vector<int8_t> spins(N, 1);
+  std::vector<int32_t> cn(N);
+  rbim_sweep_stats st{};
+  int rc = run_single(ctx, L, engine, spins.data(), nullptr, nullptr, occ.data(), flips.data(), 0,
+                      0, 0, 0, 1, cn.data(), &st, nullptr);
+  if (rc != RBIM_OK) return rc;
+  if (canon) std::memcpy(canon, cn.data(), 4 * (size_t)N);
+  if (n_clusters) *n_clusters = (int32_t)st.n_clusters;
+  if (wrap_xy) {
+    wrap_xy[0] = st.wrap_x;
+    wrap_xy[1] = st.wrap_y;
+  }
+  if (labels) return ranks_from_canon(ctx, cn.data(), N, labels, nullptr);
+  return RBIM_OK;
+}
+
+int rbim_activate(rbim_ctx* ctx, int32_t L, const int8_t* spins, const int8_t* Jh, const int8_t* Jv,
+                  uint32_t thr31, uint64_t seed, uint32_t lattice_id, uint32_t sweep,
+                  uint8_t* active, uint8_t* flipbits) {
+  if (!ctx || !spins || !Jh || !Jv || !active) return set_err(ctx, RBIM_E_INVAL, "NULL argument");
+  if (!valid_L(L)) return set_err(ctx, RBIM_E_INVAL, "L must be at least 3 (got %d)", L);
+  if (thr31 > (1u << 31)) return set_err(ctx, RBIM_E_INVAL, "thr31 exceeds 2^31");
+  const int N = L * L;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  std::vector<int8_t> s(spins, spins + N);
+  std::vector<uint8_t> occ(N);
+  int rc = run_single(ctx, L, RBIM_ENGINE_PLAIN, s.data(), Jh, Jv, nullptr, nullptr, thr31, seed,
+                      lattice_id, sweep, 0, nullptr, nullptr, occ.data());
+  if (rc != RBIM_OK) return rc;
+  for (int n = 0; n < N; ++n) {
+    active[2 * n] = occ[n] & 1u;
+    active[2 * n + 1] = (occ[n] >> 1) & 1u;
+    if (flipbits) flipbits[n] = (occ[n] >> 2) & 1u;
+  }
+  return RBIM_OK;
+}
+
+int rbim_sweep_given_bits(rbim_ctx* ctx, int32_t L, int8_t* spins, const int8_t* Jh,
+                          const int8_t* Jv, const uint8_t* active, const uint8_t* flip_by_minsite,
+                          int32_t engine, int32_t* labels, rbim_sweep_stats* out) {
+  if (!ctx || !spins || !Jh || !Jv || !active || !flip_by_minsite)
+    return set_err(ctx, RBIM_E_INVAL, "NULL argument");
+  if (!valid_L(L)) return set_err(ctx, RBIM_E_INVAL, "L must be at least 3 (got %d)", L);
+  const int N = L * L;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  std::vector<uint8_t> occ = pack_active(active, N);
+  std::vector<uint8_t> fl(N);
+  for (int n = 0; n < N; ++n) fl[n] = flip_by_minsite[n] ? 1 : 0;
+  std::vector<int32_t> cn(labels ? N : 0);
+  rbim_sweep_stats st{};
+  int rc = run_single(ctx, L, engine, spins, Jh, Jv, occ.data(), fl.data(), 0, 0, 0, 0, 1,
+                      labels ? cn.data() : nullptr, &st, nullptr);
+  if (rc != RBIM_OK) return rc;
+  if (out) *out = st;
+  if (labels) return ranks_from_canon(ctx, cn.data(), N, labels, nullptr);
+  return RBIM_OK;
+}
+
+int rbim_sweep(rbim_ctx* ctx, int32_t L, int8_t* spins, const int8_t* Jh, const int8_t* Jv,
+               uint32_t thr31, uint64_t seed, uint32_t lattice_id, uint32_t sweep, int32_t measure,
+               int32_t engine, rbim_sweep_stats* out) {
+  if (!ctx || !spins || !Jh || !Jv) return set_err(ctx, RBIM_E_INVAL, "NULL argument");
+  if (thr31 > (1u << 31)) return set_err(ctx, RBIM_E_INVAL, "thr31 exceeds 2^31");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  rbim_sweep_stats st{};
+  int rc = run_single(ctx, L, engine, spins, Jh, Jv, nullptr, nullptr, thr31, seed, lattice_id,
+                      sweep, measure, nullptr, &st, nullptr);
+  if (rc == RBIM_OK && out && measure) *out = st;
+  return rc;
+}
+
+int rbim_cluster_statistics(rbim_ctx* ctx, int64_t n_sites, const int32_t* labels,
+                            const int8_t* spins, int32_t n_clusters, int64_t* sizes, int64_t* mags) {
+  if (!ctx || !labels || !spins || !sizes || !mags) return set_err(ctx, RBIM_E_INVAL, "NULL argument");
+  if (n_sites <= 0 || n_clusters <= 0) return set_err(ctx, RBIM_E_INVAL, "empty input");
+  for (int64_t i = 0; i < n_sites; ++i)
+    if (labels[i] < 0 || labels[i] >= n_clusters)
+      return set_err(ctx, RBIM_E_INVAL, "label %d at site %lld outside [0, %d)", labels[i],
+                     (long long)i, n_clusters);
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  DevArena a;
+  void* p;
+  HIP_TRY(ctx, a.alloc(&p, (size_t)n_clusters * 16 + (size_t)n_sites * 5 + 16));
+  unsigned long long* d_sz = (unsigned long long*)p;
+  unsigned long long* d_mg = d_sz + n_clusters;
+  int32_t* d_lab = (int32_t*)(d_mg + n_clusters);
+  int8_t* d_sp = (int8_t*)(d_lab + n_sites);
+  hipError_t e = hipMemsetAsync(d_sz, 0, (size_t)n_clusters * 16, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_lab, labels, 4 * (size_t)n_sites, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_sp, spins, (size_t)n_sites, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = launch_cluster_stats(d_lab, d_sp, n_sites, d_sz, d_mg, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(sizes, d_sz, 8 * (size_t)n_clusters, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(mags, d_mg, 8 * (size_t)n_clusters, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  a.release();
+  if (e != hipSuccess) return set_err(ctx, RBIM_E_HIP, "cluster statistics failed: %s", hipGetErrorString(e));
+  return RBIM_OK;
+}
+
+int rbim_energy_magnetization(rbim_ctx* ctx, int32_t L, const int8_t* spins, const int8_t* Jh,
+                              const int8_t* Jv, int64_t* energy, int64_t* magnetization) {
+  if (!ctx || !spins || !Jh || !Jv) return set_err(ctx, RBIM_E_INVAL, "NULL argument");
+  if (!valid_L(L)) return set_err(ctx, RBIM_E_INVAL, "L must be at least 3 (got %d)", L);
+  const int N = L * L;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  std::vector<uint8_t> jb(N);
+  for (int n = 0; n < N; ++n) jb[n] = (uint8_t)((Jh[n] < 0 ? 1 : 0) | (Jv[n] < 0 ? 2 : 0));
+  DevArena a;
+  void* p;
+  HIP_TRY(ctx, a.alloc(&p, 16 + 2 * (size_t)N));
+  unsigned long long* d_em = (unsigned long long*)p;
+  int8_t* d_sp = (int8_t*)(d_em + 2);
+  uint8_t* d_jb = (uint8_t*)(d_sp + N);
+  long long em[2] = {0, 0};
+  hipError_t e = hipMemsetAsync(d_em, 0, 16, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_sp, spins, N, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_jb, jb.data(), N, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = launch_energy_mag(d_sp, d_jb, L, d_em, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(em, d_em, 16, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  a.release();
+  if (e != hipSuccess) return set_err(ctx, RBIM_E_HIP, "energy failed: %s", hipGetErrorString(e));
+  if (energy) *energy = em[0];
+  if (magnetization) *magnetization = em[1];
+  return RBIM_OK;
+}
+
+// ------------------------------------------------------------------ batch
+static int check_desc(rbim_ctx* ctx, const rbim_batch_desc* d) {
+  if (!d) return set_err(ctx, RBIM_E_INVAL, "desc is NULL");
+  if (d->n_lattices <= 0) return set_err(ctx, RBIM_E_INVAL, "n_lattices must be positive");
+  if (!d->L || !d->Jh || !d->Jv || !d->thr31 || !d->seed || !d->lattice_id)
+    return set_err(ctx, RBIM_E_INVAL, "NULL array in batch description");
+  if (d->n_bins < 1) return set_err(ctx, RBIM_E_INVAL, "n_bins must be positive, got %d", d->n_bins);
+  if (d->sweeps_per_bin < 1)
+    return set_err(ctx, RBIM_E_INVAL, "sweeps_per_bin must be at least 1");
+  if (d->n_therm < 0) return set_err(ctx, RBIM_E_INVAL, "n_therm must be non-negative");
+  if (d->n_therm + (int64_t)d->n_bins * d->sweeps_per_bin > 0xFFFFFFFFll)
+    return set_err(ctx, RBIM_E_INVAL, "more than 2^32 sweeps per lattice");
+  if (d->engine < RBIM_ENGINE_AUTO || d->engine > RBIM_ENGINE_BATCHED)
+    return set_err(ctx, RBIM_E_INVAL, "unknown engine %d", d->engine);
+  if (d->threads != 0 && d->threads != 64 && d->threads != 128 && d->threads != 256 &&
+      d->threads != 512 && d->threads != 1024)
+    return set_err(ctx, RBIM_E_INVAL, "threads must be 64, 128, 256, 512 or 1024");
+  for (int i = 0; i < d->n_lattices; ++i) {
+    if (!valid_L(d->L[i]))
+      return set_err(ctx, RBIM_E_INVAL, "L must be at least 3 (got %d for lattice %d)", d->L[i], i);
+    if (d->engine == RBIM_ENGINE_BATCHED && d->L[i] > RBIM_LDS_MAX_L)
+      return set_err(ctx, RBIM_E_INVAL, "the LDS engine holds L <= %d (lattice %d has L=%d)",
+                     RBIM_LDS_MAX_L, i, d->L[i]);
+    if (d->thr31[i] > (1u << 31)) return set_err(ctx, RBIM_E_INVAL, "thr31 exceeds 2^31");
+  }
+  return RBIM_OK;
+}
+
+static bool use_lds(const rbim_batch_desc* d, int L) {
+  return d->engine == RBIM_ENGINE_BATCHED || (d->engine == RBIM_ENGINE_AUTO && L <= RBIM_LDS_MAX_L);
+}
+
+int rbim_batch_footprint(const rbim_batch_desc* d, uint64_t* bytes) {
+  int rc = check_desc(nullptr, d);
+  if (rc != RBIM_OK) return rc;
+  std::map<std::pair<int, int>, int> counts;
+  for (int i = 0; i < d->n_lattices; ++i) counts[{d->L[i], use_lds(d, d->L[i]) ? 1 : 0}]++;
+  uint64_t b = 0;
+  for (auto& kv : counts) b += group_bytes(kv.first.first, kv.second, kv.first.second != 0, d->n_bins);
+  if (bytes) *bytes = b;
+  return RBIM_OK;
+}
+
+int rbim_batch_create(rbim_ctx* ctx, const rbim_batch_desc* d, rbim_batch** out) {
+  if (!ctx || !out) return set_err(ctx, RBIM_E_INVAL, "NULL argument");
+  int rc = check_desc(ctx, d);
+  if (rc != RBIM_OK) return rc;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  uint64_t need = 0;
+  rbim_batch_footprint(d, &need);
+  size_t fr = 0, tot = 0;
+  HIP_TRY(ctx, hipMemGetInfo(&fr, &tot));
+  if (need + (64ull << 20) > fr)
+    return set_err(ctx, RBIM_E_ADMIT, "batch needs %llu bytes, %llu free", (unsigned long long)need,
+                   (unsigned long long)fr);
+  rbim_batch* b = new rbim_batch();
+  b->ctx = ctx;
+  b->n_lat = d->n_lattices;
+  b->n_therm = d->n_therm;
+  b->spb = d->sweeps_per_bin;
+  b->n_bins = d->n_bins;
+  b->total = d->n_therm + (int64_t)d->n_bins * d->sweeps_per_bin;
+  b->off.resize(d->n_lattices + 1);
+  b->Ls.assign(d->L, d->L + d->n_lattices);
+  b->off[0] = 0;
+  for (int i = 0; i < d->n_lattices; ++i) b->off[i + 1] = b->off[i] + (int64_t)d->L[i] * d->L[i];
+  hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+  if (e != hipSuccess) {
+    rbim_batch_destroy(b);
+    return set_err(ctx, RBIM_E_HIP, "stream/event creation failed: %s", hipGetErrorString(e));
+  }
+  std::map<std::pair<int, int>, int> gid;
+  for (int i = 0; i < d->n_lattices; ++i) {
+    const int L = d->L[i];
+    const int lds = use_lds(d, L) ? 1 : 0;
+    auto key = std::make_pair(L, lds);
+    if (!gid.count(key)) {
+      gid[key] = (int)b->groups.size();
+      b->groups.emplace_back();
+      Group& g = b->groups.back();
+      g.L = L;
+      g.N = L * L;
+      g.lds = lds != 0;
+      g.threads = d->threads ? d->threads : default_threads(L);
+      g.per_launch = d->sweeps_per_launch > 0 ? d->sweeps_per_launch
+                                              : (g.lds ? default_lds_sweeps(L) : 16);
+    }
+    b->groups[gid[key]].idx.push_back(i);
+  }
+  for (Group& g : b->groups) {
+    g.n_lat = (int)g.idx.size();
+    std::vector<uint8_t> jb((size_t)g.N * g.n_lat);
+    std::vector<uint32_t> thr(g.n_lat), lid(g.n_lat);
+    std::vector<uint64_t> seed(g.n_lat);
+    std::vector<int8_t> sp(d->spins0 ? (size_t)g.N * g.n_lat : 0);
+    for (int k = 0; k < g.n_lat; ++k) {
+      const int i = g.idx[k];
+      const int64_t o = b->off[i];
+      for (int n = 0; n < g.N; ++n)
+        jb[(size_t)k * g.N + n] = (uint8_t)((d->Jh[o + n] < 0 ? 1 : 0) | (d->Jv[o + n] < 0 ? 2 : 0));
+      if (d->spins0) std::memcpy(&sp[(size_t)k * g.N], d->spins0 + o, g.N);
+      thr[k] = d->thr31[i];
+      seed[k] = d->seed[i];
+      lid[k] = d->lattice_id[i];
+    }
+    rc = group_alloc(ctx, g, jb, thr, seed, lid, d->spins0 ? sp.data() : nullptr, d->start,
+                     d->n_therm, d->sweeps_per_bin, d->n_bins, 0);
+    if (rc != RBIM_OK) {
+      rbim_batch_destroy(b);
+      return rc;
+    }
+  }
+  *out = b;
+  return RBIM_OK;
+}
+
+int rbim_batch_run(rbim_batch* b, int64_t n_sweeps) {
+  if (!b) return set_err(nullptr, RBIM_E_INVAL, "batch is NULL");
+  rbim_ctx* ctx = b->ctx;
+  if (n_sweeps < 0) return set_err(ctx, RBIM_E_INVAL, "n_sweeps must be non-negative");
+  if (b->issued + n_sweeps > b->total)
+    return set_err(ctx, RBIM_E_STATE, "run of %lld sweeps passes the end (%lld of %lld issued)",
+                   (long long)n_sweeps, (long long)b->issued, (long long)b->total);
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  for (Group& g : b->groups) {
+    int rc = group_run(ctx, g, n_sweeps, b->stream, &b->launches);
+    if (rc != RBIM_OK) return rc;
+  }
+  b->issued += n_sweeps;
+  return RBIM_OK;
+}
+
+int rbim_batch_run_timed(rbim_batch* b, int64_t n_sweeps, double* elapsed_ms) {
+  if (!b) return set_err(nullptr, RBIM_E_INVAL, "batch is NULL");
+  rbim_ctx* ctx = b->ctx;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipEventRecord(b->ev0, b->stream));
+  int rc = rbim_batch_run(b, n_sweeps);
+  if (rc != RBIM_OK) return rc;
+  HIP_TRY(ctx, hipEventRecord(b->ev1, b->stream));
+  HIP_TRY(ctx, hipEventSynchronize(b->ev1));
+  float ms = 0.f;
+  HIP_TRY(ctx, hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  if (elapsed_ms) *elapsed_ms = ms;
+  return RBIM_OK;
+}
+
+int rbim_batch_query(rbim_batch* b, int32_t* done) {
+  if (!b || !done) return set_err(nullptr, RBIM_E_INVAL, "NULL argument");
+  hipError_t e = hipStreamQuery(b->stream);
+  if (e == hipSuccess) { *done = 1; return RBIM_OK; }
+  if (e == hipErrorNotReady) { *done = 0; return RBIM_OK; }
+  return set_err(b->ctx, RBIM_E_HIP, "stream query failed: %s", hipGetErrorString(e));
+}
+
+int rbim_batch_sync(rbim_batch* b) {
+  if (!b) return set_err(nullptr, RBIM_E_INVAL, "batch is NULL");
+  HIP_TRY(b->ctx, hipStreamSynchronize(b->stream));
+  return RBIM_OK;
+}
+
+int rbim_batch_moments(rbim_batch* b, uint64_t* out) {
+  if (!b || !out) return set_err(nullptr, RBIM_E_INVAL, "NULL argument");
+  rbim_ctx* ctx = b->ctx;
+  HIP_TRY(ctx, hipStreamSynchronize(b->stream));
+  const size_t per = (size_t)b->n_bins * RBIM_ACC_SLOTS;
+  for (Group& g : b->groups) {
+    std::vector<uint64_t> h(per * g.n_lat);
+    HIP_TRY(ctx, hipMemcpy(h.data(), g.set.acc, h.size() * 8, hipMemcpyDeviceToHost));
+    for (int k = 0; k < g.n_lat; ++k) std::memcpy(out + per * g.idx[k], &h[per * k], per * 8);
+  }
+  return RBIM_OK;
+}
+
+int rbim_batch_spins(rbim_batch* b, int8_t* out) {
+  if (!b || !out) return set_err(nullptr, RBIM_E_INVAL, "NULL argument");
+  rbim_ctx* ctx = b->ctx;
+  HIP_TRY(ctx, hipStreamSynchronize(b->stream));
+  for (Group& g : b->groups) {
+    std::vector<int8_t> h((size_t)g.N * g.n_lat);
+    HIP_TRY(ctx, hipMemcpy(h.data(), g.set.spins, h.size(), hipMemcpyDeviceToHost));
+    for (int k = 0; k < g.n_lat; ++k) std::memcpy(out + b->off[g.idx[k]], &h[(size_t)g.N * k], g.N);
+  }
+  return RBIM_OK;
+}
+
+int rbim_batch_info_get(rbim_batch* b, rbim_batch_info* o) {
+  if (!b || !o) return set_err(nullptr, RBIM_E_INVAL, "NULL argument");
+  std::memset(o, 0, sizeof *o);
+  o->n_groups = (int32_t)b->groups.size();
+  for (Group& g : b->groups) {
+    if (g.lds) ++o->lds_groups; else ++o->tiled_groups;
+    o->device_bytes += (int64_t)g.arena.bytes;
+  }
+  if (!b->groups.empty()) {
+    o->threads = b->groups[0].threads;
+    o->sweeps_per_launch = b->groups[0].per_launch;
+  }
+  o->sweeps_done = b->issued;
+  o->total_sweeps = b->total;
+  o->kernel_launches = b->launches;
+  return RBIM_OK;
+}
+
+int rbim_batch_destroy(rbim_batch* b) {
+  if (!b) return RBIM_OK;
+  (void)hipSetDevice(b->ctx->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (Group& g : b->groups) group_free(g);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return RBIM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,95 @@
+// Internal declarations shared by the gfx950 kernels and the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/rbim_hip.h"
+#include "philox.h"
+
+// Largest lattice the LDS-resident kernel holds: 14-bit parent field.
+#define RBIM_LDS_MAX_L 128
+#define RBIM_LDS_MAX_N (RBIM_LDS_MAX_L * RBIM_LDS_MAX_L)
+// Tiled path: 32 x 32 site tiles, 256 threads per tile workgroup.
+#define RBIM_TILE 32
+#define RBIM_TILE_THREADS 256
+
+// Per-lattice state common to both engines (device pointers).
+struct LatticeSet {
+  int32_t L, N, n_lat;
+  uint32_t magic;           // ceil(2^32 / L): x = umulhi(n, magic) for n < 2^20
+  int8_t* spins;            // [n_lat][N] +-1
+  const uint8_t* jbits;     // [n_lat][N] bit0: J_h < 0, bit1: J_v < 0
+  const uint32_t* thr31;    // [n_lat]
+  const uint64_t* seed;     // [n_lat]
+  const uint32_t* lat_id;   // [n_lat]
+  int64_t* sweep_ctr;       // [n_lat] sweeps completed
+  uint64_t* acc;            // [n_lat][n_bins][RBIM_ACC_SLOTS]
+  rbim_sweep_stats* last;   // [n_lat] stats of the last measured sweep
+  int64_t n_therm, spb, n_bins, total;
+  // given-bits mode (one-lattice parity entry points); NULL in production
+  const uint8_t* given_occ; // [N] bit0 horizontal, bit1 vertical
+  const uint8_t* given_flip;// [N]
+  int32_t* canon_out;       // [N] min-site label of each site
+};
+
+// Tiled-path scratch (device pointers), [n_lat][N] unless noted.
+struct TiledScratch {
+  uint8_t* occ;        // bit0 occ_h, bit1 occ_v, bit2 flip
+  uint64_t* uf;        // parent (low 32) | wx:16 | wy:16 (high 32)
+  uint64_t* lst;       // local-cluster (size << 32 | n_plus before flip) at local roots
+  uint64_t* st;        // global-cluster (size << 32 | n_plus after flip) at roots
+  int64_t* part;       // [n_lat][n_tiles][5] partial reductions
+  uint32_t* wrapf;     // [n_lat] wrap flags
+  int32_t tiles_x, tiles_y;
+};
+
+// Launchers (stream-ordered, no host synchronisation).
+hipError_t launch_init_spins(const LatticeSet& s, int start, hipStream_t st);
+hipError_t launch_lds_sweeps(const LatticeSet& s, int threads, int sweeps, hipStream_t st);
+size_t lds_bytes(int N);
+hipError_t launch_tiled_sweep(const LatticeSet& s, const TiledScratch& t, hipStream_t st);
+hipError_t launch_label_rank(const int32_t* canon, int32_t N, int32_t* labels, int32_t* rank_tmp,
+                             int32_t* n_clusters, hipStream_t st);
+hipError_t launch_cluster_stats(const int32_t* labels, const int8_t* spins, int64_t n,
+                                unsigned long long* sizes, unsigned long long* mags,
+                                hipStream_t st);
+hipError_t launch_energy_mag(const int8_t* spins, const uint8_t* jbits, int32_t L,
+                             unsigned long long* em, hipStream_t st);
+int tiled_kernels_per_sweep();
+
+// ---------------------------------------------------------------- helpers
+__device__ __forceinline__ uint32_t site_x(uint32_t n, uint32_t magic) {
+  return __umulhi(n, magic);
+}
+
+// Accumulate one measured sweep into a 20-slot bin (single thread).
+__device__ __forceinline__ void acc_add_u128(uint64_t* lo, uint64_t* hi, uint64_t vlo,
+                                             uint64_t vhi) {
+  const uint64_t nlo = *lo + vlo;
+  *hi += vhi + (nlo < vlo ? 1ull : 0ull);
+  *lo = nlo;
+}
+
+__device__ __forceinline__ void acc_sweep(uint64_t* a, const rbim_sweep_stats& s) {
+  const int64_t E = s.energy, M = s.magnetization;
+  const uint64_t aE = (uint64_t)(E < 0 ? -E : E);
+  const uint64_t aM = (uint64_t)(M < 0 ? -M : M);
+  a[RBIM_ACC_E] = (uint64_t)((int64_t)a[RBIM_ACC_E] + E);
+  acc_add_u128(&a[RBIM_ACC_E2_LO], &a[RBIM_ACC_E2_HI], aE * aE, __umul64hi(aE, aE));
+  a[RBIM_ACC_M] = (uint64_t)((int64_t)a[RBIM_ACC_M] + M);
+  a[RBIM_ACC_ABSM] += aM;
+  const uint64_t m2 = aM * aM;  // |M| <= 2^20: M^2 < 2^40 exact
+  acc_add_u128(&a[RBIM_ACC_M2_LO], &a[RBIM_ACC_M2_HI], m2, 0ull);
+  acc_add_u128(&a[RBIM_ACC_M4_LO], &a[RBIM_ACC_M4_HI], m2 * m2, __umul64hi(m2, m2));
+  a[RBIM_ACC_SMAX] += (uint64_t)s.largest_size;
+  a[RBIM_ACC_ABSMC] += (uint64_t)s.largest_abs_mag;
+  a[RBIM_ACC_S2] += (uint64_t)s.second_size;
+  const double ratio =
+      s.largest_size > 0 ? (double)s.largest_abs_mag / (double)s.largest_size : 0.0;
+  a[RBIM_ACC_RATIO] = (uint64_t)__double_as_longlong(__longlong_as_double((long long)a[RBIM_ACC_RATIO]) + ratio);
+  a[RBIM_ACC_NCL] += (uint64_t)s.n_clusters;
+  a[RBIM_ACC_WX] += (uint64_t)(s.wrap_x != 0);
+  a[RBIM_ACC_WY] += (uint64_t)(s.wrap_y != 0);
+  a[RBIM_ACC_WANY] += (uint64_t)((s.wrap_x | s.wrap_y) != 0);
+  a[RBIM_ACC_COUNT] += 1ull;
+}

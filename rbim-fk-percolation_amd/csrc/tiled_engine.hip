@@ -1,0 +1,479 @@
+// Tiled Swendsen-Wang engine for any L >= 3 on gfx950: the plain path (one
+// lattice, sweep by sweep) and the large-lattice path (L > 128, many lattices
+// per launch).  A sweep is five stream-ordered kernels:
+//
+//   K1 tile      32x32 tile per workgroup: Philox activation of every site,
+//                occupation/flip bits to HBM, union-find of the bonds inside
+//                the tile in LDS (no winding: a tile never crosses the periodic
+//                boundary, so every path inside it has winding 0), each site
+//                pointed at its tile-local root (= min site of its tile-local
+//                cluster), local sizes / n_plus recorded at local roots.
+//   K2 merge     one thread per tile-boundary bond (including the periodic
+//                ones): union in HBM with 64-bit CAS on (parent | wx | wy),
+//                larger root hooked under smaller; a bond that finds both ends
+//                under one root is a non-tree bond and its loop winding decides
+//                wrap_x / wrap_y (_unionfind.py:101-124).  Bonds inside a tile
+//                close only windingless loops and need no test.
+//   K3 flip      per site: global root, s ^= flipbit(root); each local root
+//                adds its local cluster's (size, n_plus after flip) to its
+//                global root -- one atomic per local cluster, not per site.
+//   K4 measure   per tile: E, M, n_clusters, two largest clusters (64-bit keys
+//                size << 32 | ~root) -> per-tile partials.
+//   K5 finalize  per lattice: fold the partials, fold the sweep into its bin's
+//                moment sums (simulation.py:212-236), advance the sweep counter.
+//
+// Every kernel reads the lattice's own sweep counter from HBM, so a captured
+// graph of sweeps replays without host arguments changing.
+#include "engine.h"
+
+namespace {
+
+constexpr int T = RBIM_TILE;
+constexpr int NTH = RBIM_TILE_THREADS;
+constexpr int PARTS = 5;
+
+__device__ __forceinline__ uint64_t gpack(uint32_t p, int wx, int wy) {
+  return (uint64_t)p | ((uint64_t)(uint16_t)(int16_t)wx << 32) |
+         ((uint64_t)(uint16_t)(int16_t)wy << 48);
+}
+__device__ __forceinline__ uint32_t gpar(uint64_t v) { return (uint32_t)v; }
+__device__ __forceinline__ int gwx(uint64_t v) { return (int)(int16_t)(uint16_t)(v >> 32); }
+__device__ __forceinline__ int gwy(uint64_t v) { return (int)(int16_t)(uint16_t)(v >> 48); }
+
+__device__ __forceinline__ uint64_t gload(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void gstore(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Root and tree-path winding in the HBM forest, with path halving.
+__device__ uint32_t gfind(uint64_t* uf, uint32_t x, int& wx, int& wy) {
+  wx = 0;
+  wy = 0;
+  uint64_t v = gload(&uf[x]);
+  while (true) {
+    const uint32_t p = gpar(v);
+    if (p == x) return x;
+    const uint64_t vp = gload(&uf[p]);
+    const uint32_t gp = gpar(vp);
+    if (gp == p) {
+      wx += gwx(v);
+      wy += gwy(v);
+      return p;
+    }
+    const int hx = gwx(v) + gwx(vp), hy = gwy(v) + gwy(vp);
+    gstore(&uf[x], gpack(gp, hx, hy));
+    wx += hx;
+    wy += hy;
+    x = gp;
+    v = gload(&uf[gp]);
+  }
+}
+
+__device__ __forceinline__ bool lattice_active(const LatticeSet& s, int lat) {
+  return s.sweep_ctr[lat] < s.total;
+}
+
+// ----------------------------------------------------------------- K1
+__device__ __forceinline__ uint32_t lfind(uint32_t* par, uint32_t x) {
+  uint32_t p = par[x];
+  while (p != x) {
+    const uint32_t gp = par[p];
+    if (gp == p) return p;
+    par[x] = gp;
+    x = gp;
+    p = par[x];
+  }
+  return x;
+}
+
+__device__ __forceinline__ void lunion(uint32_t* par, uint32_t a, uint32_t b) {
+  while (true) {
+    const uint32_t ra = lfind(par, a), rb = lfind(par, b);
+    if (ra == rb) return;
+    const uint32_t hi = max(ra, rb), lo = min(ra, rb);
+    if (atomicCAS(&par[hi], hi, lo) == hi) return;
+  }
+}
+
+__global__ __launch_bounds__(NTH) void k1_tile(LatticeSet s, TiledScratch t) {
+  __shared__ uint32_t par[T * T];
+  __shared__ uint32_t lcnt[T * T];
+  __shared__ uint8_t bits[T * T];  // bit0 spin, bit1 occ_h, bit2 occ_v
+  const int lat = blockIdx.y;
+  if (!lattice_active(s, lat)) return;
+  const int L = s.L, N = s.N;
+  const int tx = blockIdx.x % t.tiles_x, ty = blockIdx.x / t.tiles_x;
+  const int x0 = tx * T, y0 = ty * T;
+  const int w = min(T, L - x0), h = min(T, L - y0);
+  const size_t base = (size_t)lat * N;
+  const int8_t* sp = s.spins + base;
+  const uint8_t* jb = s.jbits + base;
+  const uint32_t sweep = (uint32_t)s.sweep_ctr[lat];
+  const uint32_t thr = s.thr31[lat];
+  const uint64_t seed = s.seed[lat];
+  const uint32_t lid = s.lat_id[lat];
+  const bool given = s.given_occ != nullptr;
+  const bool even = (L & 1) == 0;
+
+  // activation: thread handles the local pair (lx, 2k), (lx, 2k+1)
+  for (int i = threadIdx.x; i < T * T / 2; i += NTH) {
+    const int lx = i / (T / 2), ly0 = 2 * (i % (T / 2));
+    if (lx >= w || ly0 >= h) continue;
+    const int x = x0 + lx;
+    rbim_u32x4 r = {0u, 0u, 0u, 0u};
+    const uint32_t n0 = (uint32_t)(x * L + y0 + ly0);
+    if (!given && even) r = rbim_sweep_draws(seed, lid, sweep, n0 >> 1);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int ly = ly0 + j;
+      if (ly >= h) break;
+      const int y = y0 + ly;
+      const uint32_t n = (uint32_t)(x * L + y);
+      if (!given && !even) r = rbim_sweep_draws(seed, lid, sweep, n >> 1);
+      const uint32_t nr = (x == L - 1) ? (uint32_t)y : n + L;
+      const uint32_t nu = (y == L - 1) ? n - y : n + 1;
+      const int s0 = sp[n];
+      uint32_t oh, ov, fl;
+      if (given) {
+        const uint32_t o = s.given_occ[n];
+        oh = o & 1u;
+        ov = (o >> 1) & 1u;
+        fl = s.given_flip[n] & 1u;
+      } else {
+        const uint32_t jbn = jb[n];
+        const int jh = (jbn & 1u) ? -1 : 1, jv = (jbn & 2u) ? -1 : 1;
+        const bool odd = (n & 1u) != 0;
+        const uint32_t dh = odd ? r.z : r.x, dv = odd ? r.w : r.y;
+        oh = (jh * s0 * sp[nr] > 0) && ((dh >> 1) < thr) ? 1u : 0u;
+        ov = (jv * s0 * sp[nu] > 0) && ((dv >> 1) < thr) ? 1u : 0u;
+        fl = dh & 1u;
+      }
+      t.occ[base + n] = (uint8_t)(oh | (ov << 1) | (fl << 2));
+      const int li = lx * T + ly;
+      bits[li] = (uint8_t)((s0 > 0 ? 1u : 0u) | (oh << 1) | (ov << 2));
+      par[li] = (uint32_t)li;
+      lcnt[li] = 0u;
+    }
+  }
+  __syncthreads();
+  // local union over bonds with both ends in the tile and no periodic crossing
+  for (int li = threadIdx.x; li < T * T; li += NTH) {
+    const int lx = li / T, ly = li % T;
+    if (lx >= w || ly >= h) continue;
+    const uint32_t b = bits[li];
+    if ((b & 2u) && lx + 1 < w && x0 + lx + 1 < L) lunion(par, li, li + T);
+    if ((b & 4u) && ly + 1 < h && y0 + ly + 1 < L) lunion(par, li, li + 1);
+  }
+  __syncthreads();
+  for (int li = threadIdx.x; li < T * T; li += NTH) {
+    const int lx = li / T, ly = li % T;
+    if (lx >= w || ly >= h) continue;
+    const uint32_t r = lfind(par, li);
+    atomicAdd(&lcnt[r], 0x10000u | (bits[li] & 1u));
+    const uint32_t rg = (uint32_t)((x0 + (int)(r / T)) * L + y0 + (int)(r % T));
+    const uint32_t n = (uint32_t)((x0 + lx) * L + y0 + ly);
+    t.uf[base + n] = gpack(rg, 0, 0);
+  }
+  __syncthreads();
+  for (int li = threadIdx.x; li < T * T; li += NTH) {
+    const int lx = li / T, ly = li % T;
+    if (lx >= w || ly >= h) continue;
+    const uint32_t n = (uint32_t)((x0 + lx) * L + y0 + ly);
+    const uint32_t c = (par[li] == (uint32_t)li) ? lcnt[li] : 0u;
+    t.lst[base + n] = ((uint64_t)(c >> 16) << 32) | (uint64_t)(c & 0xFFFFu);
+    t.st[base + n] = 0ull;
+  }
+}
+
+// ----------------------------------------------------------------- K2
+__global__ void k2_merge(LatticeSet s, TiledScratch t) {
+  const int lat = blockIdx.y;
+  if (!lattice_active(s, lat)) return;
+  const int L = s.L;
+  const int nh = t.tiles_x * L, nv = t.tiles_y * L;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nh + nv) return;
+  int x, y, horiz;
+  if (i < nh) {
+    x = min((i / L) * T + T - 1, L - 1);
+    y = i % L;
+    horiz = 1;
+  } else {
+    const int k = i - nh;
+    y = min((k / L) * T + T - 1, L - 1);
+    x = k % L;
+    horiz = 0;
+  }
+  const size_t base = (size_t)lat * s.N;
+  const uint32_t a = (uint32_t)(x * L + y);
+  const uint32_t o = t.occ[base + a];
+  if (!(o & (horiz ? 1u : 2u))) return;
+  const uint32_t b = horiz ? ((x == L - 1) ? (uint32_t)y : a + L) : ((y == L - 1) ? a - y : a + 1);
+  const int cx = (horiz && x == L - 1) ? 1 : 0;
+  const int cy = (!horiz && y == L - 1) ? 1 : 0;
+  uint64_t* uf = t.uf + base;
+  while (true) {
+    int ax, ay, bx, by;
+    const uint32_t ra = gfind(uf, a, ax, ay);
+    const uint32_t rb = gfind(uf, b, bx, by);
+    if (ra == rb) {
+      const uint32_t wbits = ((cx + bx - ax) != 0 ? 1u : 0u) | ((cy + by - ay) != 0 ? 2u : 0u);
+      if (wbits) atomicOr(&t.wrapf[lat], wbits);
+      return;
+    }
+    if (ra < rb) {
+      if (atomicCAS((unsigned long long*)&uf[rb], (unsigned long long)rb,
+                    (unsigned long long)gpack(ra, ax - bx - cx, ay - by - cy)) == rb)
+        return;
+    } else {
+      if (atomicCAS((unsigned long long*)&uf[ra], (unsigned long long)ra,
+                    (unsigned long long)gpack(rb, bx - ax + cx, by - ay + cy)) == ra)
+        return;
+    }
+  }
+}
+
+// ----------------------------------------------------------------- K3
+__global__ void k3_flip(LatticeSet s, TiledScratch t) {
+  const int lat = blockIdx.y;
+  if (!lattice_active(s, lat)) return;
+  const int N = s.N;
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  const size_t base = (size_t)lat * N;
+  uint64_t* uf = t.uf + base;
+  int wx, wy;
+  const uint32_t r = gfind(uf, (uint32_t)n, wx, wy);
+  const uint32_t fl = (t.occ[base + r] >> 2) & 1u;
+  const int8_t old = s.spins[base + n];
+  if (fl) s.spins[base + n] = (int8_t)-old;
+  const uint64_t lc = t.lst[base + n];
+  if (lc != 0ull) {
+    const uint64_t size = lc >> 32, np = lc & 0xFFFFFFFFull;
+    const uint64_t np_new = fl ? size - np : np;
+    atomicAdd((unsigned long long*)&t.st[base + r], (unsigned long long)((size << 32) | np_new));
+  }
+  if (s.canon_out) s.canon_out[n] = (int32_t)r;
+}
+
+// ----------------------------------------------------------------- K4
+__device__ __forceinline__ void top2_merge64(uint64_t& k1, uint64_t& k2, uint64_t o1, uint64_t o2) {
+  const uint64_t hi = max(k1, o1);
+  const uint64_t lo = max(min(k1, o1), max(k2, o2));
+  k1 = hi;
+  k2 = lo;
+}
+
+__global__ __launch_bounds__(NTH) void k4_measure(LatticeSet s, TiledScratch t) {
+  __shared__ int64_t red[NTH / 64][PARTS];
+  const int lat = blockIdx.y;
+  if (!lattice_active(s, lat)) return;
+  if (s.sweep_ctr[lat] < s.n_therm) return;
+  const int L = s.L, N = s.N;
+  const int tx = blockIdx.x % t.tiles_x, ty = blockIdx.x / t.tiles_x;
+  const int x0 = tx * T, y0 = ty * T;
+  const int w = min(T, L - x0), h = min(T, L - y0);
+  const size_t base = (size_t)lat * N;
+  const int8_t* sp = s.spins + base;
+  const uint8_t* jb = s.jbits + base;
+  int64_t e = 0, m = 0, ncl = 0;
+  uint64_t k1 = 0, k2 = 0;
+  for (int li = threadIdx.x; li < T * T; li += NTH) {
+    const int lx = li / T, ly = li % T;
+    if (lx >= w || ly >= h) continue;
+    const int x = x0 + lx, y = y0 + ly;
+    const uint32_t n = (uint32_t)(x * L + y);
+    const uint32_t nr = (x == L - 1) ? (uint32_t)y : n + L;
+    const uint32_t nu = (y == L - 1) ? n - y : n + 1;
+    const int s0 = sp[n];
+    const uint32_t j = jb[n];
+    e -= (int64_t)((j & 1u) ? -1 : 1) * s0 * sp[nr];
+    e -= (int64_t)((j & 2u) ? -1 : 1) * s0 * sp[nu];
+    m += s0;
+    if (gpar(t.uf[base + n]) == n) {
+      ++ncl;
+      const uint64_t size = t.st[base + n] >> 32;
+      top2_merge64(k1, k2, (size << 32) | (uint64_t)(0xFFFFFFFFu - n), 0ull);
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    e += __shfl_xor(e, off);
+    m += __shfl_xor(m, off);
+    ncl += __shfl_xor(ncl, off);
+    const uint64_t o1 = __shfl_xor(k1, off), o2 = __shfl_xor(k2, off);
+    top2_merge64(k1, k2, o1, o2);
+  }
+  if (lane == 0) {
+    red[wave][0] = e;
+    red[wave][1] = m;
+    red[wave][2] = ncl;
+    red[wave][3] = (int64_t)k1;
+    red[wave][4] = (int64_t)k2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int wv = 1; wv < NTH / 64; ++wv) {
+      e += red[wv][0];
+      m += red[wv][1];
+      ncl += red[wv][2];
+      top2_merge64(k1, k2, (uint64_t)red[wv][3], (uint64_t)red[wv][4]);
+    }
+    int64_t* p = t.part + ((size_t)lat * t.tiles_x * t.tiles_y + blockIdx.x) * PARTS;
+    p[0] = e;
+    p[1] = m;
+    p[2] = ncl;
+    p[3] = (int64_t)k1;
+    p[4] = (int64_t)k2;
+  }
+}
+
+// ----------------------------------------------------------------- K5
+__global__ void k5_finalize(LatticeSet s, TiledScratch t) {
+  const int lat = blockIdx.x;
+  if (threadIdx.x != 0) return;
+  const int64_t g = s.sweep_ctr[lat];
+  if (g >= s.total) return;
+  const uint32_t wf = t.wrapf[lat];
+  t.wrapf[lat] = 0u;
+  if (g >= s.n_therm) {
+    const int nt = t.tiles_x * t.tiles_y;
+    const int64_t* p = t.part + (size_t)lat * nt * PARTS;
+    int64_t e = 0, m = 0, ncl = 0;
+    uint64_t k1 = 0, k2 = 0;
+    for (int i = 0; i < nt; ++i) {
+      e += p[i * PARTS + 0];
+      m += p[i * PARTS + 1];
+      ncl += p[i * PARTS + 2];
+      top2_merge64(k1, k2, (uint64_t)p[i * PARTS + 3], (uint64_t)p[i * PARTS + 4]);
+    }
+    const size_t base = (size_t)lat * s.N;
+    rbim_sweep_stats st;
+    st.energy = e;
+    st.magnetization = m;
+    st.n_clusters = ncl;
+    const uint32_t r1 = 0xFFFFFFFFu - (uint32_t)k1;
+    const uint64_t c1 = t.st[base + r1];
+    const int64_t s1 = (int64_t)(c1 >> 32), mc1 = 2 * (int64_t)(c1 & 0xFFFFFFFFull) - s1;
+    st.largest_size = s1;
+    st.largest_abs_mag = mc1 < 0 ? -mc1 : mc1;
+    if (k2 != 0ull) {
+      const uint32_t r2 = 0xFFFFFFFFu - (uint32_t)k2;
+      const uint64_t c2 = t.st[base + r2];
+      const int64_t s2 = (int64_t)(c2 >> 32), mc2 = 2 * (int64_t)(c2 & 0xFFFFFFFFull) - s2;
+      st.second_size = s2;
+      st.second_abs_mag = mc2 < 0 ? -mc2 : mc2;
+    } else {
+      st.second_size = 0;
+      st.second_abs_mag = 0;
+    }
+    st.wrap_x = (int32_t)(wf & 1u);
+    st.wrap_y = (int32_t)((wf >> 1) & 1u);
+    const int64_t bin = (g - s.n_therm) / s.spb;
+    acc_sweep(s.acc + ((size_t)lat * s.n_bins + bin) * RBIM_ACC_SLOTS, st);
+    if (s.last) s.last[lat] = st;
+  }
+  s.sweep_ctr[lat] = g + 1;
+}
+
+// ------------------------------------------------------ one-lattice helpers
+// labels[n] = rank of canon[n] among roots (roots are the sites with
+// canon[n] == n), i.e. the reference's min-site-order numbering.
+__global__ __launch_bounds__(1024) void label_rank_kernel(const int32_t* canon, int32_t N,
+                                                          int32_t* labels, int32_t* rank,
+                                                          int32_t* n_clusters) {
+  __shared__ int32_t sums[1024];
+  const int tid = threadIdx.x;
+  const int chunk = (N + 1023) / 1024;
+  const int lo = min(N, tid * chunk), hi = min(N, lo + chunk);
+  int c = 0;
+  for (int n = lo; n < hi; ++n) c += canon[n] == n;
+  sums[tid] = c;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int v = tid >= off ? sums[tid - off] : 0;
+    __syncthreads();
+    sums[tid] += v;
+    __syncthreads();
+  }
+  int r = sums[tid] - c;
+  for (int n = lo; n < hi; ++n)
+    if (canon[n] == n) rank[n] = r++;
+  if (tid == 1023) *n_clusters = sums[1023];
+  __syncthreads();
+  __threadfence_block();
+  for (int n = tid; n < N; n += 1024) labels[n] = rank[canon[n]];
+}
+
+__global__ void cluster_stats_kernel(const int32_t* labels, const int8_t* spins, int64_t n,
+                                     unsigned long long* sizes, unsigned long long* mags) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  atomicAdd(&sizes[labels[i]], 1ull);
+  atomicAdd(&mags[labels[i]], (unsigned long long)(long long)spins[i]);  // two's complement
+}
+
+__global__ void energy_mag_kernel(const int8_t* spins, const uint8_t* jb, int32_t L,
+                                  unsigned long long* em) {
+  const int N = L * L;
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  long long e = 0, m = 0;
+  if (n < N) {
+    const int x = n / L, y = n % L;
+    const int nr = (x == L - 1) ? y : n + L;
+    const int nu = (y == L - 1) ? n - y : n + 1;
+    const int s0 = spins[n];
+    e -= (long long)((jb[n] & 1u) ? -1 : 1) * s0 * spins[nr];
+    e -= (long long)((jb[n] & 2u) ? -1 : 1) * s0 * spins[nu];
+    m = s0;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    e += __shfl_xor(e, off);
+    m += __shfl_xor(m, off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&em[0], (unsigned long long)e);
+    atomicAdd(&em[1], (unsigned long long)m);
+  }
+}
+
+}  // namespace
+
+int tiled_kernels_per_sweep() { return 5; }
+
+hipError_t launch_tiled_sweep(const LatticeSet& s, const TiledScratch& t, hipStream_t st) {
+  const int ntiles = t.tiles_x * t.tiles_y;
+  hipLaunchKernelGGL(k1_tile, dim3(ntiles, s.n_lat), dim3(NTH), 0, st, s, t);
+  const int nb = (t.tiles_x + t.tiles_y) * s.L;
+  hipLaunchKernelGGL(k2_merge, dim3((nb + 255) / 256, s.n_lat), dim3(256), 0, st, s, t);
+  hipLaunchKernelGGL(k3_flip, dim3((s.N + 255) / 256, s.n_lat), dim3(256), 0, st, s, t);
+  hipLaunchKernelGGL(k4_measure, dim3(ntiles, s.n_lat), dim3(NTH), 0, st, s, t);
+  hipLaunchKernelGGL(k5_finalize, dim3(s.n_lat), dim3(64), 0, st, s, t);
+  return hipGetLastError();
+}
+
+hipError_t launch_label_rank(const int32_t* canon, int32_t N, int32_t* labels, int32_t* rank_tmp,
+                             int32_t* n_clusters, hipStream_t st) {
+  hipLaunchKernelGGL(label_rank_kernel, dim3(1), dim3(1024), 0, st, canon, N, labels, rank_tmp,
+                     n_clusters);
+  return hipGetLastError();
+}
+
+hipError_t launch_cluster_stats(const int32_t* labels, const int8_t* spins, int64_t n,
+                                unsigned long long* sizes, unsigned long long* mags, hipStream_t st) {
+  hipLaunchKernelGGL(cluster_stats_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                     labels, spins, n, sizes, mags);
+  return hipGetLastError();
+}
+
+hipError_t launch_energy_mag(const int8_t* spins, const uint8_t* jbits, int32_t L,
+                             unsigned long long* em, hipStream_t st) {
+  const int N = L * L;
+  hipLaunchKernelGGL(energy_mag_kernel, dim3((N + 255) / 256), dim3(256), 0, st, spins, jbits, L,
+                     em);
+  return hipGetLastError();
+}

@@ -1,0 +1,248 @@
+"""Bit-exact parity of the HIP engine with the oracle, through the C ABI.
+
+Both engines are checked: ENGINE_PLAIN (tiled kernels, any L) and
+ENGINE_BATCHED (LDS-resident kernel, L <= 128).  Given identical spins,
+couplings, occupation and flip bits, labels (reference numbering and min-site
+canonical form), n_clusters, wrap_x / wrap_y, sizes, M_C, E and M must be
+identical; full trajectories (spins and the 20-slot moment sums, including the
+FP64 |M_C|/S_max sum) must be identical sweep for sweep.
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import random_disorder
+from samples import column_case, diagonal_loop, double_winding, sample_configurations
+
+pytestmark = pytest.mark.gpu
+
+PLAIN, BATCHED = 1, 2
+
+
+@pytest.fixture(scope="module")
+def hip():
+    from rbim import _hip
+
+    _hip.context()
+    return _hip
+
+
+def gpu_label(hip, active, L, engine):
+    act = np.ascontiguousarray(active, dtype=np.uint8)
+    labels = np.empty(L * L, np.int32)
+    canon = np.empty(L * L, np.int32)
+    n = ctypes.c_int32()
+    wraps = np.zeros(2, np.int32)
+    ctx = hip.context()
+    hip.check(hip.lib().rbim_label_and_wrap(ctx, L, hip.ptr(act), engine, hip.ptr(labels),
+                                             hip.ptr(canon), ctypes.byref(n), hip.ptr(wraps)), ctx)
+    return n.value, labels, canon, bool(wraps[0]), bool(wraps[1])
+
+
+def gpu_sweep_given_bits(hip, s, engine):
+    L = s["L"]
+    spins = s["spins"].copy()
+    act = np.ascontiguousarray(s["active"], np.uint8)
+    flips = np.ascontiguousarray(s["flips"], np.uint8)
+    labels = np.empty(L * L, np.int32)
+    st = hip.SweepStats()
+    ctx = hip.context()
+    hip.check(hip.lib().rbim_sweep_given_bits(ctx, L, hip.ptr(spins), hip.ptr(s["J_h"]),
+                                               hip.ptr(s["J_v"]), hip.ptr(act), hip.ptr(flips),
+                                               engine, hip.ptr(labels), ctypes.byref(st)), ctx)
+    stats = {name: getattr(st, name) for name, _ in hip.SweepStats._fields_}
+    stats["wrap_x"] = bool(stats["wrap_x"])
+    stats["wrap_y"] = bool(stats["wrap_y"])
+    return spins, stats, labels
+
+
+SIZES = (3, 4, 5, 6, 8, 10, 16, 24, 32, 64)
+
+
+@pytest.mark.parametrize("engine", [PLAIN, BATCHED])
+def test_label_and_wrap_bit_exact(hip, coracle, engine):
+    checked = wrapped = 0
+    for s in sample_configurations(coracle, SIZES, (0.0, 0.1, 0.5), (1.0, 2.27, 4.0), 12, seed=2024):
+        L = s["L"]
+        n_ref, lab_ref, wx_ref, wy_ref = coracle.c_label_and_wrap(s["active"], L)
+        n, labels, canon, wx, wy = gpu_label(hip, s["active"], L, engine)
+        case = f"L={L} p={s['p']} T={s['T']}"
+        assert n == n_ref, case
+        assert np.array_equal(labels, lab_ref), case
+        assert np.array_equal(canon, coracle.canonical(lab_ref)), case
+        assert (wx, wy) == (wx_ref, wy_ref), case
+        checked += 1
+        wrapped += int(wx_ref or wy_ref)
+    assert checked >= 1000 and wrapped > 100
+
+
+@pytest.mark.parametrize("engine", [PLAIN, BATCHED])
+def test_label_known_answers(hip, engine):
+    L = 8
+    none = np.zeros(2 * L * L, bool)
+    assert gpu_label(hip, none, L, engine)[0] == L * L
+    assert gpu_label(hip, none, L, engine)[3:] == (False, False)
+    n, _, canon, wx, wy = gpu_label(hip, ~none, L, engine)
+    assert n == 1 and wx and wy and np.all(canon == 0)
+    open_col, closed_col = column_case(6)
+    assert gpu_label(hip, open_col, 6, engine)[3:] == (False, False)
+    assert gpu_label(hip, closed_col, 6, engine)[3:] == (False, True)
+    for L in (3, 4, 7, 40):
+        assert gpu_label(hip, diagonal_loop(L), L, engine)[3:] == (True, True)
+    assert gpu_label(hip, double_winding(6), 6, engine)[3:] == (False, True)
+
+
+def test_label_and_wrap_large_lattices(hip, coracle):
+    """Tiled path with several tiles per side, partial edge tiles and odd L."""
+    checked = 0
+    for s in sample_configurations(coracle, (33, 47, 100, 129, 256), (0.0, 0.12), (1.5, 2.27, 3.0), 2,
+                                   seed=77):
+        L = s["L"]
+        n_ref, lab_ref, wx_ref, wy_ref = coracle.c_label_and_wrap(s["active"], L)
+        n, labels, canon, wx, wy = gpu_label(hip, s["active"], L, PLAIN)
+        assert (n, wx, wy) == (n_ref, wx_ref, wy_ref), f"L={L}"
+        assert np.array_equal(labels, lab_ref), f"L={L}"
+        checked += 1
+    assert checked == 60
+
+
+def test_activation_bit_exact(hip, coracle):
+    rng = np.random.default_rng(31)
+    for L in (3, 7, 16, 33, 64, 130):
+        J_h, J_v = random_disorder(rng, L, 0.2)
+        spins = np.where(rng.random(L * L) < 0.5, -1, 1).astype(np.int8)
+        for beta, key, lid, sweep in [(0.0, 1, 0, 0), (0.45, 2**63 + 5, 9, 77), (50.0, 3, 2, 5)]:
+            thr = coracle.threshold31(beta)
+            a_ref, f_ref = coracle.c_activate(spins, J_h, J_v, thr, key, lid, sweep)
+            active = np.empty(2 * L * L, np.uint8)
+            flips = np.empty(L * L, np.uint8)
+            ctx = hip.context()
+            hip.check(hip.lib().rbim_activate(ctx, L, hip.ptr(spins), hip.ptr(J_h), hip.ptr(J_v),
+                                               thr, key, lid, sweep, hip.ptr(active),
+                                               hip.ptr(flips)), ctx)
+            assert np.array_equal(active.astype(bool), a_ref), f"L={L} beta={beta}"
+            assert np.array_equal(flips, f_ref), f"L={L} beta={beta}"
+
+
+@pytest.mark.parametrize("engine", [PLAIN, BATCHED])
+def test_sweep_given_bits_bit_exact(hip, coracle, engine):
+    n = 0
+    for s in sample_configurations(coracle, (3, 5, 8, 16, 32, 48, 64), (0.0, 0.1), (1.2, 2.27, 3.5), 3,
+                                   seed=99):
+        sp_ref, st_ref, lab_ref = coracle.c_sweep_given_bits(s["spins"], s["J_h"], s["J_v"],
+                                                              s["active"], s["flips"])
+        sp, st, labels = gpu_sweep_given_bits(hip, s, engine)
+        case = f"L={s['L']} p={s['p']} T={s['T']}"
+        assert np.array_equal(sp, sp_ref), case
+        assert np.array_equal(labels, lab_ref), case
+        assert st == st_ref, case
+        n += 1
+    assert n == 126
+
+
+def _oracle_trajectory(coracle, spec, n_sweeps, n_therm, spb, n_bins, start="hot"):
+    from rbim.clusters import activation_threshold
+
+    s0 = coracle.c_init_spins(spec.L, spec.seed, spec.lattice_id, start)
+    return coracle.c_run(spec.L, s0, spec.J_h, spec.J_v, activation_threshold(spec.beta), spec.seed,
+                         spec.lattice_id, 0, n_sweeps, n_therm, spb, n_bins)
+
+
+def _specs(Ls, seed=0, p=0.1, temps=(1.4, 2.27, 3.2)):
+    from rbim.engine import LatticeSpec
+
+    rng = np.random.default_rng(seed)
+    specs = []
+    k = 0
+    for L in Ls:
+        J_h, J_v = random_disorder(rng, L, p)
+        for T in temps:
+            specs.append(LatticeSpec(L=L, J_h=J_h, J_v=J_v, beta=1.0 / T,
+                                     seed=0x243F6A8885A308D3 + 17 * L, lattice_id=k))
+            k += 1
+    return specs
+
+
+@pytest.mark.parametrize("engine", [PLAIN, BATCHED])
+def test_trajectories_bit_exact(hip, coracle, engine):
+    from rbim.engine import Batch
+
+    specs = _specs((3, 5, 16, 17, 32, 64))
+    n_therm, spb, n_bins = 10, 9, 5
+    total = n_therm + spb * n_bins
+    with Batch(specs, n_therm=n_therm, n_bins=n_bins, sweeps_per_bin=spb, engine=engine,
+               sweeps_per_launch=7) as batch:
+        batch.run(13)  # split runs: bins and graphs straddle calls
+        batch.run(total - 13)
+        acc = batch.moments()
+        spins = batch.spins()
+    for i, spec in enumerate(specs):
+        s_ref, acc_ref = _oracle_trajectory(coracle, spec, total, n_therm, spb, n_bins)
+        assert np.array_equal(spins[i], s_ref), f"lattice {i} L={spec.L}"
+        assert np.array_equal(acc[i], acc_ref), f"lattice {i} L={spec.L}"
+
+
+def test_lds_kernel_l128_and_cold_start(hip, coracle):
+    from rbim.engine import Batch
+
+    specs = _specs((100, 128), temps=(2.0, 2.6))
+    n_therm, spb, n_bins = 4, 5, 2
+    total = n_therm + spb * n_bins
+    with Batch(specs, n_therm=n_therm, n_bins=n_bins, sweeps_per_bin=spb, engine=BATCHED,
+               start="cold") as batch:
+        batch.run()
+        acc = batch.moments()
+        spins = batch.spins()
+    for i, spec in enumerate(specs):
+        s_ref, acc_ref = _oracle_trajectory(coracle, spec, total, n_therm, spb, n_bins, start="cold")
+        assert np.array_equal(spins[i], s_ref) and np.array_equal(acc[i], acc_ref), f"L={spec.L}"
+
+
+def test_tiled_large_lattices_bit_exact(hip, coracle):
+    from rbim.engine import Batch
+
+    specs = _specs((160, 256, 512), temps=(2.0, 2.4), seed=4)
+    n_therm, spb, n_bins = 3, 3, 2
+    total = n_therm + spb * n_bins
+    with Batch(specs, n_therm=n_therm, n_bins=n_bins, sweeps_per_bin=spb, engine=PLAIN,
+               sweeps_per_launch=4) as batch:
+        batch.run()
+        acc = batch.moments()
+        spins = batch.spins()
+    for i, spec in enumerate(specs):
+        s_ref, acc_ref = _oracle_trajectory(coracle, spec, total, n_therm, spb, n_bins)
+        assert np.array_equal(spins[i], s_ref) and np.array_equal(acc[i], acc_ref), f"L={spec.L}"
+
+
+def test_batched_equals_plain_and_is_launch_shape_invariant(hip):
+    from rbim.engine import Batch
+
+    specs = _specs((16, 32), seed=8, temps=(1.8, 2.3, 2.9))
+    out = []
+    for engine, threads, per_launch in [(PLAIN, 0, 5), (BATCHED, 64, 3), (BATCHED, 256, 50),
+                                        (BATCHED, 1024, 1000)]:
+        with Batch(specs, n_therm=20, n_bins=4, sweeps_per_bin=10, engine=engine, threads=threads,
+                   sweeps_per_launch=per_launch) as batch:
+            batch.run()
+            out.append((batch.moments(), batch.spins()))
+    for acc, spins in out[1:]:
+        assert np.array_equal(acc, out[0][0])
+        for a, b in zip(spins, out[0][1]):
+            assert np.array_equal(a, b)
+    # a lattice's result does not depend on the rest of its batch
+    with Batch(specs[2:3], n_therm=20, n_bins=4, sweeps_per_bin=10, engine=BATCHED) as alone:
+        alone.run()
+        assert np.array_equal(alone.moments()[0], out[0][0][2])
+
+
+def test_run_past_end_is_refused(hip):
+    from rbim.engine import Batch
+    from rbim._hip import EngineError
+
+    specs = _specs((8,), temps=(2.0,))
+    with Batch(specs, n_therm=1, n_bins=1, sweeps_per_bin=2) as batch:
+        batch.run(3)
+        with pytest.raises(EngineError):
+            batch.run(1)
