@@ -3,22 +3,32 @@
 // One workgroup owns one lattice for a whole block of sweeps: spins, couplings,
 // the union-find forest and the per-cluster counters all live in LDS, so a
 // sweep touches HBM only through the moment accumulators at bin boundaries.
-// Per sweep (src/rbim/swendsen_wang.py:103-172):
-//   A  activation   Philox draws for each site pair; satisfied bonds are
-//                   occupied when draw < thr31 (clusters.py:48-79)
-//   B  union        hook-and-compress on LDS atomics; the forest word packs
-//                   parent (14 bits) with the winding numbers (wx, wy, 9 bits
-//                   each) of the tree path to the parent.  A bond whose ends
-//                   already share a root is a non-tree bond; its loop winding
-//                   c + w(b) - w(a) decides wrap_x / wrap_y exactly
-//                   (_unionfind.py:101-124)
-//   CD flip         root = min site of the cluster (larger roots always hook
-//                   under smaller ones); s ^= flipbit(root); per-root counter
-//                   += (1 << 16) | [s = +1]  (swendsen_wang.py:140-141,
-//                   clusters.py:98-118)
-//   E  measure      E, M, n_clusters and the two largest clusters (size desc,
-//                   min site asc) by wavefront reductions; one lane folds the
-//                   sweep into its bin's moment sums (simulation.py:212-236).
+// Each lane owns a site pair (2q, 2q+1) -- the unit of one Philox4x32-10 call
+// -- in every phase.  Per sweep g (src/rbim/swendsen_wang.py:103-172):
+//
+//   A  activation  Philox draws of the pair; satisfied bonds are occupied when
+//                  draw < thr31 (clusters.py:48-79).  Occupied in-row bonds
+//                  (y -> y+1 inside the 128-site chunk of a wavefront) are
+//                  merged without atomics: two ballots give every site the
+//                  start of its run, which becomes its parent (winding 0).
+//                  The same pass measures sweep g-1 (below).
+//   B  union       the remaining occupied bonds (all x bonds, y bonds that
+//                  wrap or leave the chunk) by hook-and-compress on LDS CAS.
+//                  The forest word packs parent (14 bits) with the windings
+//                  (wx, wy; 9 bits each) of the tree path to the parent; the
+//                  larger root always hooks under the smaller, so every root
+//                  is its cluster's minimum site.  A bond that finds both ends
+//                  under one root is a non-tree bond: its loop winding
+//                  c + w(b) - w(a) decides wrap_x / wrap_y exactly
+//                  (_unionfind.py:101-124).
+//   C  flip        s ^= flipbit(root); cnt[root] += (1 << 16) | [s = +1]
+//                  (swendsen_wang.py:140-141, clusters.py:98-118).
+//   measurement of sweep g (fused into A of sweep g+1, or a closing pass):
+//                  satisfied bonds, up spins and roots are counted with
+//                  ballot + popcount; the two largest clusters (size desc,
+//                  min site asc) by a 64-bit key max-reduction.  Lane 0 then
+//                  folds the sweep into its bin's moment sums
+//                  (simulation.py:212-236).
 #include "engine.h"
 
 namespace {
@@ -57,7 +67,7 @@ __device__ __forceinline__ uint32_t lds_find(uint32_t* uf, uint32_t x, int& wx, 
   }
 }
 
-__device__ __forceinline__ uint32_t lds_root(uint32_t* uf, uint32_t x) {
+__device__ __forceinline__ uint32_t lds_root(const uint32_t* uf, uint32_t x) {
   uint32_t v = uf[x];
   while ((v & PMASK) != x) {
     const uint32_t p = v & PMASK;
@@ -88,23 +98,50 @@ __device__ __forceinline__ uint32_t lds_union(uint32_t* uf, uint32_t a, uint32_t
   }
 }
 
-__device__ __forceinline__ void top2_merge(uint32_t& k1, uint32_t& k2, uint32_t o1, uint32_t o2) {
-  const uint32_t hi = max(k1, o1);
-  const uint32_t lo = max(min(k1, o1), max(k2, o2));
+// Cluster key: size (bits 32..), ~root (16..29), n_plus (0..14).  The max key is
+// the largest cluster, ties to the smaller root; n_plus rides along.
+__device__ __forceinline__ uint64_t cluster_key(uint32_t c, uint32_t root) {
+  return ((uint64_t)(c >> 16) << 32) | ((uint64_t)(PMASK - root) << 16) | (uint64_t)(c & 0xFFFFu);
+}
+
+__device__ __forceinline__ void top2_merge(uint64_t& k1, uint64_t& k2, uint64_t o1, uint64_t o2) {
+  const uint64_t hi = max(k1, o1);
+  const uint64_t lo = max(min(k1, o1), max(k2, o2));
   k1 = hi;
   k2 = lo;
+}
+
+__device__ __forceinline__ uint32_t popc_ballot(bool pred) {
+  return (uint32_t)__popcll(__ballot(pred));
+}
+
+// Geometry of one site: coordinates and the two forward neighbours.
+struct Site {
+  uint32_t n, x, y, nr, nu;
+  bool valid;
+};
+
+__device__ __forceinline__ Site make_site(uint32_t n, uint32_t N, uint32_t L, uint32_t magic) {
+  Site s;
+  s.n = n;
+  s.valid = n < N;
+  s.x = site_x(n, magic);
+  s.y = n - s.x * L;
+  s.nr = (s.x == L - 1) ? s.y : n + L;
+  s.nu = (s.y == L - 1) ? n - s.y : n + 1;
+  return s;
 }
 
 template <int NT>
 __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t sweeps) {
   constexpr int NW = NT / 64;
   extern __shared__ uint32_t smem[];
-  __shared__ int32_t red_i[NW][3];
-  __shared__ uint32_t red_k[NW][2];
-  __shared__ uint32_t wrapflag;
+  __shared__ uint32_t red_c[NW][3];  // satisfied bonds, up spins, roots
+  __shared__ uint64_t red_k[NW][2];  // two largest cluster keys
+  __shared__ uint32_t wrapflag[2];   // by sweep parity
   __shared__ uint64_t a[RBIM_ACC_SLOTS];  // running sums of the open bin (lane 0)
 
-  const int N = s.N;
+  const uint32_t N = (uint32_t)s.N;
   const uint32_t L = (uint32_t)s.L;
   const uint32_t magic = s.magic;
   uint32_t* uf = smem;
@@ -115,11 +152,18 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
+  const uint64_t below = (1ull << lane) - 1ull;
+  const uint32_t npairs = (N + 1) >> 1;
+  const int iters = (int)((npairs + NT - 1) / NT);
 
   int8_t* gsp = s.spins + (size_t)lat * N;
   const uint8_t* gj = s.jbits + (size_t)lat * N;
-  for (int n = tid; n < N; n += NT) sb[n] = (uint8_t)((gsp[n] > 0 ? 1u : 0u) | ((uint32_t)gj[n] & 3u) << 1);
-  if (tid == 0) wrapflag = 0u;
+  for (uint32_t n = tid; n < N; n += NT)
+    sb[n] = (uint8_t)((gsp[n] > 0 ? 1u : 0u) | ((uint32_t)gj[n] & 3u) << 1);
+  if (tid == 0) {
+    wrapflag[0] = 0u;
+    wrapflag[1] = 0u;
+  }
 
   const int64_t g0 = s.sweep_ctr[lat];
   const int64_t gend = min(g0 + (int64_t)sweeps, s.total);
@@ -129,160 +173,228 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
   const bool given = s.given_occ != nullptr;
   uint64_t* gacc = s.acc + (size_t)lat * s.n_bins * RBIM_ACC_SLOTS;
 
-  int64_t cur_bin = -1;
+  int64_t cur_bin = -1;  // lane 0 of wave 0 only
   __syncthreads();
 
-  for (int64_t g = g0; g < gend; ++g) {
-    // ---------------------------------------------------------------- A
-    const int npairs = (N + 1) >> 1;
-    for (int q = tid; q < npairs; q += NT) {
-      rbim_u32x4 r = {0u, 0u, 0u, 0u};
-      if (!given) r = rbim_sweep_draws(seed, lid, (uint32_t)g, (uint32_t)q);
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const uint32_t n = 2u * q + j;
-        if (n >= (uint32_t)N) break;
-        const uint32_t x = site_x(n, magic);
-        const uint32_t y = n - x * L;
-        const uint32_t nr = (x == L - 1) ? y : n + L;
-        const uint32_t nu = (y == L - 1) ? n - y : n + 1;
-        const uint32_t b = sb[n];
-        const uint32_t sp = b & 1u;
-        uint32_t oh, ov, fl;
-        if (given) {
-          const uint32_t o = s.given_occ[n];
-          oh = o & 1u;
-          ov = (o >> 1) & 1u;
-          fl = s.given_flip[n] & 1u;
-        } else {
-          const uint32_t sat_h = ~(sp ^ sb[nr] ^ (b >> 1)) & 1u;
-          const uint32_t sat_v = ~(sp ^ sb[nu] ^ (b >> 2)) & 1u;
-          const uint32_t dh = j ? r.z : r.x;
-          const uint32_t dv = j ? r.w : r.y;
-          oh = sat_h & ((dh >> 1) < thr ? 1u : 0u);
-          ov = sat_v & ((dv >> 1) < thr ? 1u : 0u);
-          fl = dh & 1u;
-        }
-        sb[n] = (uint8_t)((b & 7u) | (oh << 3) | (ov << 4) | (fl << 5));
-        uf[n] = n;
-        cnt[n] = 0u;
-      }
-    }
-    __syncthreads();
-    // ---------------------------------------------------------------- B
-    uint32_t wraps = 0u;
-    for (int n = tid; n < N; n += NT) {
-      const uint32_t b = sb[n];
-      if ((b & 0x18u) == 0u) continue;
-      const uint32_t x = site_x((uint32_t)n, magic);
-      const uint32_t y = (uint32_t)n - x * L;
-      if (b & 0x08u) wraps |= lds_union(uf, n, (x == L - 1) ? y : n + L, x == L - 1 ? 1 : 0, 0);
-      if (b & 0x10u) wraps |= lds_union(uf, n, (y == L - 1) ? n - y : n + 1, 0, y == L - 1 ? 1 : 0);
-    }
-    if (wraps) atomicOr(&wrapflag, wraps);
-    __syncthreads();
-    // --------------------------------------------------------------- CD
-    for (int n = tid; n < N; n += NT) {
-      const uint32_t r = lds_root(uf, (uint32_t)n);
-      const uint32_t b = sb[n];
-      const uint32_t ns = (b ^ (sb[r] >> 5)) & 1u;
-      sb[n] = (uint8_t)((b & ~1u) | ns);
-      atomicAdd(&cnt[r], 0x10000u | ns);
-      if (s.canon_out) s.canon_out[n] = (int32_t)r;
-    }
-    __syncthreads();
-    const bool measure = g >= s.n_therm;
-    if (!measure) {
-      if (tid == 0) wrapflag = 0u;
-      continue;  // the barrier at the top of A orders this reset
-    }
-    // ---------------------------------------------------------------- E
-    int32_t e = 0, m = 0, ncl = 0;
-    uint32_t k1 = 0u, k2 = 0u;
-    for (int n = tid; n < N; n += NT) {
-      const uint32_t x = site_x((uint32_t)n, magic);
-      const uint32_t y = (uint32_t)n - x * L;
-      const uint32_t b = sb[n];
-      const uint32_t sp = b & 1u;
-      const uint32_t sr = sb[(x == L - 1) ? y : n + L];
-      const uint32_t su = sb[(y == L - 1) ? n - y : n + 1];
-      const uint32_t sat_h = ~(sp ^ sr ^ (b >> 1)) & 1u;
-      const uint32_t sat_v = ~(sp ^ su ^ (b >> 2)) & 1u;
-      e += 2 - 2 * (int32_t)(sat_h + sat_v);
-      m += 2 * (int32_t)sp - 1;
-      if ((uf[n] & PMASK) == (uint32_t)n) {
-        ++ncl;
-        const uint32_t key = ((cnt[n] >> 16) << 14) | (PMASK - (uint32_t)n);
-        top2_merge(k1, k2, key, 0u);
-      }
-    }
+  // Measurement bookkeeping for one pass (counts are wave-uniform).
+  uint32_t m_sat = 0u, m_up = 0u, m_root = 0u;
+  uint64_t k1 = 0ull, k2 = 0ull;
+  auto measure_site = [&](const Site& st, uint32_t b, uint32_t sat_h, uint32_t sat_v) {
+    m_sat += popc_ballot(st.valid && sat_h) + popc_ballot(st.valid && sat_v);
+    m_up += popc_ballot(st.valid && (b & 1u));
+    const bool root = st.valid && (uf[st.n] & PMASK) == st.n;
+    m_root += popc_ballot(root);
+    if (root) top2_merge(k1, k2, cluster_key(cnt[st.n], st.n), 0ull);
+  };
+  auto publish = [&]() {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
-      e += __shfl_xor(e, off);
-      m += __shfl_xor(m, off);
-      ncl += __shfl_xor(ncl, off);
-      const uint32_t o1 = __shfl_xor(k1, off);
-      const uint32_t o2 = __shfl_xor(k2, off);
+      const uint64_t o1 = __shfl_xor(k1, off);
+      const uint64_t o2 = __shfl_xor(k2, off);
       top2_merge(k1, k2, o1, o2);
     }
     if (lane == 0) {
-      red_i[wave][0] = e;
-      red_i[wave][1] = m;
-      red_i[wave][2] = ncl;
+      red_c[wave][0] = m_sat;
+      red_c[wave][1] = m_up;
+      red_c[wave][2] = m_root;
       red_k[wave][0] = k1;
       red_k[wave][1] = k2;
     }
-    __syncthreads();
-    if (tid == 0) {
-      for (int w = 1; w < NW; ++w) {
-        e += red_i[w][0];
-        m += red_i[w][1];
-        ncl += red_i[w][2];
-        top2_merge(k1, k2, red_k[w][0], red_k[w][1]);
+    m_sat = m_up = m_root = 0u;
+    k1 = k2 = 0ull;
+  };
+  // Lane 0 of wave 0: fold the published partials of sweep gm into its bin.
+  auto finalize = [&](int64_t gm) {
+    uint32_t sat = 0u, up = 0u, roots = 0u;
+    uint64_t f1 = 0ull, f2 = 0ull;
+    for (int w = 0; w < NW; ++w) {
+      sat += red_c[w][0];
+      up += red_c[w][1];
+      roots += red_c[w][2];
+      top2_merge(f1, f2, red_k[w][0], red_k[w][1]);
+    }
+    rbim_sweep_stats st;
+    st.energy = 2 * (int64_t)N - 2 * (int64_t)sat;
+    st.magnetization = 2 * (int64_t)up - (int64_t)N;
+    st.n_clusters = roots;
+    const int64_t s1 = (int64_t)(f1 >> 32), mc1 = 2 * (int64_t)(f1 & 0xFFFFu) - s1;
+    st.largest_size = s1;
+    st.largest_abs_mag = mc1 < 0 ? -mc1 : mc1;
+    if (f2 != 0ull) {
+      const int64_t s2 = (int64_t)(f2 >> 32), mc2 = 2 * (int64_t)(f2 & 0xFFFFu) - s2;
+      st.second_size = s2;
+      st.second_abs_mag = mc2 < 0 ? -mc2 : mc2;
+    } else {
+      st.second_size = 0;
+      st.second_abs_mag = 0;
+    }
+    const uint32_t wf = wrapflag[gm & 1];
+    wrapflag[gm & 1] = 0u;
+    st.wrap_x = (int32_t)(wf & 1u);
+    st.wrap_y = (int32_t)((wf >> 1) & 1u);
+    const int64_t bin = (gm - s.n_therm) / s.spb;
+    if (bin != cur_bin) {
+      if (cur_bin >= 0)
+        for (int k = 0; k < RBIM_ACC_SLOTS; ++k) gacc[cur_bin * RBIM_ACC_SLOTS + k] = a[k];
+      for (int k = 0; k < RBIM_ACC_SLOTS; ++k) a[k] = gacc[bin * RBIM_ACC_SLOTS + k];
+      cur_bin = bin;
+    }
+    acc_sweep(a, st);
+    if (s.last) s.last[lat] = st;
+  };
+
+  bool pending = false;  // sweep g-1 still to be measured (block-uniform)
+  for (int64_t g = g0; g < gend; ++g) {
+    // ------------------------------------------------ A (+ measure g-1)
+    for (int it = 0; it < iters; ++it) {
+      const uint32_t q = (uint32_t)(it * NT + tid);
+      const uint32_t chunk = 2u * (uint32_t)(it * NT + wave * 64);
+      const Site s0 = make_site(2u * q, N, L, magic);
+      const Site s1 = make_site(2u * q + 1u, N, L, magic);
+      uint32_t b0 = 0u, b1 = 0u, sh0 = 0u, sv0 = 0u, sh1 = 0u, sv1 = 0u;
+      if (s0.valid) {
+        b0 = sb[s0.n];
+        sh0 = ~(b0 ^ sb[s0.nr] ^ (b0 >> 1)) & 1u;
+        sv0 = ~(b0 ^ sb[s0.nu] ^ (b0 >> 2)) & 1u;
       }
-      rbim_sweep_stats st;
-      st.energy = e;
-      st.magnetization = m;
-      st.n_clusters = ncl;
-      const uint32_t r1 = PMASK - (k1 & PMASK);
-      const int32_t s1 = (int32_t)(k1 >> 14);
-      const int32_t mc1 = 2 * (int32_t)(cnt[r1] & 0xFFFFu) - s1;
-      st.largest_size = s1;
-      st.largest_abs_mag = mc1 < 0 ? -mc1 : mc1;
-      if (k2 != 0u) {
-        const uint32_t r2 = PMASK - (k2 & PMASK);
-        const int32_t s2 = (int32_t)(k2 >> 14);
-        const int32_t mc2 = 2 * (int32_t)(cnt[r2] & 0xFFFFu) - s2;
-        st.second_size = s2;
-        st.second_abs_mag = mc2 < 0 ? -mc2 : mc2;
+      if (s1.valid) {
+        b1 = sb[s1.n];
+        sh1 = ~(b1 ^ sb[s1.nr] ^ (b1 >> 1)) & 1u;
+        sv1 = ~(b1 ^ sb[s1.nu] ^ (b1 >> 2)) & 1u;
+      }
+      if (pending) {
+        measure_site(s0, b0, sh0, sv0);
+        measure_site(s1, b1, sh1, sv1);
+      }
+      uint32_t oh0, ov0, fl0, oh1, ov1, fl1;
+      if (given) {
+        const uint32_t o0 = s0.valid ? s.given_occ[s0.n] : 0u;
+        const uint32_t o1 = s1.valid ? s.given_occ[s1.n] : 0u;
+        oh0 = o0 & 1u;
+        ov0 = (o0 >> 1) & 1u;
+        fl0 = s0.valid ? s.given_flip[s0.n] & 1u : 0u;
+        oh1 = o1 & 1u;
+        ov1 = (o1 >> 1) & 1u;
+        fl1 = s1.valid ? s.given_flip[s1.n] & 1u : 0u;
       } else {
-        st.second_size = 0;
-        st.second_abs_mag = 0;
+        const rbim_u32x4 r = rbim_sweep_draws(seed, lid, (uint32_t)g, q);
+        oh0 = sh0 & ((r.x >> 1) < thr ? 1u : 0u);
+        ov0 = sv0 & ((r.y >> 1) < thr ? 1u : 0u);
+        fl0 = r.x & 1u;
+        oh1 = sh1 & ((r.z >> 1) < thr ? 1u : 0u);
+        ov1 = sv1 & ((r.w >> 1) < thr ? 1u : 0u);
+        fl1 = r.z & 1u;
       }
-      st.wrap_x = (int32_t)(wrapflag & 1u);
-      st.wrap_y = (int32_t)((wrapflag >> 1) & 1u);
-      wrapflag = 0u;
-      const int64_t bin = (g - s.n_therm) / s.spb;
-      if (bin != cur_bin) {
-        if (cur_bin >= 0) {
-          for (int k = 0; k < RBIM_ACC_SLOTS; ++k) gacc[cur_bin * RBIM_ACC_SLOTS + k] = a[k];
+      // run pre-labelling of in-row bonds inside this wave's 128-site chunk
+      const bool internal = s1.valid && ov0 && s0.y != L - 1;
+      const bool outlink = s1.valid && ov1 && s1.y != L - 1;
+      const uint64_t thru = __ballot(internal && outlink);
+      const uint64_t ol = __ballot(outlink);
+      const uint64_t stop = ~thru & below;
+      uint32_t start0;
+      if (stop == 0ull) {
+        start0 = chunk;
+      } else {
+        const uint32_t j = 63u - (uint32_t)__clzll(stop);
+        start0 = chunk + 2u * j + (((ol >> j) & 1ull) ? 1u : 2u);
+      }
+      if (s0.valid) {
+        sb[s0.n] = (uint8_t)((b0 & 7u) | (oh0 << 3) | (ov0 << 4) | (fl0 << 5));
+        uf[s0.n] = start0;
+        cnt[s0.n] = 0u;
+      }
+      if (s1.valid) {
+        sb[s1.n] = (uint8_t)((b1 & 7u) | (oh1 << 3) | (ov1 << 4) | (fl1 << 5));
+        uf[s1.n] = internal ? start0 : s1.n;
+        cnt[s1.n] = 0u;
+      }
+    }
+    if (pending) publish();
+    __syncthreads();
+    if (pending && tid == 0) finalize(g - 1);
+    // ------------------------------------------------------------ B
+    uint32_t wraps = 0u;
+    for (int it = 0; it < iters; ++it) {
+      const uint32_t q = (uint32_t)(it * NT + tid);
+      const Site s0 = make_site(2u * q, N, L, magic);
+      const Site s1 = make_site(2u * q + 1u, N, L, magic);
+      if (s0.valid) {
+        const uint32_t b0 = sb[s0.n];
+        if (b0 & 0x08u) wraps |= lds_union(uf, s0.n, s0.nr, s0.x == L - 1 ? 1 : 0, 0);
+        if ((b0 & 0x10u) && s0.y == L - 1) wraps |= lds_union(uf, s0.n, s0.nu, 0, 1);
+      }
+      if (s1.valid) {
+        const uint32_t b1 = sb[s1.n];
+        if (b1 & 0x08u) wraps |= lds_union(uf, s1.n, s1.nr, s1.x == L - 1 ? 1 : 0, 0);
+        if ((b1 & 0x10u) && (s1.y == L - 1 || lane == 63))
+          wraps |= lds_union(uf, s1.n, s1.nu, 0, s1.y == L - 1 ? 1 : 0);
+      }
+    }
+    if (wraps) atomicOr(&wrapflag[g & 1], wraps);
+    __syncthreads();
+    // ------------------------------------------------------------ C
+    for (int it = 0; it < iters; ++it) {
+      const uint32_t q = (uint32_t)(it * NT + tid);
+      const uint32_t n0 = 2u * q, n1 = n0 + 1u;
+      if (n0 >= N) continue;
+      const uint32_t r0 = lds_root(uf, n0);
+      const uint32_t b0 = sb[n0];
+      const uint32_t ns0 = (b0 ^ (sb[r0] >> 5)) & 1u;
+      sb[n0] = (uint8_t)((b0 & ~1u) | ns0);
+      if (s.canon_out) s.canon_out[n0] = (int32_t)r0;
+      if (n1 < N) {
+        const uint32_t r1 = lds_root(uf, n1);
+        const uint32_t b1 = sb[n1];
+        const uint32_t ns1 = (b1 ^ (sb[r1] >> 5)) & 1u;
+        sb[n1] = (uint8_t)((b1 & ~1u) | ns1);
+        if (s.canon_out) s.canon_out[n1] = (int32_t)r1;
+        if (r1 == r0) {
+          atomicAdd(&cnt[r0], 0x20000u | (ns0 + ns1));
+        } else {
+          atomicAdd(&cnt[r0], 0x10000u | ns0);
+          atomicAdd(&cnt[r1], 0x10000u | ns1);
         }
-        for (int k = 0; k < RBIM_ACC_SLOTS; ++k) a[k] = gacc[bin * RBIM_ACC_SLOTS + k];
-        cur_bin = bin;
+      } else {
+        atomicAdd(&cnt[r0], 0x10000u | ns0);
       }
-      acc_sweep(a, st);
-      if (s.last) s.last[lat] = st;
     }
     __syncthreads();
+    pending = g >= s.n_therm;
+    // an unmeasured sweep's wrap bits are dropped; its slot is reused two
+    // sweeps later, after two more barriers
+    if (!pending && tid == 0) wrapflag[g & 1] = 0u;
+  }
+
+  if (pending) {  // measure the last sweep of this launch
+    for (int it = 0; it < iters; ++it) {
+      const uint32_t q = (uint32_t)(it * NT + tid);
+      const Site s0 = make_site(2u * q, N, L, magic);
+      const Site s1 = make_site(2u * q + 1u, N, L, magic);
+      uint32_t b0 = 0u, b1 = 0u, sh0 = 0u, sv0 = 0u, sh1 = 0u, sv1 = 0u;
+      if (s0.valid) {
+        b0 = sb[s0.n];
+        sh0 = ~(b0 ^ sb[s0.nr] ^ (b0 >> 1)) & 1u;
+        sv0 = ~(b0 ^ sb[s0.nu] ^ (b0 >> 2)) & 1u;
+      }
+      if (s1.valid) {
+        b1 = sb[s1.n];
+        sh1 = ~(b1 ^ sb[s1.nr] ^ (b1 >> 1)) & 1u;
+        sv1 = ~(b1 ^ sb[s1.nu] ^ (b1 >> 2)) & 1u;
+      }
+      measure_site(s0, b0, sh0, sv0);
+      measure_site(s1, b1, sh1, sv1);
+    }
+    publish();
+    __syncthreads();
+    if (tid == 0) finalize(gend - 1);
   }
 
   if (tid == 0) {
-    if (cur_bin >= 0) {
+    if (cur_bin >= 0)
       for (int k = 0; k < RBIM_ACC_SLOTS; ++k) gacc[cur_bin * RBIM_ACC_SLOTS + k] = a[k];
-    }
     s.sweep_ctr[lat] = gend > g0 ? gend : g0;
   }
-  for (int n = tid; n < N; n += NT) gsp[n] = (sb[n] & 1u) ? (int8_t)1 : (int8_t)-1;
+  for (uint32_t n = tid; n < N; n += NT) gsp[n] = (sb[n] & 1u) ? (int8_t)1 : (int8_t)-1;
 }
 
 __global__ void init_spins_kernel(LatticeSet s, int start) {
@@ -296,9 +408,14 @@ __global__ void init_spins_kernel(LatticeSet s, int start) {
 template <int NT>
 hipError_t launch_lds_t(const LatticeSet& s, int sweeps, hipStream_t st) {
   const size_t bytes = lds_bytes(s.N);
-  hipError_t e = hipFuncSetAttribute((const void*)lds_sweep_kernel<NT>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  if (e != hipSuccess) return e;
+  static bool configured = false;
+  if (!configured) {
+    hipError_t e = hipFuncSetAttribute((const void*)lds_sweep_kernel<NT>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds_bytes(RBIM_LDS_MAX_N));
+    if (e != hipSuccess) return e;
+    configured = true;
+  }
   hipLaunchKernelGGL(lds_sweep_kernel<NT>, dim3(s.n_lat), dim3(NT), bytes, st, s, sweeps);
   return hipGetLastError();
 }
