@@ -49,9 +49,11 @@ def scan_specs(L, p, temperatures, R, seed):
 
 
 def run_schedule(K):
-    """(n_therm, n_bins, sweeps_per_bin) with n_therm + 10 * spb == K, ~1/11 therm."""
-    n_bins = 10
-    spb = max(1, (K - K // 11) // n_bins)
+    """(n_therm, n_bins, sweeps_per_bin) with n_therm + n_bins * spb == K, ~1/11
+    thermalisation and up to 10 bins (the default scan's shape at K = 22000)."""
+    measured = max(1, K - K // 11)
+    n_bins = min(10, measured)
+    spb = measured // n_bins
     return K - n_bins * spb, n_bins, spb
 
 
@@ -143,7 +145,21 @@ def main():
     ap.add_argument("--cpu-sweeps", type=int, default=2500)
     ap.add_argument("--cpu-per-thread", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (rocprof counter "
+                         "calibration) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_dispatch": 4 * n, "dispatches": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box in this build has one MI355X "
               f"(asked for --gpus {args.gpus}); independent lattices shard across GPUs "

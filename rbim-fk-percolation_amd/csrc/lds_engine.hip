@@ -79,21 +79,66 @@ __device__ __forceinline__ uint32_t lds_root(const uint32_t* uf, uint32_t x) {
   return x;
 }
 
+// Walk a and b to their current roots together (the two chains' loads are in
+// flight at the same time), accumulating tree-path windings and halving both
+// paths.  a / b may be stale roots from a failed hook: the windings carried so
+// far stay valid because relative positions inside a tree never change.
+__device__ __forceinline__ void find2(uint32_t* uf, uint32_t& a, int& ax, int& ay, uint32_t& b,
+                                      int& bx, int& by) {
+  uint32_t va = uf[a], vb = uf[b];
+  while (true) {
+    const uint32_t pa = va & PMASK, pb = vb & PMASK;
+    const bool da = pa == a, db = pb == b;
+    if (da && db) return;
+    const uint32_t wa = da ? va : uf[pa];
+    const uint32_t wb = db ? vb : uf[pb];
+    if (!da) {
+      const uint32_t ga = wa & PMASK;
+      if (ga == pa) {
+        ax += uf_wx(va);
+        ay += uf_wy(va);
+        a = pa;
+        va = wa;
+      } else {
+        const int hx = uf_wx(va) + uf_wx(wa), hy = uf_wy(va) + uf_wy(wa);
+        uf[a] = uf_pack(ga, hx, hy);
+        ax += hx;
+        ay += hy;
+        a = ga;
+        va = uf[ga];
+      }
+    }
+    if (!db) {
+      const uint32_t gb = wb & PMASK;
+      if (gb == pb) {
+        bx += uf_wx(vb);
+        by += uf_wy(vb);
+        b = pb;
+        vb = wb;
+      } else {
+        const int hx = uf_wx(vb) + uf_wx(wb), hy = uf_wy(vb) + uf_wy(wb);
+        uf[b] = uf_pack(gb, hx, hy);
+        bx += hx;
+        by += hy;
+        b = gb;
+        vb = uf[gb];
+      }
+    }
+  }
+}
+
 // Union of the occupied bond a -> b that crosses the periodic boundary
 // (cx, cy) times.  Returns wrap bits (1: x, 2: y) when the bond closes a loop.
 __device__ __forceinline__ uint32_t lds_union(uint32_t* uf, uint32_t a, uint32_t b, int cx,
                                               int cy) {
+  int ax = 0, ay = 0, bx = 0, by = 0;
   while (true) {
-    int ax, ay, bx, by;
-    const uint32_t ra = lds_find(uf, a, ax, ay);
-    const uint32_t rb = lds_find(uf, b, bx, by);
-    if (ra == rb) {
-      return ((cx + bx - ax) != 0 ? 1u : 0u) | ((cy + by - ay) != 0 ? 2u : 0u);
-    }
-    if (ra < rb) {
-      if (atomicCAS(&uf[rb], rb, uf_pack(ra, ax - bx - cx, ay - by - cy)) == rb) return 0u;
+    find2(uf, a, ax, ay, b, bx, by);
+    if (a == b) return ((cx + bx - ax) != 0 ? 1u : 0u) | ((cy + by - ay) != 0 ? 2u : 0u);
+    if (a < b) {
+      if (atomicCAS(&uf[b], b, uf_pack(a, ax - bx - cx, ay - by - cy)) == b) return 0u;
     } else {
-      if (atomicCAS(&uf[ra], ra, uf_pack(rb, bx - ax + cx, by - ay + cy)) == ra) return 0u;
+      if (atomicCAS(&uf[a], a, uf_pack(b, bx - ax + cx, by - ay + cy)) == a) return 0u;
     }
   }
 }
@@ -115,24 +160,38 @@ __device__ __forceinline__ uint32_t popc_ballot(bool pred) {
   return (uint32_t)__popcll(__ballot(pred));
 }
 
-// Geometry of one site: coordinates and the two forward neighbours.
-struct Site {
-  uint32_t n, x, y, nr, nu;
-  bool valid;
+// Geometry of the site pair (2q, 2q+1).  LC > 0 is a compile-time L.
+struct Pair {
+  uint32_t n0, n1, x0, y0, x1, y1, nr0, nu0, nr1, nu1;
+  bool v0, v1;
 };
 
-__device__ __forceinline__ Site make_site(uint32_t n, uint32_t N, uint32_t L, uint32_t magic) {
-  Site s;
-  s.n = n;
-  s.valid = n < N;
-  s.x = site_x(n, magic);
-  s.y = n - s.x * L;
-  s.nr = (s.x == L - 1) ? s.y : n + L;
-  s.nu = (s.y == L - 1) ? n - s.y : n + 1;
-  return s;
+template <int LC>
+__device__ __forceinline__ Pair make_pair(uint32_t q, uint32_t N, uint32_t Lr, uint32_t magic) {
+  const uint32_t L = LC ? (uint32_t)LC : Lr;
+  Pair p;
+  p.n0 = 2u * q;
+  p.n1 = p.n0 + 1u;
+  p.v0 = p.n0 < N;
+  p.v1 = p.n1 < N;
+  p.x0 = LC ? p.n0 / (uint32_t)LC : site_x(p.n0, magic);
+  p.y0 = p.n0 - p.x0 * L;
+  if (LC != 0 && (LC & 1) == 0) {  // even L: the pair never straddles a row
+    p.x1 = p.x0;
+    p.y1 = p.y0 + 1u;
+  } else {
+    const bool edge = p.y0 == L - 1u;
+    p.x1 = edge ? p.x0 + 1u : p.x0;
+    p.y1 = edge ? 0u : p.y0 + 1u;
+  }
+  p.nr0 = (p.x0 == L - 1u) ? p.y0 : p.n0 + L;
+  p.nu0 = (p.y0 == L - 1u) ? p.n0 - p.y0 : p.n0 + 1u;
+  p.nr1 = (p.x1 == L - 1u) ? p.y1 : p.n1 + L;
+  p.nu1 = (p.y1 == L - 1u) ? p.n1 - p.y1 : p.n1 + 1u;
+  return p;
 }
 
-template <int NT>
+template <int NT, int LC>
 __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t sweeps) {
   constexpr int NW = NT / 64;
   extern __shared__ uint32_t smem[];
@@ -141,8 +200,8 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
   __shared__ uint32_t wrapflag[2];   // by sweep parity
   __shared__ uint64_t a[RBIM_ACC_SLOTS];  // running sums of the open bin (lane 0)
 
-  const uint32_t N = (uint32_t)s.N;
-  const uint32_t L = (uint32_t)s.L;
+  const uint32_t N = LC ? (uint32_t)(LC * LC) : (uint32_t)s.N;
+  const uint32_t L = LC ? (uint32_t)LC : (uint32_t)s.L;
   const uint32_t magic = s.magic;
   uint32_t* uf = smem;
   uint32_t* cnt = smem + N;
@@ -179,12 +238,12 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
   // Measurement bookkeeping for one pass (counts are wave-uniform).
   uint32_t m_sat = 0u, m_up = 0u, m_root = 0u;
   uint64_t k1 = 0ull, k2 = 0ull;
-  auto measure_site = [&](const Site& st, uint32_t b, uint32_t sat_h, uint32_t sat_v) {
-    m_sat += popc_ballot(st.valid && sat_h) + popc_ballot(st.valid && sat_v);
-    m_up += popc_ballot(st.valid && (b & 1u));
-    const bool root = st.valid && (uf[st.n] & PMASK) == st.n;
+  auto measure_site = [&](bool valid, uint32_t n, uint32_t b, uint32_t sat_h, uint32_t sat_v) {
+    m_sat += popc_ballot(valid && sat_h) + popc_ballot(valid && sat_v);
+    m_up += popc_ballot(valid && (b & 1u));
+    const bool root = valid && (uf[n] & PMASK) == n;
     m_root += popc_ballot(root);
-    if (root) top2_merge(k1, k2, cluster_key(cnt[st.n], st.n), 0ull);
+    if (root) top2_merge(k1, k2, cluster_key(cnt[n], n), 0ull);
   };
   auto publish = [&]() {
 #pragma unroll
@@ -242,40 +301,46 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
     acc_sweep(a, st);
     if (s.last) s.last[lat] = st;
   };
+  // satisfied flags of the pair's four bonds, from the current spins
+  auto satisfied = [&](const Pair& p, uint32_t& b0, uint32_t& b1, uint32_t& sh0, uint32_t& sv0,
+                       uint32_t& sh1, uint32_t& sv1) {
+    b0 = b1 = sh0 = sv0 = sh1 = sv1 = 0u;
+    if (p.v0) {
+      b0 = sb[p.n0];
+      sh0 = ~(b0 ^ sb[p.nr0] ^ (b0 >> 1)) & 1u;
+      sv0 = ~(b0 ^ sb[p.nu0] ^ (b0 >> 2)) & 1u;
+    }
+    if (p.v1) {
+      b1 = sb[p.n1];
+      sh1 = ~(b1 ^ sb[p.nr1] ^ (b1 >> 1)) & 1u;
+      sv1 = ~(b1 ^ sb[p.nu1] ^ (b1 >> 2)) & 1u;
+    }
+  };
 
   bool pending = false;  // sweep g-1 still to be measured (block-uniform)
   for (int64_t g = g0; g < gend; ++g) {
     // ------------------------------------------------ A (+ measure g-1)
+#pragma unroll 1
     for (int it = 0; it < iters; ++it) {
       const uint32_t q = (uint32_t)(it * NT + tid);
       const uint32_t chunk = 2u * (uint32_t)(it * NT + wave * 64);
-      const Site s0 = make_site(2u * q, N, L, magic);
-      const Site s1 = make_site(2u * q + 1u, N, L, magic);
-      uint32_t b0 = 0u, b1 = 0u, sh0 = 0u, sv0 = 0u, sh1 = 0u, sv1 = 0u;
-      if (s0.valid) {
-        b0 = sb[s0.n];
-        sh0 = ~(b0 ^ sb[s0.nr] ^ (b0 >> 1)) & 1u;
-        sv0 = ~(b0 ^ sb[s0.nu] ^ (b0 >> 2)) & 1u;
-      }
-      if (s1.valid) {
-        b1 = sb[s1.n];
-        sh1 = ~(b1 ^ sb[s1.nr] ^ (b1 >> 1)) & 1u;
-        sv1 = ~(b1 ^ sb[s1.nu] ^ (b1 >> 2)) & 1u;
-      }
+      const Pair p = make_pair<LC>(q, N, L, magic);
+      uint32_t b0, b1, sh0, sv0, sh1, sv1;
+      satisfied(p, b0, b1, sh0, sv0, sh1, sv1);
       if (pending) {
-        measure_site(s0, b0, sh0, sv0);
-        measure_site(s1, b1, sh1, sv1);
+        measure_site(p.v0, p.n0, b0, sh0, sv0);
+        measure_site(p.v1, p.n1, b1, sh1, sv1);
       }
       uint32_t oh0, ov0, fl0, oh1, ov1, fl1;
       if (given) {
-        const uint32_t o0 = s0.valid ? s.given_occ[s0.n] : 0u;
-        const uint32_t o1 = s1.valid ? s.given_occ[s1.n] : 0u;
+        const uint32_t o0 = p.v0 ? s.given_occ[p.n0] : 0u;
+        const uint32_t o1 = p.v1 ? s.given_occ[p.n1] : 0u;
         oh0 = o0 & 1u;
         ov0 = (o0 >> 1) & 1u;
-        fl0 = s0.valid ? s.given_flip[s0.n] & 1u : 0u;
+        fl0 = p.v0 ? s.given_flip[p.n0] & 1u : 0u;
         oh1 = o1 & 1u;
         ov1 = (o1 >> 1) & 1u;
-        fl1 = s1.valid ? s.given_flip[s1.n] & 1u : 0u;
+        fl1 = p.v1 ? s.given_flip[p.n1] & 1u : 0u;
       } else {
         const rbim_u32x4 r = rbim_sweep_draws(seed, lid, (uint32_t)g, q);
         oh0 = sh0 & ((r.x >> 1) < thr ? 1u : 0u);
@@ -286,8 +351,8 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
         fl1 = r.z & 1u;
       }
       // run pre-labelling of in-row bonds inside this wave's 128-site chunk
-      const bool internal = s1.valid && ov0 && s0.y != L - 1;
-      const bool outlink = s1.valid && ov1 && s1.y != L - 1;
+      const bool internal = p.v1 && ov0 && p.y0 != L - 1u;
+      const bool outlink = p.v1 && ov1 && p.y1 != L - 1u;
       const uint64_t thru = __ballot(internal && outlink);
       const uint64_t ol = __ballot(outlink);
       const uint64_t stop = ~thru & below;
@@ -298,15 +363,15 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
         const uint32_t j = 63u - (uint32_t)__clzll(stop);
         start0 = chunk + 2u * j + (((ol >> j) & 1ull) ? 1u : 2u);
       }
-      if (s0.valid) {
-        sb[s0.n] = (uint8_t)((b0 & 7u) | (oh0 << 3) | (ov0 << 4) | (fl0 << 5));
-        uf[s0.n] = start0;
-        cnt[s0.n] = 0u;
+      if (p.v0) {
+        sb[p.n0] = (uint8_t)((b0 & 7u) | (oh0 << 3) | (ov0 << 4) | (fl0 << 5));
+        uf[p.n0] = start0;
+        cnt[p.n0] = 0u;
       }
-      if (s1.valid) {
-        sb[s1.n] = (uint8_t)((b1 & 7u) | (oh1 << 3) | (ov1 << 4) | (fl1 << 5));
-        uf[s1.n] = internal ? start0 : s1.n;
-        cnt[s1.n] = 0u;
+      if (p.v1) {
+        sb[p.n1] = (uint8_t)((b1 & 7u) | (oh1 << 3) | (ov1 << 4) | (fl1 << 5));
+        uf[p.n1] = internal ? start0 : p.n1;
+        cnt[p.n1] = 0u;
       }
     }
     if (pending) publish();
@@ -314,40 +379,57 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
     if (pending && tid == 0) finalize(g - 1);
     // ------------------------------------------------------------ B
     uint32_t wraps = 0u;
+#pragma unroll 1
     for (int it = 0; it < iters; ++it) {
       const uint32_t q = (uint32_t)(it * NT + tid);
-      const Site s0 = make_site(2u * q, N, L, magic);
-      const Site s1 = make_site(2u * q + 1u, N, L, magic);
-      if (s0.valid) {
-        const uint32_t b0 = sb[s0.n];
-        if (b0 & 0x08u) wraps |= lds_union(uf, s0.n, s0.nr, s0.x == L - 1 ? 1 : 0, 0);
-        if ((b0 & 0x10u) && s0.y == L - 1) wraps |= lds_union(uf, s0.n, s0.nu, 0, 1);
+      const Pair p = make_pair<LC>(q, N, L, magic);
+      // up to four union requests per lane, handled one per trip of a
+      // wave-uniform loop: h bond of n0, wrapping v bond of n0, h bond of n1,
+      // v bond of n1 when it wraps or leaves the wave's chunk
+      uint32_t req = 0u;
+      if (p.v0) {
+        const uint32_t b0 = sb[p.n0];
+        req |= (b0 >> 3) & 1u;
+        req |= ((b0 >> 4) & 1u & (p.y0 == L - 1u ? 1u : 0u)) << 1;
       }
-      if (s1.valid) {
-        const uint32_t b1 = sb[s1.n];
-        if (b1 & 0x08u) wraps |= lds_union(uf, s1.n, s1.nr, s1.x == L - 1 ? 1 : 0, 0);
-        if ((b1 & 0x10u) && (s1.y == L - 1 || lane == 63))
-          wraps |= lds_union(uf, s1.n, s1.nu, 0, s1.y == L - 1 ? 1 : 0);
+      if (p.v1) {
+        const uint32_t b1 = sb[p.n1];
+        req |= ((b1 >> 3) & 1u) << 2;
+        req |= ((b1 >> 4) & 1u & ((p.y1 == L - 1u || lane == 63) ? 1u : 0u)) << 3;
+      }
+      while (__ballot(req != 0u)) {
+        if (req) {
+          const uint32_t k = (uint32_t)__builtin_ctz(req);
+          req &= req - 1u;
+          const uint32_t a = k < 2u ? p.n0 : p.n1;
+          const uint32_t b = k == 0u ? p.nr0 : k == 1u ? p.nu0 : k == 2u ? p.nr1 : p.nu1;
+          const int cx = (k == 0u && p.x0 == L - 1u) || (k == 2u && p.x1 == L - 1u) ? 1 : 0;
+          const int cy = k == 1u || (k == 3u && p.y1 == L - 1u) ? 1 : 0;
+          wraps |= lds_union(uf, a, b, cx, cy);
+        }
       }
     }
     if (wraps) atomicOr(&wrapflag[g & 1], wraps);
     __syncthreads();
     // ------------------------------------------------------------ C
+#pragma unroll 1
     for (int it = 0; it < iters; ++it) {
       const uint32_t q = (uint32_t)(it * NT + tid);
-      const uint32_t n0 = 2u * q, n1 = n0 + 1u;
-      if (n0 >= N) continue;
-      const uint32_t r0 = lds_root(uf, n0);
-      const uint32_t b0 = sb[n0];
+      const Pair p = make_pair<LC>(q, N, L, magic);
+      if (!p.v0) continue;
+      const uint32_t r0 = lds_root(uf, p.n0);
+      const uint32_t b0 = sb[p.n0];
       const uint32_t ns0 = (b0 ^ (sb[r0] >> 5)) & 1u;
-      sb[n0] = (uint8_t)((b0 & ~1u) | ns0);
-      if (s.canon_out) s.canon_out[n0] = (int32_t)r0;
-      if (n1 < N) {
-        const uint32_t r1 = lds_root(uf, n1);
-        const uint32_t b1 = sb[n1];
+      sb[p.n0] = (uint8_t)((b0 & ~1u) | ns0);
+      if (s.canon_out) s.canon_out[p.n0] = (int32_t)r0;
+      if (p.v1) {
+        // n1 shares n0's run (and root) when the in-row bond n0 -> n1 is occupied
+        const bool linked = ((b0 >> 4) & 1u) && p.y0 != L - 1u;
+        const uint32_t r1 = linked ? r0 : lds_root(uf, p.n1);
+        const uint32_t b1 = sb[p.n1];
         const uint32_t ns1 = (b1 ^ (sb[r1] >> 5)) & 1u;
-        sb[n1] = (uint8_t)((b1 & ~1u) | ns1);
-        if (s.canon_out) s.canon_out[n1] = (int32_t)r1;
+        sb[p.n1] = (uint8_t)((b1 & ~1u) | ns1);
+        if (s.canon_out) s.canon_out[p.n1] = (int32_t)r1;
         if (r1 == r0) {
           atomicAdd(&cnt[r0], 0x20000u | (ns0 + ns1));
         } else {
@@ -366,23 +448,14 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
   }
 
   if (pending) {  // measure the last sweep of this launch
+#pragma unroll 1
     for (int it = 0; it < iters; ++it) {
       const uint32_t q = (uint32_t)(it * NT + tid);
-      const Site s0 = make_site(2u * q, N, L, magic);
-      const Site s1 = make_site(2u * q + 1u, N, L, magic);
-      uint32_t b0 = 0u, b1 = 0u, sh0 = 0u, sv0 = 0u, sh1 = 0u, sv1 = 0u;
-      if (s0.valid) {
-        b0 = sb[s0.n];
-        sh0 = ~(b0 ^ sb[s0.nr] ^ (b0 >> 1)) & 1u;
-        sv0 = ~(b0 ^ sb[s0.nu] ^ (b0 >> 2)) & 1u;
-      }
-      if (s1.valid) {
-        b1 = sb[s1.n];
-        sh1 = ~(b1 ^ sb[s1.nr] ^ (b1 >> 1)) & 1u;
-        sv1 = ~(b1 ^ sb[s1.nu] ^ (b1 >> 2)) & 1u;
-      }
-      measure_site(s0, b0, sh0, sv0);
-      measure_site(s1, b1, sh1, sv1);
+      const Pair p = make_pair<LC>(q, N, L, magic);
+      uint32_t b0, b1, sh0, sv0, sh1, sv1;
+      satisfied(p, b0, b1, sh0, sv0, sh1, sv1);
+      measure_site(p.v0, p.n0, b0, sh0, sv0);
+      measure_site(p.v1, p.n1, b1, sh1, sv1);
     }
     publish();
     __syncthreads();
@@ -405,19 +478,30 @@ __global__ void init_spins_kernel(LatticeSet s, int start) {
       start == RBIM_START_COLD ? (int8_t)1 : (int8_t)rbim_init_spin(s.seed[lat], s.lat_id[lat], (uint32_t)n);
 }
 
-template <int NT>
+template <int NT, int LC>
 hipError_t launch_lds_t(const LatticeSet& s, int sweeps, hipStream_t st) {
-  const size_t bytes = lds_bytes(s.N);
   static bool configured = false;
   if (!configured) {
-    hipError_t e = hipFuncSetAttribute((const void*)lds_sweep_kernel<NT>,
+    hipError_t e = hipFuncSetAttribute((const void*)lds_sweep_kernel<NT, LC>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)lds_bytes(RBIM_LDS_MAX_N));
     if (e != hipSuccess) return e;
     configured = true;
   }
-  hipLaunchKernelGGL(lds_sweep_kernel<NT>, dim3(s.n_lat), dim3(NT), bytes, st, s, sweeps);
+  hipLaunchKernelGGL((lds_sweep_kernel<NT, LC>), dim3(s.n_lat), dim3(NT), lds_bytes(s.N), st, s,
+                     sweeps);
   return hipGetLastError();
+}
+
+template <int NT>
+hipError_t launch_lds_l(const LatticeSet& s, int sweeps, hipStream_t st) {
+  switch (s.L) {
+    case 16: return launch_lds_t<NT, 16>(s, sweeps, st);
+    case 32: return launch_lds_t<NT, 32>(s, sweeps, st);
+    case 64: return launch_lds_t<NT, 64>(s, sweeps, st);
+    case 128: return launch_lds_t<NT, 128>(s, sweeps, st);
+    default: return launch_lds_t<NT, 0>(s, sweeps, st);
+  }
 }
 
 }  // namespace
@@ -428,11 +512,11 @@ size_t lds_bytes(int N) {
 
 hipError_t launch_lds_sweeps(const LatticeSet& s, int threads, int sweeps, hipStream_t st) {
   switch (threads) {
-    case 64: return launch_lds_t<64>(s, sweeps, st);
-    case 128: return launch_lds_t<128>(s, sweeps, st);
-    case 256: return launch_lds_t<256>(s, sweeps, st);
-    case 512: return launch_lds_t<512>(s, sweeps, st);
-    case 1024: return launch_lds_t<1024>(s, sweeps, st);
+    case 64: return launch_lds_l<64>(s, sweeps, st);
+    case 128: return launch_lds_l<128>(s, sweeps, st);
+    case 256: return launch_lds_l<256>(s, sweeps, st);
+    case 512: return launch_lds_l<512>(s, sweeps, st);
+    case 1024: return launch_lds_l<1024>(s, sweeps, st);
     default: return hipErrorInvalidValue;
   }
 }
