@@ -143,6 +143,48 @@ __device__ __forceinline__ uint32_t lds_union(uint32_t* uf, uint32_t a, uint32_t
   }
 }
 
+// Winding-free forest (every word is a bare parent): roots of a and b,
+// walked together with path halving.
+__device__ __forceinline__ void find2_plain(uint32_t* uf, uint32_t& a, uint32_t& b) {
+  uint32_t pa = uf[a], pb = uf[b];
+  while (pa != a || pb != b) {
+    const uint32_t ga = pa != a ? uf[pa] : pa;
+    const uint32_t gb = pb != b ? uf[pb] : pb;
+    if (pa != a) {
+      if (ga == pa) {
+        a = pa;
+      } else {
+        uf[a] = ga;
+        a = ga;
+        pa = uf[ga];
+      }
+    }
+    if (pb != b) {
+      if (gb == pb) {
+        b = pb;
+      } else {
+        uf[b] = gb;
+        b = gb;
+        pb = uf[gb];
+      }
+    }
+  }
+}
+
+// Union of a bond that does not cross the periodic boundary, before any bond
+// that does: every tree path then lies inside the open L x L square, so all
+// windings are 0 and a loop closed here cannot wind.
+__device__ __forceinline__ void lds_union_plain(uint32_t* uf, uint32_t a, uint32_t b) {
+  while (true) {
+    find2_plain(uf, a, b);
+    if (a == b) return;
+    const uint32_t hi = max(a, b), lo = min(a, b);
+    if (atomicCAS(&uf[hi], hi, lo) == hi) return;
+    a = lo;
+    b = hi;
+  }
+}
+
 // Cluster key: size (bits 32..), ~root (16..29), n_plus (0..14).  The max key is
 // the largest cluster, ties to the smaller root; n_plus rides along.
 __device__ __forceinline__ uint64_t cluster_key(uint32_t c, uint32_t root) {
@@ -166,14 +208,15 @@ struct Pair {
   bool v0, v1;
 };
 
-template <int LC>
+// ALL: every lane of every iteration owns a full pair (known at compile time).
+template <int LC, bool ALL>
 __device__ __forceinline__ Pair make_pair(uint32_t q, uint32_t N, uint32_t Lr, uint32_t magic) {
   const uint32_t L = LC ? (uint32_t)LC : Lr;
   Pair p;
   p.n0 = 2u * q;
   p.n1 = p.n0 + 1u;
-  p.v0 = p.n0 < N;
-  p.v1 = p.n1 < N;
+  p.v0 = ALL || p.n0 < N;
+  p.v1 = ALL || p.n1 < N;
   p.x0 = LC ? p.n0 / (uint32_t)LC : site_x(p.n0, magic);
   p.y0 = p.n0 - p.x0 * L;
   if (LC != 0 && (LC & 1) == 0) {  // even L: the pair never straddles a row
@@ -194,6 +237,7 @@ __device__ __forceinline__ Pair make_pair(uint32_t q, uint32_t N, uint32_t Lr, u
 template <int NT, int LC>
 __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t sweeps) {
   constexpr int NW = NT / 64;
+  constexpr bool ALL = LC > 0 && (LC % 2) == 0 && ((LC * LC / 2) % NT) == 0;
   extern __shared__ uint32_t smem[];
   __shared__ uint32_t red_c[NW][3];  // satisfied bonds, up spins, roots
   __shared__ uint64_t red_k[NW][2];  // two largest cluster keys
@@ -324,7 +368,7 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
     for (int it = 0; it < iters; ++it) {
       const uint32_t q = (uint32_t)(it * NT + tid);
       const uint32_t chunk = 2u * (uint32_t)(it * NT + wave * 64);
-      const Pair p = make_pair<LC>(q, N, L, magic);
+      const Pair p = make_pair<LC, ALL>(q, N, L, magic);
       uint32_t b0, b1, sh0, sv0, sh1, sv1;
       satisfied(p, b0, b1, sh0, sv0, sh1, sv1);
       if (pending) {
@@ -377,37 +421,60 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
     if (pending) publish();
     __syncthreads();
     if (pending && tid == 0) finalize(g - 1);
-    // ------------------------------------------------------------ B
-    uint32_t wraps = 0u;
+    // ----------------------------------------------------------- B1
+    // Bonds that do not cross the periodic boundary: x bonds below row L-1 and
+    // the in-row y bond that leaves the wave's chunk (lane 63).  The forest
+    // has no windings yet, so plain parent chasing suffices.
 #pragma unroll 1
     for (int it = 0; it < iters; ++it) {
       const uint32_t q = (uint32_t)(it * NT + tid);
-      const Pair p = make_pair<LC>(q, N, L, magic);
-      // up to four union requests per lane, handled one per trip of a
-      // wave-uniform loop: h bond of n0, wrapping v bond of n0, h bond of n1,
-      // v bond of n1 when it wraps or leaves the wave's chunk
+      const Pair p = make_pair<LC, ALL>(q, N, L, magic);
+      // An occupied x bond (n -> n+L) is skipped when the other three bonds
+      // of the plaquette to its left -- the x bond of n-1 and the in-row y
+      // bonds n-1 -> n and n-1+L -> n+L -- are occupied: the four close a
+      // contractible loop of winding 0, so the skipped bond changes neither
+      // the clusters nor the winding of any cycle.  Skips chain along a row;
+      // by induction the endpoints of every skipped bond stay connected
+      // through kept bonds with the same displacement.
       uint32_t req = 0u;
+      uint32_t b0 = 0u;
       if (p.v0) {
-        const uint32_t b0 = sb[p.n0];
-        req |= (b0 >> 3) & 1u;
-        req |= ((b0 >> 4) & 1u & (p.y0 == L - 1u ? 1u : 0u)) << 1;
+        b0 = sb[p.n0];
+        uint32_t h0 = (b0 >> 3) & 1u & (p.x0 != L - 1u ? 1u : 0u);
+        if (h0 && p.y0 != 0u) {
+          const uint32_t left = sb[p.n0 - 1u], left_up = sb[p.nr0 - 1u];
+          h0 &= ~((left >> 3) & (left >> 4) & (left_up >> 4)) & 1u;
+        }
+        req |= h0;
       }
       if (p.v1) {
         const uint32_t b1 = sb[p.n1];
-        req |= ((b1 >> 3) & 1u) << 2;
-        req |= ((b1 >> 4) & 1u & ((p.y1 == L - 1u || lane == 63) ? 1u : 0u)) << 3;
+        uint32_t h1 = (b1 >> 3) & 1u & (p.x1 != L - 1u ? 1u : 0u);
+        if (h1 && p.y1 != 0u) h1 &= ~((b0 >> 3) & (b0 >> 4) & (sb[p.nr0] >> 4)) & 1u;
+        req |= h1 << 1;
+        req |= ((b1 >> 4) & 1u & ((lane == 63 && p.y1 != L - 1u) ? 1u : 0u)) << 2;
       }
       while (__ballot(req != 0u)) {
         if (req) {
           const uint32_t k = (uint32_t)__builtin_ctz(req);
           req &= req - 1u;
-          const uint32_t a = k < 2u ? p.n0 : p.n1;
-          const uint32_t b = k == 0u ? p.nr0 : k == 1u ? p.nu0 : k == 2u ? p.nr1 : p.nu1;
-          const int cx = (k == 0u && p.x0 == L - 1u) || (k == 2u && p.x1 == L - 1u) ? 1 : 0;
-          const int cy = k == 1u || (k == 3u && p.y1 == L - 1u) ? 1 : 0;
-          wraps |= lds_union(uf, a, b, cx, cy);
+          lds_union_plain(uf, k == 0u ? p.n0 : p.n1, k == 0u ? p.nr0 : k == 1u ? p.nr1 : p.nu1);
         }
       }
+    }
+    __syncthreads();
+    // ----------------------------------------------------------- B2
+    // The at most 2L bonds across the periodic boundary, with windings: x
+    // bonds of row L-1 and y bonds of column L-1.  A bond that finds both ends
+    // under one root closes a loop whose winding c + w(b) - w(a) decides
+    // wrap_x / wrap_y.
+    uint32_t wraps = 0u;
+    for (uint32_t t = tid; t < 2u * L; t += NT) {
+      // t < L: y bond of row t across y = L-1; t >= L: x bond of row L-1 at y = t-L
+      const bool horiz = t >= L;
+      const uint32_t a = horiz ? (L - 1u) * L + (t - L) : t * L + (L - 1u);
+      if ((sb[a] >> (horiz ? 3 : 4)) & 1u)
+        wraps |= lds_union(uf, a, horiz ? t - L : t * L, horiz ? 1 : 0, horiz ? 0 : 1);
     }
     if (wraps) atomicOr(&wrapflag[g & 1], wraps);
     __syncthreads();
@@ -415,7 +482,7 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
 #pragma unroll 1
     for (int it = 0; it < iters; ++it) {
       const uint32_t q = (uint32_t)(it * NT + tid);
-      const Pair p = make_pair<LC>(q, N, L, magic);
+      const Pair p = make_pair<LC, ALL>(q, N, L, magic);
       if (!p.v0) continue;
       const uint32_t r0 = lds_root(uf, p.n0);
       const uint32_t b0 = sb[p.n0];
@@ -451,7 +518,7 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
 #pragma unroll 1
     for (int it = 0; it < iters; ++it) {
       const uint32_t q = (uint32_t)(it * NT + tid);
-      const Pair p = make_pair<LC>(q, N, L, magic);
+      const Pair p = make_pair<LC, ALL>(q, N, L, magic);
       uint32_t b0, b1, sh0, sv0, sh1, sv1;
       satisfied(p, b0, b1, sh0, sv0, sh1, sv1);
       measure_site(p.v0, p.n0, b0, sh0, sv0);
