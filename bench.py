@@ -96,6 +96,19 @@ def gpu_plain(spec, K):
     return spec.L * spec.L * K / (ms / 1e3)
 
 
+def profiled_traffic(L, n_lat, sweeps_per_launch, threads):
+    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs of this same default command (calibrated
+    against a plain device copy); None when the configuration differs."""
+    path = os.path.join(ROOT, "profiles", "lds_sweep_kernel_L32_batched.md")
+    if (L, n_lat, sweeps_per_launch, threads) != (32, 1280, 2000, 256) or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def cpu_baseline(specs, sweeps, per_thread):
     """Oracle C restatement, one lattice per thread on every host core."""
     from concurrent.futures import ThreadPoolExecutor
@@ -209,7 +222,7 @@ def main():
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBS,
-            "traffic": None,
+            "traffic": profiled_traffic(args.L, n_lat, res["sweeps_per_launch"], res["threads"]),
             "kernel": "lds_sweep_kernel",
             "avg_launch_ms": avg_launch_ms,
             "launches": launches,
