@@ -86,8 +86,8 @@ size_t group_bytes(int L, int n_lat, bool lds, int n_bins) {
   size_t b = N * n_lat * 2 + (size_t)n_lat * (4 + 8 + 4 + 8 + sizeof(rbim_sweep_stats)) +
              (size_t)n_lat * n_bins * RBIM_ACC_SLOTS * 8;
   if (!lds) {
-    const size_t tx = (L + RBIM_TILE - 1) / RBIM_TILE;
-    b += N * n_lat * (1 + 8 + 8 + 8) + (size_t)n_lat * tx * tx * 5 * 8 + (size_t)n_lat * 4;
+    const size_t tx = (L + 31) / 32;
+    b += N * n_lat * (1 + 2 + 8 + 4 + 8) + (size_t)n_lat * tx * tx * 5 * 8 + (size_t)n_lat * 4;
   }
   return b;
 }
@@ -144,10 +144,12 @@ int group_alloc(rbim_ctx* ctx, Group& g, const std::vector<uint8_t>& jbits,
   HIP_TRY(ctx, hipMemsetAsync(p, 0, sizeof(rbim_sweep_stats) * g.n_lat, ctx->stream));
   if (!g.lds) {
     TiledScratch& t = g.tile;
-    t.tiles_x = t.tiles_y = (g.L + RBIM_TILE - 1) / RBIM_TILE;
+    t.tile = tiled_tile_size(g.L);
+    t.tiles_x = t.tiles_y = (g.L + t.tile - 1) / t.tile;
     HIP_TRY(ctx, g.arena.alloc(&p, NN)); t.occ = (uint8_t*)p;
-    HIP_TRY(ctx, g.arena.alloc(&p, NN * 8)); t.uf = (uint64_t*)p;
-    HIP_TRY(ctx, g.arena.alloc(&p, NN * 8)); t.lst = (uint64_t*)p;
+    HIP_TRY(ctx, g.arena.alloc(&p, NN * 2)); t.lroot = (uint16_t*)p;
+    HIP_TRY(ctx, g.arena.alloc(&p, NN * 8)); t.gf = (uint64_t*)p;
+    HIP_TRY(ctx, g.arena.alloc(&p, NN * 4)); t.lst = (uint32_t*)p;
     HIP_TRY(ctx, g.arena.alloc(&p, NN * 8)); t.st = (uint64_t*)p;
     const size_t pb = (size_t)g.n_lat * t.tiles_x * t.tiles_y * 5 * 8;
     HIP_TRY(ctx, g.arena.alloc(&p, pb)); t.part = (int64_t*)p;

@@ -9,9 +9,6 @@
 // Largest lattice the LDS-resident kernel holds: 14-bit parent field.
 #define RBIM_LDS_MAX_L 128
 #define RBIM_LDS_MAX_N (RBIM_LDS_MAX_L * RBIM_LDS_MAX_L)
-// Tiled path: 32 x 32 site tiles, 256 threads per tile workgroup.
-#define RBIM_TILE 32
-#define RBIM_TILE_THREADS 256
 
 // Per-lattice state common to both engines (device pointers).
 struct LatticeSet {
@@ -32,14 +29,18 @@ struct LatticeSet {
   int32_t* canon_out;       // [N] min-site label of each site
 };
 
-// Tiled-path scratch (device pointers), [n_lat][N] unless noted.
+// Tiled-path scratch (device pointers), [n_lat][N] unless noted.  Only
+// occ and lroot are written for every site; gf, lst and st are used at
+// tile-local roots only.
 struct TiledScratch {
   uint8_t* occ;        // bit0 occ_h, bit1 occ_v, bit2 flip
-  uint64_t* uf;        // parent (low 32) | wx:16 | wy:16 (high 32)
-  uint64_t* lst;       // local-cluster (size << 32 | n_plus before flip) at local roots
-  uint64_t* st;        // global-cluster (size << 32 | n_plus after flip) at roots
+  uint16_t* lroot;     // tile-local root of every site, as a tile index lx * 32 + ly
+  uint64_t* gf;        // forest over local roots: parent (low 32) | wx:16 | wy:16
+  uint32_t* lst;       // local cluster: size << 16 | n_plus before the flip
+  uint64_t* st;        // cluster: size << 32 | n_plus after the flip, at roots
   int64_t* part;       // [n_lat][n_tiles][5] partial reductions
   uint32_t* wrapf;     // [n_lat] wrap flags
+  int32_t tile;        // tile side: 64 when L is a multiple of 64 (L >= 128), else 32
   int32_t tiles_x, tiles_y;
 };
 
@@ -56,6 +57,7 @@ hipError_t launch_cluster_stats(const int32_t* labels, const int8_t* spins, int6
 hipError_t launch_energy_mag(const int8_t* spins, const uint8_t* jbits, int32_t L,
                              unsigned long long* em, hipStream_t st);
 int tiled_kernels_per_sweep();
+int tiled_tile_size(int L);
 
 // ---------------------------------------------------------------- helpers
 __device__ __forceinline__ uint32_t site_x(uint32_t n, uint32_t magic) {

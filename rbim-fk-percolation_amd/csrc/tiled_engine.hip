@@ -5,31 +5,33 @@
 //   K1 tile      32x32 tile per workgroup: Philox activation of every site,
 //                occupation/flip bits to HBM, union-find of the bonds inside
 //                the tile in LDS (no winding: a tile never crosses the periodic
-//                boundary, so every path inside it has winding 0), each site
-//                pointed at its tile-local root (= min site of its tile-local
-//                cluster), local sizes / n_plus recorded at local roots.
+//                boundary, so every path inside it has winding 0).  Every site
+//                gets its tile-local root (= min site of its tile-local
+//                cluster); each local root opens a node of the HBM forest and
+//                records its local size / n_plus.
 //   K2 merge     one thread per tile-boundary bond (including the periodic
-//                ones): union in HBM with 64-bit CAS on (parent | wx | wy),
-//                larger root hooked under smaller; a bond that finds both ends
-//                under one root is a non-tree bond and its loop winding decides
-//                wrap_x / wrap_y (_unionfind.py:101-124).  Bonds inside a tile
-//                close only windingless loops and need no test.
-//   K3 flip      per site: global root, s ^= flipbit(root); each local root
-//                adds its local cluster's (size, n_plus after flip) to its
-//                global root -- one atomic per local cluster, not per site.
+//                ones): union of the two local roots in the HBM forest with a
+//                64-bit CAS on (parent | wx | wy), larger root hooked under
+//                smaller; a bond that finds both ends under one root is a
+//                non-tree bond and its loop winding decides wrap_x / wrap_y
+//                (_unionfind.py:101-124).  Bonds inside a tile close only
+//                windingless loops and need no test.
+//   K3 flip      per site: global root of its local root, s ^= flipbit(root);
+//                each local root adds its local cluster's (size, n_plus after
+//                the flip) to its global root -- one atomic per local cluster.
 //   K4 measure   per tile: E, M, n_clusters, two largest clusters (64-bit keys
 //                size << 32 | ~root) -> per-tile partials.
 //   K5 finalize  per lattice: fold the partials, fold the sweep into its bin's
 //                moment sums (simulation.py:212-236), advance the sweep counter.
 //
-// Every kernel reads the lattice's own sweep counter from HBM, so a captured
-// graph of sweeps replays without host arguments changing.
+// Per site and sweep only the occupation byte and the 4-byte local root are
+// written; the forest, the local counts and the cluster accumulators live at
+// local roots.  Every kernel reads the lattice's own sweep counter from HBM,
+// so a captured graph of sweeps replays without host arguments changing.
 #include "engine.h"
 
 namespace {
 
-constexpr int T = RBIM_TILE;
-constexpr int NTH = RBIM_TILE_THREADS;
 constexpr int PARTS = 5;
 
 __device__ __forceinline__ uint64_t gpack(uint32_t p, int wx, int wy) {
@@ -47,7 +49,8 @@ __device__ __forceinline__ void gstore(uint64_t* p, uint64_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Root and tree-path winding in the HBM forest, with path halving.
+// Root and tree-path winding in the HBM forest while hooks are in flight (K2):
+// agent-scope loads so a retry sees other workgroups' hooks; path halving.
 __device__ uint32_t gfind(uint64_t* uf, uint32_t x, int& wx, int& wy) {
   wx = 0;
   wy = 0;
@@ -69,6 +72,16 @@ __device__ uint32_t gfind(uint64_t* uf, uint32_t x, int& wx, int& wy) {
     x = gp;
     v = gload(&uf[gp]);
   }
+}
+
+// Root in the settled forest (K3): plain cached loads, no writes.
+__device__ __forceinline__ uint32_t groot(const uint64_t* gf, uint32_t x) {
+  uint32_t p = gpar(gf[x]);
+  while (p != x) {
+    x = p;
+    p = gpar(gf[x]);
+  }
+  return x;
 }
 
 __device__ __forceinline__ bool lattice_active(const LatticeSet& s, int lat) {
@@ -97,6 +110,7 @@ __device__ __forceinline__ void lunion(uint32_t* par, uint32_t a, uint32_t b) {
   }
 }
 
+template <int T, int NTH>
 __global__ __launch_bounds__(NTH) void k1_tile(LatticeSet s, TiledScratch t) {
   __shared__ uint32_t par[T * T];
   __shared__ uint32_t lcnt[T * T];
@@ -172,22 +186,205 @@ __global__ __launch_bounds__(NTH) void k1_tile(LatticeSet s, TiledScratch t) {
     if (lx >= w || ly >= h) continue;
     const uint32_t r = lfind(par, li);
     atomicAdd(&lcnt[r], 0x10000u | (bits[li] & 1u));
-    const uint32_t rg = (uint32_t)((x0 + (int)(r / T)) * L + y0 + (int)(r % T));
     const uint32_t n = (uint32_t)((x0 + lx) * L + y0 + ly);
-    t.uf[base + n] = gpack(rg, 0, 0);
+    t.lroot[base + n] = (uint16_t)r;
   }
   __syncthreads();
   for (int li = threadIdx.x; li < T * T; li += NTH) {
     const int lx = li / T, ly = li % T;
-    if (lx >= w || ly >= h) continue;
+    if (lx >= w || ly >= h || par[li] != (uint32_t)li) continue;
     const uint32_t n = (uint32_t)((x0 + lx) * L + y0 + ly);
-    const uint32_t c = (par[li] == (uint32_t)li) ? lcnt[li] : 0u;
-    t.lst[base + n] = ((uint64_t)(c >> 16) << 32) | (uint64_t)(c & 0xFFFFu);
+    t.gf[base + n] = gpack(n, 0, 0);
+    t.lst[base + n] = lcnt[li];
+    t.st[base + n] = 0ull;
+  }
+}
+
+// Full 32x32 tiles (L a multiple of 32): the LDS kernel's scheme on an open
+// tile.  Each lane owns a site pair (one Philox call); occupied in-row bonds
+// inside a wavefront's 128-site chunk (four tile rows) are merged by two
+// ballots (every site points at the start of its run); the remaining in-tile
+// bonds go through a winding-free union-find in LDS, skipping any x bond that
+// closes an occupied plaquette with its left neighbour (a contractible loop:
+// connectivity and windings are unchanged).
+__device__ __forceinline__ void tfind2(uint32_t* par, uint32_t& a, uint32_t& b) {
+  uint32_t pa = par[a], pb = par[b];
+  while (pa != a || pb != b) {
+    const uint32_t ga = pa != a ? par[pa] : pa;
+    const uint32_t gb = pb != b ? par[pb] : pb;
+    if (pa != a) {
+      if (ga == pa) {
+        a = pa;
+      } else {
+        par[a] = ga;
+        a = ga;
+        pa = par[ga];
+      }
+    }
+    if (pb != b) {
+      if (gb == pb) {
+        b = pb;
+      } else {
+        par[b] = gb;
+        b = gb;
+        pb = par[gb];
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void tunion(uint32_t* par, uint32_t a, uint32_t b) {
+  while (true) {
+    tfind2(par, a, b);
+    if (a == b) return;
+    const uint32_t hi = max(a, b), lo = min(a, b);
+    if (atomicCAS(&par[hi], hi, lo) == hi) return;
+    a = lo;
+    b = hi;
+  }
+}
+
+__device__ __forceinline__ uint32_t troot(const uint32_t* par, uint32_t x) {
+  uint32_t p = par[x];
+  while (p != x) {
+    const uint32_t g = par[p];
+    if (g == p) return p;
+    x = g;
+    p = par[x];
+  }
+  return x;
+}
+
+template <int T, int NTH>
+__global__ __launch_bounds__(NTH) void k1_tile_full(LatticeSet s, TiledScratch t) {
+  __shared__ uint32_t par[T * T];
+  __shared__ uint32_t lcnt[T * T];
+  __shared__ uint8_t bits[T * T];  // bit0 spin, bit1 occ_h, bit2 occ_v
+  const int lat = blockIdx.y;
+  if (!lattice_active(s, lat)) return;
+  const uint32_t L = (uint32_t)s.L, N = (uint32_t)s.N;
+  const uint32_t tx = blockIdx.x % t.tiles_x, ty = blockIdx.x / t.tiles_x;
+  const uint32_t x0 = tx * T, y0 = ty * T;
+  const size_t base = (size_t)lat * N;
+  const int8_t* sp = s.spins + base;
+  const uint8_t* jb = s.jbits + base;
+  const uint32_t sweep = (uint32_t)s.sweep_ctr[lat];
+  const uint32_t thr = s.thr31[lat];
+  const uint64_t seed = s.seed[lat];
+  const uint32_t lid = s.lat_id[lat];
+  const bool given = s.given_occ != nullptr;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint64_t below = (1ull << lane) - 1ull;
+  constexpr int ITERS = T * T / 2 / NTH;
+
+#pragma unroll 1
+  for (int it = 0; it < ITERS; ++it) {
+    const uint32_t ql = (uint32_t)(it * NTH + tid);
+    const uint32_t lx = ql / (T / 2), ly0 = (ql % (T / 2)) * 2u;
+    const uint32_t li0 = ql << 1;
+    const uint32_t x = x0 + lx, y = y0 + ly0;
+    const uint32_t n0 = x * L + y, n1 = n0 + 1u;
+    const uint32_t nr0 = (x == L - 1u) ? y : n0 + L;
+    const uint32_t nu1 = (y + 1u == L - 1u) ? n1 - (y + 1u) : n1 + 1u;
+    const int s0 = sp[n0], s1 = sp[n1];
+    uint32_t oh0, ov0, fl0, oh1, ov1, fl1;
+    if (given) {
+      const uint32_t o0 = s.given_occ[n0], o1 = s.given_occ[n1];
+      oh0 = o0 & 1u;
+      ov0 = (o0 >> 1) & 1u;
+      fl0 = s.given_flip[n0] & 1u;
+      oh1 = o1 & 1u;
+      ov1 = (o1 >> 1) & 1u;
+      fl1 = s.given_flip[n1] & 1u;
+    } else {
+      const uint32_t j0 = jb[n0], j1 = jb[n1];
+      const int sr0 = sp[nr0], sr1 = sp[nr0 + 1u], su1 = sp[nu1];
+      const rbim_u32x4 r = rbim_sweep_draws(seed, lid, sweep, n0 >> 1);
+      oh0 = (((j0 & 1u) ? -1 : 1) * s0 * sr0 > 0) && ((r.x >> 1) < thr) ? 1u : 0u;
+      ov0 = (((j0 & 2u) ? -1 : 1) * s0 * s1 > 0) && ((r.y >> 1) < thr) ? 1u : 0u;
+      oh1 = (((j1 & 1u) ? -1 : 1) * s1 * sr1 > 0) && ((r.z >> 1) < thr) ? 1u : 0u;
+      ov1 = (((j1 & 2u) ? -1 : 1) * s1 * su1 > 0) && ((r.w >> 1) < thr) ? 1u : 0u;
+      fl0 = r.x & 1u;
+      fl1 = r.z & 1u;
+    }
+    t.occ[base + n0] = (uint8_t)(oh0 | (ov0 << 1) | (fl0 << 2));
+    t.occ[base + n1] = (uint8_t)(oh1 | (ov1 << 1) | (fl1 << 2));
+    bits[li0] = (uint8_t)((s0 > 0 ? 1u : 0u) | (oh0 << 1) | (ov0 << 2));
+    bits[li0 + 1u] = (uint8_t)((s1 > 0 ? 1u : 0u) | (oh1 << 1) | (ov1 << 2));
+    const bool internal = ov0 != 0u;                       // n0 -> n1, same row and tile
+    const bool outlink = ov1 != 0u && ly0 + 1u != T - 1u;  // n1 -> n1+1 inside the tile
+    const uint64_t thru = __ballot(internal && outlink);
+    const uint64_t ol = __ballot(outlink);
+    const uint64_t stop = ~thru & below;
+    const uint32_t chunk = 2u * (uint32_t)(it * NTH + wave * 64);
+    uint32_t start0 = chunk;
+    if (stop != 0ull) {
+      const uint32_t j = 63u - (uint32_t)__clzll(stop);
+      start0 = chunk + 2u * j + (((ol >> j) & 1ull) ? 1u : 2u);
+    }
+    par[li0] = start0;
+    par[li0 + 1u] = internal ? start0 : li0 + 1u;
+    lcnt[li0] = 0u;
+    lcnt[li0 + 1u] = 0u;
+  }
+  __syncthreads();
+#pragma unroll 1
+  for (int it = 0; it < ITERS; ++it) {
+    const uint32_t ql = (uint32_t)(it * NTH + tid);
+    const uint32_t lx = ql / (T / 2), ly0 = (ql % (T / 2)) * 2u;
+    const uint32_t li0 = ql << 1, li1 = li0 + 1u;
+    const uint32_t b0 = bits[li0], b1 = bits[li1];
+    uint32_t req = 0u;
+    if (lx + 1u < T) {
+      uint32_t h0 = (b0 >> 1) & 1u;
+      if (h0 && ly0 != 0u) {
+        const uint32_t left = bits[li0 - 1u], left_up = bits[li0 - 1u + T];
+        h0 &= ~((left >> 1) & (left >> 2) & (left_up >> 2)) & 1u;
+      }
+      uint32_t h1 = (b1 >> 1) & 1u;
+      if (h1) h1 &= ~((b0 >> 1) & (b0 >> 2) & (bits[li0 + T] >> 2)) & 1u;
+      req = h0 | (h1 << 1);
+    }
+    req |= ((b1 >> 2) & 1u & ((lane == 63 && ly0 + 1u != T - 1u) ? 1u : 0u)) << 2;
+    while (__ballot(req != 0u)) {
+      if (req) {
+        const uint32_t k = (uint32_t)__builtin_ctz(req);
+        req &= req - 1u;
+        tunion(par, k == 0u ? li0 : li1, k == 0u ? li0 + T : k == 1u ? li1 + T : li1 + 1u);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll 1
+  for (int it = 0; it < ITERS; ++it) {
+    const uint32_t ql = (uint32_t)(it * NTH + tid);
+    const uint32_t lx = ql / (T / 2), ly0 = (ql % (T / 2)) * 2u;
+    const uint32_t li0 = ql << 1, li1 = li0 + 1u;
+    const uint32_t b0 = bits[li0], b1 = bits[li1];
+    const uint32_t r0 = troot(par, li0);
+    const uint32_t r1 = (b0 & 4u) ? r0 : troot(par, li1);
+    if (r0 == r1) {
+      atomicAdd(&lcnt[r0], 0x20000u | ((b0 & 1u) + (b1 & 1u)));
+    } else {
+      atomicAdd(&lcnt[r0], 0x10000u | (b0 & 1u));
+      atomicAdd(&lcnt[r1], 0x10000u | (b1 & 1u));
+    }
+    const uint32_t n0 = (x0 + lx) * L + y0 + ly0;
+    t.lroot[base + n0] = (uint16_t)r0;
+    t.lroot[base + n0 + 1u] = (uint16_t)r1;
+  }
+  __syncthreads();
+  for (uint32_t li = tid; li < T * T; li += NTH) {
+    if (par[li] != li) continue;
+    const uint32_t n = (x0 + li / T) * L + y0 + li % T;
+    t.gf[base + n] = gpack(n, 0, 0);
+    t.lst[base + n] = lcnt[li];
     t.st[base + n] = 0ull;
   }
 }
 
 // ----------------------------------------------------------------- K2
+template <int T>
 __global__ void k2_merge(LatticeSet s, TiledScratch t) {
   const int lat = blockIdx.y;
   if (!lattice_active(s, lat)) return;
@@ -213,11 +410,19 @@ __global__ void k2_merge(LatticeSet s, TiledScratch t) {
   const uint32_t b = horiz ? ((x == L - 1) ? (uint32_t)y : a + L) : ((y == L - 1) ? a - y : a + 1);
   const int cx = (horiz && x == L - 1) ? 1 : 0;
   const int cy = (!horiz && y == L - 1) ? 1 : 0;
-  uint64_t* uf = t.uf + base;
+  uint64_t* uf = t.gf + base;
+  // a site -> its local root has winding 0 (inside one tile)
+  auto local_root = [&](uint32_t site, uint32_t sx, uint32_t sy) {
+    const uint32_t r = t.lroot[base + site];
+    return ((sx & ~(uint32_t)(T - 1)) + (r / T)) * (uint32_t)L + (sy & ~(uint32_t)(T - 1)) + (r % T);
+  };
+  const uint32_t bxs = horiz ? (x == L - 1 ? 0u : (uint32_t)x + 1u) : (uint32_t)x;
+  const uint32_t bys = horiz ? (uint32_t)y : (y == L - 1 ? 0u : (uint32_t)y + 1u);
+  const uint32_t la = local_root(a, (uint32_t)x, (uint32_t)y), lb = local_root(b, bxs, bys);
   while (true) {
     int ax, ay, bx, by;
-    const uint32_t ra = gfind(uf, a, ax, ay);
-    const uint32_t rb = gfind(uf, b, bx, by);
+    const uint32_t ra = gfind(uf, la, ax, ay);
+    const uint32_t rb = gfind(uf, lb, bx, by);
     if (ra == rb) {
       const uint32_t wbits = ((cx + bx - ax) != 0 ? 1u : 0u) | ((cy + by - ay) != 0 ? 2u : 0u);
       if (wbits) atomicOr(&t.wrapf[lat], wbits);
@@ -236,26 +441,50 @@ __global__ void k2_merge(LatticeSet s, TiledScratch t) {
 }
 
 // ----------------------------------------------------------------- K3
-__global__ void k3_flip(LatticeSet s, TiledScratch t) {
+// One workgroup per tile: each tile-local root resolves its global root and
+// that root's flip bit once (into an LDS table); every site then flips from
+// the table, and each local root adds its counts to the global root.
+template <int T, int NTH>
+__global__ __launch_bounds__(NTH) void k3_flip(LatticeSet s, TiledScratch t) {
+  __shared__ uint32_t rmap[T * T];  // global root | flip << 31, at local roots
+  constexpr int PER = T * T / NTH;
   const int lat = blockIdx.y;
   if (!lattice_active(s, lat)) return;
-  const int N = s.N;
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
+  const uint32_t L = (uint32_t)s.L, N = (uint32_t)s.N;
+  const uint32_t tx = blockIdx.x % t.tiles_x, ty = blockIdx.x / t.tiles_x;
+  const uint32_t x0 = tx * T, y0 = ty * T;
+  const uint32_t w = min((uint32_t)T, L - x0), h = min((uint32_t)T, L - y0);
   const size_t base = (size_t)lat * N;
-  uint64_t* uf = t.uf + base;
-  int wx, wy;
-  const uint32_t r = gfind(uf, (uint32_t)n, wx, wy);
-  const uint32_t fl = (t.occ[base + r] >> 2) & 1u;
-  const int8_t old = s.spins[base + n];
-  if (fl) s.spins[base + n] = (int8_t)-old;
-  const uint64_t lc = t.lst[base + n];
-  if (lc != 0ull) {
-    const uint64_t size = lc >> 32, np = lc & 0xFFFFFFFFull;
-    const uint64_t np_new = fl ? size - np : np;
-    atomicAdd((unsigned long long*)&t.st[base + r], (unsigned long long)((size << 32) | np_new));
+  uint32_t lr[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const uint32_t li = (uint32_t)(k * NTH + threadIdx.x), lx = li / T, ly = li % T;
+    lr[k] = 0xFFFFFFFFu;
+    if (lx >= w || ly >= h) continue;
+    const uint32_t n = (x0 + lx) * L + y0 + ly;
+    lr[k] = t.lroot[base + n];
+    if (lr[k] == li) {
+      const uint32_t r = groot(t.gf + base, n);
+      rmap[li] = r | (((uint32_t)t.occ[base + r] >> 2) & 1u) << 31;
+    }
   }
-  if (s.canon_out) s.canon_out[n] = (int32_t)r;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    if (lr[k] == 0xFFFFFFFFu) continue;
+    const uint32_t li = (uint32_t)(k * NTH + threadIdx.x), lx = li / T, ly = li % T;
+    const uint32_t n = (x0 + lx) * L + y0 + ly;
+    const uint32_t m = rmap[lr[k]];
+    const uint32_t r = m & 0x7FFFFFFFu, fl = m >> 31;
+    if (fl) s.spins[base + n] = (int8_t)-s.spins[base + n];
+    if (lr[k] == li) {
+      const uint32_t lc = t.lst[base + n];
+      const uint64_t size = lc >> 16, np = lc & 0xFFFFu;
+      const uint64_t np_new = fl ? size - np : np;
+      atomicAdd((unsigned long long*)&t.st[base + r], (unsigned long long)((size << 32) | np_new));
+    }
+    if (s.canon_out) s.canon_out[n] = (int32_t)r;
+  }
 }
 
 // ----------------------------------------------------------------- K4
@@ -266,6 +495,7 @@ __device__ __forceinline__ void top2_merge64(uint64_t& k1, uint64_t& k2, uint64_
   k2 = lo;
 }
 
+template <int T, int NTH>
 __global__ __launch_bounds__(NTH) void k4_measure(LatticeSet s, TiledScratch t) {
   __shared__ int64_t red[NTH / 64][PARTS];
   const int lat = blockIdx.y;
@@ -292,7 +522,7 @@ __global__ __launch_bounds__(NTH) void k4_measure(LatticeSet s, TiledScratch t) 
     e -= (int64_t)((j & 1u) ? -1 : 1) * s0 * sp[nr];
     e -= (int64_t)((j & 2u) ? -1 : 1) * s0 * sp[nu];
     m += s0;
-    if (gpar(t.uf[base + n]) == n) {
+    if (t.lroot[base + n] == (uint32_t)li && gpar(t.gf[base + n]) == n) {
       ++ncl;
       const uint64_t size = t.st[base + n] >> 32;
       top2_merge64(k1, k2, (size << 32) | (uint64_t)(0xFFFFFFFFu - n), 0ull);
@@ -445,15 +675,28 @@ __global__ void energy_mag_kernel(const int8_t* spins, const uint8_t* jb, int32_
 
 int tiled_kernels_per_sweep() { return 5; }
 
-hipError_t launch_tiled_sweep(const LatticeSet& s, const TiledScratch& t, hipStream_t st) {
+int tiled_tile_size(int L) {
+  return (L % 64 == 0 && L >= 128) ? 64 : 32;
+}
+
+template <int T, int NTH>
+hipError_t launch_tiled_t(const LatticeSet& s, const TiledScratch& t, hipStream_t st) {
   const int ntiles = t.tiles_x * t.tiles_y;
-  hipLaunchKernelGGL(k1_tile, dim3(ntiles, s.n_lat), dim3(NTH), 0, st, s, t);
+  if (s.L % T == 0)
+    hipLaunchKernelGGL((k1_tile_full<T, NTH>), dim3(ntiles, s.n_lat), dim3(NTH), 0, st, s, t);
+  else
+    hipLaunchKernelGGL((k1_tile<T, NTH>), dim3(ntiles, s.n_lat), dim3(NTH), 0, st, s, t);
   const int nb = (t.tiles_x + t.tiles_y) * s.L;
-  hipLaunchKernelGGL(k2_merge, dim3((nb + 255) / 256, s.n_lat), dim3(256), 0, st, s, t);
-  hipLaunchKernelGGL(k3_flip, dim3((s.N + 255) / 256, s.n_lat), dim3(256), 0, st, s, t);
-  hipLaunchKernelGGL(k4_measure, dim3(ntiles, s.n_lat), dim3(NTH), 0, st, s, t);
+  hipLaunchKernelGGL((k2_merge<T>), dim3((nb + 255) / 256, s.n_lat), dim3(256), 0, st, s, t);
+  hipLaunchKernelGGL((k3_flip<T, NTH>), dim3(ntiles, s.n_lat), dim3(NTH), 0, st, s, t);
+  hipLaunchKernelGGL((k4_measure<T, NTH>), dim3(ntiles, s.n_lat), dim3(NTH), 0, st, s, t);
   hipLaunchKernelGGL(k5_finalize, dim3(s.n_lat), dim3(64), 0, st, s, t);
   return hipGetLastError();
+}
+
+hipError_t launch_tiled_sweep(const LatticeSet& s, const TiledScratch& t, hipStream_t st) {
+  if (t.tile == 64) return launch_tiled_t<64, 512>(s, t, st);
+  return launch_tiled_t<32, 256>(s, t, st);
 }
 
 hipError_t launch_label_rank(const int32_t* canon, int32_t N, int32_t* labels, int32_t* rank_tmp,

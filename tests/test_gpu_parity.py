@@ -97,15 +97,17 @@ def test_label_known_answers(hip, engine):
 def test_label_and_wrap_large_lattices(hip, coracle):
     """Tiled path with several tiles per side, partial edge tiles and odd L."""
     checked = 0
-    for s in sample_configurations(coracle, (33, 47, 100, 129, 256), (0.0, 0.12), (1.5, 2.27, 3.0), 2,
-                                   seed=77):
+    # tile sides: 32 with partial edge tiles (33, 47, 100, 129), 32 full (96),
+    # 64 full (128, 192, 256)
+    for s in sample_configurations(coracle, (33, 47, 96, 100, 128, 129, 192, 256), (0.0, 0.12),
+                                   (1.5, 2.27, 3.0), 2, seed=77):
         L = s["L"]
         n_ref, lab_ref, wx_ref, wy_ref = coracle.c_label_and_wrap(s["active"], L)
         n, labels, canon, wx, wy = gpu_label(hip, s["active"], L, PLAIN)
         assert (n, wx, wy) == (n_ref, wx_ref, wy_ref), f"L={L}"
         assert np.array_equal(labels, lab_ref), f"L={L}"
         checked += 1
-    assert checked == 60
+    assert checked == 96
 
 
 def test_activation_bit_exact(hip, coracle):
