@@ -16,6 +16,7 @@
 struct rbim_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  std::mutex mu;  // guards err and the context stream (batches are created from several threads)
   std::string err;
 };
 
@@ -30,7 +31,10 @@ int set_err(rbim_ctx* ctx, int code, const char* fmt, ...) {
   va_start(ap, fmt);
   vsnprintf(buf, sizeof buf, fmt, ap);
   va_end(ap);
-  if (ctx) ctx->err = buf;
+  if (ctx) {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->err = buf;
+  }
   std::lock_guard<std::mutex> lk(g_err_mu);
   g_err = buf;
   return code;
@@ -324,10 +328,14 @@ extern "C" {
 int rbim_abi_version(void) { return RBIM_ABI_VERSION; }
 
 const char* rbim_last_error(const rbim_ctx* ctx) {
-  if (ctx) return ctx->err.c_str();
-  std::lock_guard<std::mutex> lk(g_err_mu);
   static thread_local std::string copy;
-  copy = g_err;
+  if (ctx) {
+    std::lock_guard<std::mutex> lk(const_cast<rbim_ctx*>(ctx)->mu);
+    copy = ctx->err;
+  } else {
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    copy = g_err;
+  }
   return copy.c_str();
 }
 

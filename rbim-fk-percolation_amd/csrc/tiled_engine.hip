@@ -74,13 +74,17 @@ __device__ uint32_t gfind(uint64_t* uf, uint32_t x, int& wx, int& wy) {
   }
 }
 
-// Root in the settled forest (K3): plain cached loads, no writes.
-__device__ __forceinline__ uint32_t groot(const uint64_t* gf, uint32_t x) {
+// Root in the settled forest (K3), then the start node is pointed straight at
+// it.  Windings are no longer needed once K2 has decided wrapping, and a
+// concurrent reader sees either the old ancestor chain or the root.
+__device__ __forceinline__ uint32_t groot(uint64_t* gf, uint32_t x) {
+  const uint32_t start = x;
   uint32_t p = gpar(gf[x]);
   while (p != x) {
     x = p;
     p = gpar(gf[x]);
   }
+  if (x != start) gf[start] = gpack(x, 0, 0);
   return x;
 }
 

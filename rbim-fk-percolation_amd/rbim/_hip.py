@@ -29,6 +29,7 @@ __all__ = [
     "lib",
     "context",
     "check",
+    "device_memory",
 ]
 
 ACC_SLOTS = 20
@@ -192,6 +193,14 @@ def context():
                 raise EngineUnavailable(status, _message(None))
             _ctx = ctx
         return _ctx
+
+
+def device_memory():
+    """(free, total) bytes of the context's GPU."""
+    ctx = context()
+    free, total = ctypes.c_uint64(), ctypes.c_uint64()
+    check(lib().rbim_device_memory(ctx, ctypes.byref(free), ctypes.byref(total)), ctx)
+    return int(free.value), int(total.value)
 
 
 def ptr(array: Optional[np.ndarray]):

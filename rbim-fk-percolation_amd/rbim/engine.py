@@ -134,6 +134,13 @@ class Batch:
             self._spins0 = np.ascontiguousarray(np.concatenate([
                 (np.asarray(s.spins0, np.int8).reshape(-1) if s.spins0 is not None
                  else np.ones(s.L * s.L, np.int8)) for s in lattices]))
+        if threads == 0 and sweeps_per_launch == 0 and len(set(self.Ls.tolist())) == 1:
+            from rbim import autotune
+
+            tuned = autotune.lookup(int(self.Ls[0]), self.n)
+            if tuned:
+                threads, sweeps_per_launch = int(tuned["threads"]), int(tuned["sweeps_per_launch"])
+        self.threads, self.sweeps_per_launch = int(threads), int(sweeps_per_launch)
         desc = _hip.BatchDesc()
         desc.n_lattices = self.n
         desc.L = _hip.ptr(self.Ls)

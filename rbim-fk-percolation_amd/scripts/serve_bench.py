@@ -1,0 +1,61 @@
+#!/usr/bin/env python3
+"""Serving benchmark: a Poisson stream of temperature-scan requests through the
+continuous-batching ScanServer, reporting p50 / p99 latency and site-updates/s.
+
+Default stream: mean 20 requests/s for 5 s, L from {32, 64, 128, 256},
+p ~ U[0, 0.12], 8-40 temperatures in [1, 3], 4-32 realizations,
+1000-20000 sweeps, each request's sweep count multiplied by --sweep-scale
+(the full-size stream offers far more work than one GPU can absorb in real
+time, so a scaled stream is what measures latency rather than queue growth).
+
+    python rbim-fk-percolation_amd/scripts/serve_bench.py --rate 20 --duration 5 --sweep-scale 0.02
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+
+from rbim.serving import ScanServer, poisson_requests  # noqa: E402
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--rate", type=float, default=20.0, help="mean arrivals per second")
+    ap.add_argument("--duration", type=float, default=5.0, help="arrival window in seconds")
+    ap.add_argument("--sweep-scale", type=float, default=0.02, help="multiplier on each request's sweeps")
+    ap.add_argument("--max-active", type=int, default=16)
+    ap.add_argument("--chunk", type=int, default=500, help="sweeps per scheduling step")
+    ap.add_argument("--seed", type=int, default=0)
+    args = ap.parse_args(argv)
+    arrivals = poisson_requests(args.rate, args.duration, seed=args.seed)
+    for _, req in arrivals:
+        req.n_sweeps = max(req.n_bins, int(req.n_sweeps * args.sweep_scale))
+    offered = sum(req.L * req.L * req.n_realizations * len(req.temperatures) *
+                  (req.n_sweeps + max(1, req.n_sweeps // 10)) for _, req in arrivals)
+    server = ScanServer(max_active=args.max_active, chunk_sweeps=args.chunk)
+    t0 = time.perf_counter()
+    tickets = []
+    for at, req in arrivals:
+        delay = at - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        tickets.append(server.submit(req))
+    for t in tickets:
+        t.result()
+    wall = time.perf_counter() - t0
+    stats = server.stats()
+    server.close()
+    stats.update({"requests": len(arrivals), "rate_hz": args.rate, "duration_s": args.duration,
+                  "sweep_scale": args.sweep_scale, "offered_site_updates": float(offered),
+                  "wall_s": wall})
+    print(json.dumps(stats))
+
+
+if __name__ == "__main__":
+    main()
