@@ -91,7 +91,7 @@ size_t group_bytes(int L, int n_lat, bool lds, int n_bins) {
              (size_t)n_lat * n_bins * RBIM_ACC_SLOTS * 8;
   if (!lds) {
     const size_t tx = (L + 31) / 32;
-    b += N * n_lat * (1 + 2 + 8 + 4 + 8) + (size_t)n_lat * tx * tx * 5 * 8 + (size_t)n_lat * 4;
+    b += N * n_lat * (1 + 2 + 8 + 4 + 8) + (size_t)n_lat * tx * tx * 7 * 8 + (size_t)n_lat * 12;
   }
   return b;
 }
@@ -155,10 +155,12 @@ int group_alloc(rbim_ctx* ctx, Group& g, const std::vector<uint8_t>& jbits,
     HIP_TRY(ctx, g.arena.alloc(&p, NN * 8)); t.gf = (uint64_t*)p;
     HIP_TRY(ctx, g.arena.alloc(&p, NN * 4)); t.lst = (uint32_t*)p;
     HIP_TRY(ctx, g.arena.alloc(&p, NN * 8)); t.st = (uint64_t*)p;
-    const size_t pb = (size_t)g.n_lat * t.tiles_x * t.tiles_y * 5 * 8;
+    const size_t pb = (size_t)g.n_lat * t.tiles_x * t.tiles_y * 7 * 8;
     HIP_TRY(ctx, g.arena.alloc(&p, pb)); t.part = (int64_t*)p;
     HIP_TRY(ctx, hipMemsetAsync(p, 0, pb, ctx->stream));
-    HIP_TRY(ctx, g.arena.alloc(&p, 4 * g.n_lat)); t.wrapf = (uint32_t*)p;
+    HIP_TRY(ctx, g.arena.alloc(&p, 8 * g.n_lat)); t.wrapf = (uint32_t*)p;
+    HIP_TRY(ctx, hipMemsetAsync(p, 0, 8 * g.n_lat, ctx->stream));
+    HIP_TRY(ctx, g.arena.alloc(&p, 4 * g.n_lat)); t.pending = (int32_t*)p;
     HIP_TRY(ctx, hipMemsetAsync(p, 0, 4 * g.n_lat, ctx->stream));
   }
   if (spins0) {
@@ -208,6 +210,10 @@ int group_run(rbim_ctx* ctx, Group& g, int64_t n, hipStream_t stream, int64_t* l
     HIP_TRY(ctx, launch_tiled_sweep(g.set, g.tile, stream));
     *launches += tiled_kernels_per_sweep();
   }
+  // each sweep's measurement rides in the next sweep's first kernel; the
+  // last one of the run is measured here so the moment sums are complete
+  HIP_TRY(ctx, launch_tiled_flush(g.set, g.tile, stream));
+  *launches += 2;
   return RBIM_OK;
 }
 

@@ -38,8 +38,9 @@ struct TiledScratch {
   uint64_t* gf;        // forest over local roots: parent (low 32) | wx:16 | wy:16
   uint32_t* lst;       // local cluster: size << 16 | n_plus before the flip
   uint64_t* st;        // cluster: size << 32 | n_plus after the flip, at roots
-  int64_t* part;       // [n_lat][n_tiles][5] partial reductions
-  uint32_t* wrapf;     // [n_lat] wrap flags
+  int64_t* part;       // [n_lat][n_tiles][7] partial measurements of one sweep
+  uint32_t* wrapf;     // [n_lat][2] wrap flags by sweep parity
+  int32_t* pending;    // [n_lat] the last completed sweep still awaits its measurement
   int32_t tile;        // tile side: 64 when L is a multiple of 64 (L >= 128), else 32
   int32_t tiles_x, tiles_y;
 };
@@ -49,6 +50,8 @@ hipError_t launch_init_spins(const LatticeSet& s, int start, hipStream_t st);
 hipError_t launch_lds_sweeps(const LatticeSet& s, int threads, int sweeps, hipStream_t st);
 size_t lds_bytes(int N);
 hipError_t launch_tiled_sweep(const LatticeSet& s, const TiledScratch& t, hipStream_t st);
+// Measures and folds the pending sweep at the end of a run of tiled sweeps.
+hipError_t launch_tiled_flush(const LatticeSet& s, const TiledScratch& t, hipStream_t st);
 hipError_t launch_label_rank(const int32_t* canon, int32_t N, int32_t* labels, int32_t* rank_tmp,
                              int32_t* n_clusters, hipStream_t st);
 hipError_t launch_cluster_stats(const int32_t* labels, const int8_t* spins, int64_t n,

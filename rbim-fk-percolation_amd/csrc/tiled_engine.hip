@@ -32,7 +32,7 @@
 
 namespace {
 
-constexpr int PARTS = 5;
+constexpr int PARTS = 7;  // e, m, n_clusters, key1, key2, n_plus1, n_plus2
 
 __device__ __forceinline__ uint64_t gpack(uint32_t p, int wx, int wy) {
   return (uint64_t)p | ((uint64_t)(uint16_t)(int16_t)wx << 32) |
@@ -92,6 +92,110 @@ __device__ __forceinline__ bool lattice_active(const LatticeSet& s, int lat) {
   return s.sweep_ctr[lat] < s.total;
 }
 
+// ------------------------------------------------------------ measurement
+// Per-tile partial of one sweep's measurement: E, M, n_clusters and the two
+// largest clusters (key = size << 32 | ~root, ties to the smaller root) with
+// their n_plus carried along, so the partial is final even after the next
+// sweep has reused the cluster counters.
+struct TileMeas {
+  int32_t e, m, ncl;  // one tile: |e| <= 2 T^2
+  uint64_t k1, k2;
+  uint32_t p1, p2;
+};
+
+__device__ __forceinline__ void top2_insert(TileMeas& a, uint64_t k, uint32_t p) {
+  if (k > a.k1) {
+    a.k2 = a.k1;
+    a.p2 = a.p1;
+    a.k1 = k;
+    a.p1 = p;
+  } else if (k > a.k2) {
+    a.k2 = k;
+    a.p2 = p;
+  }
+}
+
+// Bits of the per-site occ byte: bonds and flip bit of the sweep, and -- set by
+// K3 at the root of every cluster -- the root flag the next measurement reads.
+constexpr uint32_t OCC_ROOT = 8u;
+
+// Bond energy of site n's two forward bonds (couplings j: bit 0 horizontal
+// negative, bit 1 vertical negative) with neighbour spins sr and su.
+__device__ __forceinline__ int bond_energy(uint32_t j, int s0, int sr, int su) {
+  return -(((j & 1u) ? -1 : 1) * s0 * sr) - (((j & 2u) ? -1 : 1) * s0 * su);
+}
+
+// Cluster rooted at site n with counters c = size << 32 | n_plus.
+__device__ __forceinline__ void meas_root(TileMeas& a, uint64_t c, uint32_t n) {
+  ++a.ncl;
+  top2_insert(a, ((c >> 32) << 32) | (uint64_t)(0xFFFFFFFFu - n), (uint32_t)c);
+}
+
+// Workgroup reduction of the lanes' partials in two halves, so the cluster
+// part (read before the sweep overwrites the roots) and the spin part (summed
+// while the sweep reads the spins) need not be live at the same time: each
+// wave reduces into red[wave][.] and, after a barrier, thread 0 combines.
+__device__ void wave_clusters(TileMeas a, int64_t (*red)[PARTS]) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a.ncl += __shfl_xor(a.ncl, off);
+    const uint64_t o1 = __shfl_xor(a.k1, off), o2 = __shfl_xor(a.k2, off);
+    const uint32_t q1 = __shfl_xor(a.p1, off), q2 = __shfl_xor(a.p2, off);
+    top2_insert(a, o1, q1);
+    top2_insert(a, o2, q2);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    int64_t* r = red[threadIdx.x >> 6];
+    r[2] = a.ncl;
+    r[3] = (int64_t)a.k1;
+    r[4] = (int64_t)a.k2;
+    r[5] = a.p1;
+    r[6] = a.p2;
+  }
+}
+
+__device__ void wave_spins(int32_t e, int32_t m, int64_t (*red)[PARTS]) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    e += __shfl_xor(e, off);
+    m += __shfl_xor(m, off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[threadIdx.x >> 6][0] = e;
+    red[threadIdx.x >> 6][1] = m;
+  }
+}
+
+template <int NTH>
+__device__ void meas_combine(int64_t (*red)[PARTS], int64_t* out) {
+  TileMeas a = {0, 0, 0, 0ull, 0ull, 0u, 0u};
+  int64_t e = 0, m = 0, ncl = 0;
+#pragma unroll 1
+  for (int w = 0; w < NTH / 64; ++w) {
+    e += red[w][0];
+    m += red[w][1];
+    ncl += red[w][2];
+    top2_insert(a, (uint64_t)red[w][3], (uint32_t)red[w][5]);
+    top2_insert(a, (uint64_t)red[w][4], (uint32_t)red[w][6]);
+  }
+  out[0] = e;
+  out[1] = m;
+  out[2] = ncl;
+  out[3] = (int64_t)a.k1;
+  out[4] = (int64_t)a.k2;
+  out[5] = a.p1;
+  out[6] = a.p2;
+}
+
+// Both halves at once (one barrier).
+template <int NTH>
+__device__ void meas_publish(TileMeas a, int64_t (*red)[PARTS], int64_t* out) {
+  wave_clusters(a, red);
+  wave_spins(a.e, a.m, red);
+  __syncthreads();
+  if (threadIdx.x == 0) meas_combine<NTH>(red, out);
+}
+
 // ----------------------------------------------------------------- K1
 __device__ __forceinline__ uint32_t lfind(uint32_t* par, uint32_t x) {
   uint32_t p = par[x];
@@ -116,6 +220,7 @@ __device__ __forceinline__ void lunion(uint32_t* par, uint32_t a, uint32_t b) {
 
 template <int T, int NTH>
 __global__ __launch_bounds__(NTH) void k1_tile(LatticeSet s, TiledScratch t) {
+  __shared__ int64_t red[NTH / 64][PARTS];
   __shared__ uint32_t par[T * T];
   __shared__ uint32_t lcnt[T * T];
   __shared__ uint8_t bits[T * T];  // bit0 spin, bit1 occ_h, bit2 occ_v
@@ -134,6 +239,8 @@ __global__ __launch_bounds__(NTH) void k1_tile(LatticeSet s, TiledScratch t) {
   const uint32_t lid = s.lat_id[lat];
   const bool given = s.given_occ != nullptr;
   const bool even = (L & 1) == 0;
+  const bool meas = t.pending[lat] != 0;  // the previous sweep is measured here
+  TileMeas ms = {0, 0, 0, 0ull, 0ull, 0u, 0u};
 
   // activation: thread handles the local pair (lx, 2k), (lx, 2k+1)
   for (int i = threadIdx.x; i < T * T / 2; i += NTH) {
@@ -153,6 +260,11 @@ __global__ __launch_bounds__(NTH) void k1_tile(LatticeSet s, TiledScratch t) {
       const uint32_t nr = (x == L - 1) ? (uint32_t)y : n + L;
       const uint32_t nu = (y == L - 1) ? n - y : n + 1;
       const int s0 = sp[n];
+      if (meas) {
+        ms.e += bond_energy(jb[n], s0, sp[nr], sp[nu]);
+        ms.m += s0;
+        if (t.occ[base + n] & OCC_ROOT) meas_root(ms, t.st[base + n], n);
+      }
       uint32_t oh, ov, fl;
       if (given) {
         const uint32_t o = s.given_occ[n];
@@ -175,6 +287,7 @@ __global__ __launch_bounds__(NTH) void k1_tile(LatticeSet s, TiledScratch t) {
       lcnt[li] = 0u;
     }
   }
+  if (meas) meas_publish<NTH>(ms, red, t.part + ((size_t)lat * t.tiles_x * t.tiles_y + blockIdx.x) * PARTS);
   __syncthreads();
   // local union over bonds with both ends in the tile and no periodic crossing
   for (int li = threadIdx.x; li < T * T; li += NTH) {
@@ -260,7 +373,9 @@ __device__ __forceinline__ uint32_t troot(const uint32_t* par, uint32_t x) {
 }
 
 template <int T, int NTH>
-__global__ __launch_bounds__(NTH) void k1_tile_full(LatticeSet s, TiledScratch t) {
+// (8 waves per SIMD: the LDS of four 512-thread tiles fits a CU)
+__global__ __launch_bounds__(NTH, 8) void k1_tile_full(LatticeSet s, TiledScratch t) {
+  __shared__ int64_t red[NTH / 64][PARTS];
   __shared__ uint32_t par[T * T];
   __shared__ uint32_t lcnt[T * T];
   __shared__ uint8_t bits[T * T];  // bit0 spin, bit1 occ_h, bit2 occ_v
@@ -280,6 +395,40 @@ __global__ __launch_bounds__(NTH) void k1_tile_full(LatticeSet s, TiledScratch t
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint64_t below = (1ull << lane) - 1ull;
   constexpr int ITERS = T * T / 2 / NTH;
+  const bool meas = t.pending[lat] != 0;  // the previous sweep is measured here
+  int64_t* const meas_out = t.part + ((size_t)lat * t.tiles_x * t.tiles_y + blockIdx.x) * PARTS;
+  int32_t me = 0, mm = 0;
+  if (meas) {
+    TileMeas ms = {0, 0, 0, 0ull, 0ull, 0u, 0u};
+    // clusters of the previous sweep: root flags of this thread's pairs first,
+    // then the counters of the flagged roots, all loads in flight together
+    uint32_t rf[ITERS];
+#pragma unroll
+    for (int it = 0; it < ITERS; ++it) {
+      const uint32_t ql = (uint32_t)(it * NTH + tid);
+      const uint32_t n0 = (x0 + ql / (T / 2)) * L + y0 + (ql % (T / 2)) * 2u;
+      rf[it] = *reinterpret_cast<const uint16_t*>(t.occ + base + n0);  // n0 even: aligned
+    }
+    // unflagged sites load the tile's first counter instead (one cached line),
+    // so every load is issued before the first result is needed
+    const uint32_t dummy = x0 * L + y0;
+    uint64_t c[2 * ITERS];
+#pragma unroll
+    for (int it = 0; it < ITERS; ++it) {
+      const uint32_t ql = (uint32_t)(it * NTH + tid);
+      const uint32_t n0 = (x0 + ql / (T / 2)) * L + y0 + (ql % (T / 2)) * 2u;
+      c[2 * it] = t.st[base + ((rf[it] & OCC_ROOT) ? n0 : dummy)];
+      c[2 * it + 1] = t.st[base + ((rf[it] & (OCC_ROOT << 8)) ? n0 + 1u : dummy)];
+    }
+#pragma unroll
+    for (int it = 0; it < ITERS; ++it) {
+      const uint32_t ql = (uint32_t)(it * NTH + tid);
+      const uint32_t n0 = (x0 + ql / (T / 2)) * L + y0 + (ql % (T / 2)) * 2u;
+      if (rf[it] & OCC_ROOT) meas_root(ms, c[2 * it], n0);
+      if (rf[it] & (OCC_ROOT << 8)) meas_root(ms, c[2 * it + 1], n0 + 1u);
+    }
+    wave_clusters(ms, red);
+  }
 
 #pragma unroll 1
   for (int it = 0; it < ITERS; ++it) {
@@ -291,6 +440,12 @@ __global__ __launch_bounds__(NTH) void k1_tile_full(LatticeSet s, TiledScratch t
     const uint32_t nr0 = (x == L - 1u) ? y : n0 + L;
     const uint32_t nu1 = (y + 1u == L - 1u) ? n1 - (y + 1u) : n1 + 1u;
     const int s0 = sp[n0], s1 = sp[n1];
+    const int sr0 = sp[nr0], sr1 = sp[nr0 + 1u], su1 = sp[nu1];
+    const uint32_t j0 = jb[n0], j1 = jb[n1];
+    if (meas) {
+      me += bond_energy(j0, s0, sr0, s1) + bond_energy(j1, s1, sr1, su1);
+      mm += s0 + s1;
+    }
     uint32_t oh0, ov0, fl0, oh1, ov1, fl1;
     if (given) {
       const uint32_t o0 = s.given_occ[n0], o1 = s.given_occ[n1];
@@ -301,8 +456,6 @@ __global__ __launch_bounds__(NTH) void k1_tile_full(LatticeSet s, TiledScratch t
       ov1 = (o1 >> 1) & 1u;
       fl1 = s.given_flip[n1] & 1u;
     } else {
-      const uint32_t j0 = jb[n0], j1 = jb[n1];
-      const int sr0 = sp[nr0], sr1 = sp[nr0 + 1u], su1 = sp[nu1];
       const rbim_u32x4 r = rbim_sweep_draws(seed, lid, sweep, n0 >> 1);
       oh0 = (((j0 & 1u) ? -1 : 1) * s0 * sr0 > 0) && ((r.x >> 1) < thr) ? 1u : 0u;
       ov0 = (((j0 & 2u) ? -1 : 1) * s0 * s1 > 0) && ((r.y >> 1) < thr) ? 1u : 0u;
@@ -331,7 +484,9 @@ __global__ __launch_bounds__(NTH) void k1_tile_full(LatticeSet s, TiledScratch t
     lcnt[li0] = 0u;
     lcnt[li0 + 1u] = 0u;
   }
+  if (meas) wave_spins(me, mm, red);
   __syncthreads();
+  if (meas && tid == 0) meas_combine<NTH>(red, meas_out);
 #pragma unroll 1
   for (int it = 0; it < ITERS; ++it) {
     const uint32_t ql = (uint32_t)(it * NTH + tid);
@@ -429,7 +584,7 @@ __global__ void k2_merge(LatticeSet s, TiledScratch t) {
     const uint32_t rb = gfind(uf, lb, bx, by);
     if (ra == rb) {
       const uint32_t wbits = ((cx + bx - ax) != 0 ? 1u : 0u) | ((cy + by - ay) != 0 ? 2u : 0u);
-      if (wbits) atomicOr(&t.wrapf[lat], wbits);
+      if (wbits) atomicOr(&t.wrapf[2 * lat + (s.sweep_ctr[lat] & 1)], wbits);
       return;
     }
     if (ra < rb) {
@@ -469,7 +624,10 @@ __global__ __launch_bounds__(NTH) void k3_flip(LatticeSet s, TiledScratch t) {
     lr[k] = t.lroot[base + n];
     if (lr[k] == li) {
       const uint32_t r = groot(t.gf + base, n);
-      rmap[li] = r | (((uint32_t)t.occ[base + r] >> 2) & 1u) << 31;
+      const uint32_t o = t.occ[base + r];
+      rmap[li] = r | ((o >> 2) & 1u) << 31;
+      // other tiles read only the flip bit of this byte, which stays as it is
+      if (r == n) t.occ[base + n] = (uint8_t)(o | OCC_ROOT);
     }
   }
   __syncthreads();
@@ -492,19 +650,13 @@ __global__ __launch_bounds__(NTH) void k3_flip(LatticeSet s, TiledScratch t) {
 }
 
 // ----------------------------------------------------------------- K4
-__device__ __forceinline__ void top2_merge64(uint64_t& k1, uint64_t& k2, uint64_t o1, uint64_t o2) {
-  const uint64_t hi = max(k1, o1);
-  const uint64_t lo = max(min(k1, o1), max(k2, o2));
-  k1 = hi;
-  k2 = lo;
-}
-
 template <int T, int NTH>
 __global__ __launch_bounds__(NTH) void k4_measure(LatticeSet s, TiledScratch t) {
+  // measures the settled state after the last sweep of a run (earlier sweeps
+  // are measured inside the next sweep's K1)
   __shared__ int64_t red[NTH / 64][PARTS];
   const int lat = blockIdx.y;
-  if (!lattice_active(s, lat)) return;
-  if (s.sweep_ctr[lat] < s.n_therm) return;
+  if (t.pending[lat] == 0) return;
   const int L = s.L, N = s.N;
   const int tx = blockIdx.x % t.tiles_x, ty = blockIdx.x / t.tiles_x;
   const int x0 = tx * T, y0 = ty * T;
@@ -512,8 +664,7 @@ __global__ __launch_bounds__(NTH) void k4_measure(LatticeSet s, TiledScratch t) 
   const size_t base = (size_t)lat * N;
   const int8_t* sp = s.spins + base;
   const uint8_t* jb = s.jbits + base;
-  int64_t e = 0, m = 0, ncl = 0;
-  uint64_t k1 = 0, k2 = 0;
+  TileMeas ms = {0, 0, 0, 0ull, 0ull, 0u, 0u};
   for (int li = threadIdx.x; li < T * T; li += NTH) {
     const int lx = li / T, ly = li % T;
     if (lx >= w || ly >= h) continue;
@@ -521,96 +672,71 @@ __global__ __launch_bounds__(NTH) void k4_measure(LatticeSet s, TiledScratch t) 
     const uint32_t n = (uint32_t)(x * L + y);
     const uint32_t nr = (x == L - 1) ? (uint32_t)y : n + L;
     const uint32_t nu = (y == L - 1) ? n - y : n + 1;
-    const int s0 = sp[n];
-    const uint32_t j = jb[n];
-    e -= (int64_t)((j & 1u) ? -1 : 1) * s0 * sp[nr];
-    e -= (int64_t)((j & 2u) ? -1 : 1) * s0 * sp[nu];
-    m += s0;
-    if (t.lroot[base + n] == (uint32_t)li && gpar(t.gf[base + n]) == n) {
-      ++ncl;
-      const uint64_t size = t.st[base + n] >> 32;
-      top2_merge64(k1, k2, (size << 32) | (uint64_t)(0xFFFFFFFFu - n), 0ull);
-    }
+    ms.e += bond_energy(jb[n], sp[n], sp[nr], sp[nu]);
+    ms.m += sp[n];
+    if (t.occ[base + n] & OCC_ROOT) meas_root(ms, t.st[base + n], n);
   }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    e += __shfl_xor(e, off);
-    m += __shfl_xor(m, off);
-    ncl += __shfl_xor(ncl, off);
-    const uint64_t o1 = __shfl_xor(k1, off), o2 = __shfl_xor(k2, off);
-    top2_merge64(k1, k2, o1, o2);
-  }
-  if (lane == 0) {
-    red[wave][0] = e;
-    red[wave][1] = m;
-    red[wave][2] = ncl;
-    red[wave][3] = (int64_t)k1;
-    red[wave][4] = (int64_t)k2;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int wv = 1; wv < NTH / 64; ++wv) {
-      e += red[wv][0];
-      m += red[wv][1];
-      ncl += red[wv][2];
-      top2_merge64(k1, k2, (uint64_t)red[wv][3], (uint64_t)red[wv][4]);
-    }
-    int64_t* p = t.part + ((size_t)lat * t.tiles_x * t.tiles_y + blockIdx.x) * PARTS;
-    p[0] = e;
-    p[1] = m;
-    p[2] = ncl;
-    p[3] = (int64_t)k1;
-    p[4] = (int64_t)k2;
-  }
+  meas_publish<NTH>(ms, red, t.part + ((size_t)lat * t.tiles_x * t.tiles_y + blockIdx.x) * PARTS);
 }
 
 // ----------------------------------------------------------------- K5
+// Fold the tile partials of sweep gm into its bin (single thread).
+__device__ void finalize_sweep(const LatticeSet& s, const TiledScratch& t, int lat, int64_t gm) {
+  const int nt = t.tiles_x * t.tiles_y;
+  const int64_t* p = t.part + (size_t)lat * nt * PARTS;
+  TileMeas a = {0, 0, 0, 0ull, 0ull, 0u, 0u};
+  int64_t e = 0, m = 0, ncl = 0;
+  for (int i = 0; i < nt; ++i) {
+    e += p[i * PARTS + 0];
+    m += p[i * PARTS + 1];
+    ncl += p[i * PARTS + 2];
+    top2_insert(a, (uint64_t)p[i * PARTS + 3], (uint32_t)p[i * PARTS + 5]);
+    top2_insert(a, (uint64_t)p[i * PARTS + 4], (uint32_t)p[i * PARTS + 6]);
+  }
+  rbim_sweep_stats st;
+  st.energy = e;
+  st.magnetization = m;
+  st.n_clusters = ncl;
+  const int64_t s1 = (int64_t)(a.k1 >> 32), mc1 = 2 * (int64_t)a.p1 - s1;
+  st.largest_size = s1;
+  st.largest_abs_mag = mc1 < 0 ? -mc1 : mc1;
+  if (a.k2 != 0ull) {
+    const int64_t s2 = (int64_t)(a.k2 >> 32), mc2 = 2 * (int64_t)a.p2 - s2;
+    st.second_size = s2;
+    st.second_abs_mag = mc2 < 0 ? -mc2 : mc2;
+  } else {
+    st.second_size = 0;
+    st.second_abs_mag = 0;
+  }
+  uint32_t* wf = &t.wrapf[2 * lat + (gm & 1)];
+  st.wrap_x = (int32_t)(*wf & 1u);
+  st.wrap_y = (int32_t)((*wf >> 1) & 1u);
+  *wf = 0u;
+  const int64_t bin = (gm - s.n_therm) / s.spb;
+  acc_sweep(s.acc + ((size_t)lat * s.n_bins + bin) * RBIM_ACC_SLOTS, st);
+  if (s.last) s.last[lat] = st;
+}
+
+// After K1..K3 of sweep g: finalize sweep g-1 (measured in K1 of g), mark
+// sweep g pending (or drop its wrap bits) and advance the counter.
 __global__ void k5_finalize(LatticeSet s, TiledScratch t) {
-  const int lat = blockIdx.x;
-  if (threadIdx.x != 0) return;
+  const int lat = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lat >= s.n_lat) return;
   const int64_t g = s.sweep_ctr[lat];
   if (g >= s.total) return;
-  const uint32_t wf = t.wrapf[lat];
-  t.wrapf[lat] = 0u;
-  if (g >= s.n_therm) {
-    const int nt = t.tiles_x * t.tiles_y;
-    const int64_t* p = t.part + (size_t)lat * nt * PARTS;
-    int64_t e = 0, m = 0, ncl = 0;
-    uint64_t k1 = 0, k2 = 0;
-    for (int i = 0; i < nt; ++i) {
-      e += p[i * PARTS + 0];
-      m += p[i * PARTS + 1];
-      ncl += p[i * PARTS + 2];
-      top2_merge64(k1, k2, (uint64_t)p[i * PARTS + 3], (uint64_t)p[i * PARTS + 4]);
-    }
-    const size_t base = (size_t)lat * s.N;
-    rbim_sweep_stats st;
-    st.energy = e;
-    st.magnetization = m;
-    st.n_clusters = ncl;
-    const uint32_t r1 = 0xFFFFFFFFu - (uint32_t)k1;
-    const uint64_t c1 = t.st[base + r1];
-    const int64_t s1 = (int64_t)(c1 >> 32), mc1 = 2 * (int64_t)(c1 & 0xFFFFFFFFull) - s1;
-    st.largest_size = s1;
-    st.largest_abs_mag = mc1 < 0 ? -mc1 : mc1;
-    if (k2 != 0ull) {
-      const uint32_t r2 = 0xFFFFFFFFu - (uint32_t)k2;
-      const uint64_t c2 = t.st[base + r2];
-      const int64_t s2 = (int64_t)(c2 >> 32), mc2 = 2 * (int64_t)(c2 & 0xFFFFFFFFull) - s2;
-      st.second_size = s2;
-      st.second_abs_mag = mc2 < 0 ? -mc2 : mc2;
-    } else {
-      st.second_size = 0;
-      st.second_abs_mag = 0;
-    }
-    st.wrap_x = (int32_t)(wf & 1u);
-    st.wrap_y = (int32_t)((wf >> 1) & 1u);
-    const int64_t bin = (g - s.n_therm) / s.spb;
-    acc_sweep(s.acc + ((size_t)lat * s.n_bins + bin) * RBIM_ACC_SLOTS, st);
-    if (s.last) s.last[lat] = st;
-  }
+  if (t.pending[lat]) finalize_sweep(s, t, lat, g - 1);
+  const bool measured = g >= s.n_therm;
+  t.pending[lat] = measured ? 1 : 0;
+  if (!measured) t.wrapf[2 * lat + (g & 1)] = 0u;
   s.sweep_ctr[lat] = g + 1;
+}
+
+// End of a run: finalize the pending sweep measured by k4_measure.
+__global__ void k5_flush(LatticeSet s, TiledScratch t) {
+  const int lat = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lat >= s.n_lat || t.pending[lat] == 0) return;
+  finalize_sweep(s, t, lat, s.sweep_ctr[lat] - 1);
+  t.pending[lat] = 0;
 }
 
 // ------------------------------------------------------ one-lattice helpers
@@ -677,7 +803,7 @@ __global__ void energy_mag_kernel(const int8_t* spins, const uint8_t* jb, int32_
 
 }  // namespace
 
-int tiled_kernels_per_sweep() { return 5; }
+int tiled_kernels_per_sweep() { return 4; }
 
 int tiled_tile_size(int L) {
   return (L % 64 == 0 && L >= 128) ? 64 : 32;
@@ -693,14 +819,25 @@ hipError_t launch_tiled_t(const LatticeSet& s, const TiledScratch& t, hipStream_
   const int nb = (t.tiles_x + t.tiles_y) * s.L;
   hipLaunchKernelGGL((k2_merge<T>), dim3((nb + 255) / 256, s.n_lat), dim3(256), 0, st, s, t);
   hipLaunchKernelGGL((k3_flip<T, NTH>), dim3(ntiles, s.n_lat), dim3(NTH), 0, st, s, t);
-  hipLaunchKernelGGL((k4_measure<T, NTH>), dim3(ntiles, s.n_lat), dim3(NTH), 0, st, s, t);
-  hipLaunchKernelGGL(k5_finalize, dim3(s.n_lat), dim3(64), 0, st, s, t);
+  hipLaunchKernelGGL(k5_finalize, dim3((s.n_lat + 63) / 64), dim3(64), 0, st, s, t);
+  return hipGetLastError();
+}
+
+template <int T, int NTH>
+hipError_t launch_flush_t(const LatticeSet& s, const TiledScratch& t, hipStream_t st) {
+  hipLaunchKernelGGL((k4_measure<T, NTH>), dim3(t.tiles_x * t.tiles_y, s.n_lat), dim3(NTH), 0, st, s, t);
+  hipLaunchKernelGGL(k5_flush, dim3((s.n_lat + 63) / 64), dim3(64), 0, st, s, t);
   return hipGetLastError();
 }
 
 hipError_t launch_tiled_sweep(const LatticeSet& s, const TiledScratch& t, hipStream_t st) {
   if (t.tile == 64) return launch_tiled_t<64, 512>(s, t, st);
   return launch_tiled_t<32, 256>(s, t, st);
+}
+
+hipError_t launch_tiled_flush(const LatticeSet& s, const TiledScratch& t, hipStream_t st) {
+  if (t.tile == 64) return launch_flush_t<64, 512>(s, t, st);
+  return launch_flush_t<32, 256>(s, t, st);
 }
 
 hipError_t launch_label_rank(const int32_t* canon, int32_t N, int32_t* labels, int32_t* rank_tmp,
