@@ -614,21 +614,43 @@ __global__ __launch_bounds__(NTH) void k3_flip(LatticeSet s, TiledScratch t) {
   const uint32_t x0 = tx * T, y0 = ty * T;
   const uint32_t w = min((uint32_t)T, L - x0), h = min((uint32_t)T, L - y0);
   const size_t base = (size_t)lat * N;
-  uint32_t lr[PER];
+  // every load this thread needs is issued up front: the site's local root
+  // and spin, and at local roots the forest entry, occ byte and local counts
+  // (other sites load the tile's first entries instead: one cached line)
+  const uint32_t dummy = x0 * L + y0;
+  uint32_t lr[PER], lc[PER], oc[PER];
+  uint64_t gv[PER];
+  int sv[PER];
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     const uint32_t li = (uint32_t)(k * NTH + threadIdx.x), lx = li / T, ly = li % T;
-    lr[k] = 0xFFFFFFFFu;
-    if (lx >= w || ly >= h) continue;
+    const bool in = lx < w && ly < h;
+    const uint32_t n = in ? (x0 + lx) * L + y0 + ly : dummy;
+    lr[k] = in ? (uint32_t)t.lroot[base + n] : 0xFFFFFFFFu;
+    sv[k] = s.spins[base + n];
+  }
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const uint32_t li = (uint32_t)(k * NTH + threadIdx.x), lx = li / T, ly = li % T;
+    const uint32_t n = lr[k] == li ? (x0 + lx) * L + y0 + ly : dummy;
+    gv[k] = t.gf[base + n];
+    oc[k] = t.occ[base + n];
+    lc[k] = t.lst[base + n];
+  }
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const uint32_t li = (uint32_t)(k * NTH + threadIdx.x), lx = li / T, ly = li % T;
+    if (lr[k] != li) continue;
     const uint32_t n = (x0 + lx) * L + y0 + ly;
-    lr[k] = t.lroot[base + n];
-    if (lr[k] == li) {
-      const uint32_t r = groot(t.gf + base, n);
-      const uint32_t o = t.occ[base + r];
-      rmap[li] = r | ((o >> 2) & 1u) << 31;
-      // other tiles read only the flip bit of this byte, which stays as it is
-      if (r == n) t.occ[base + n] = (uint8_t)(o | OCC_ROOT);
+    uint32_t r = gpar(gv[k]), o = oc[k];
+    if (r != n) {  // merged across tiles: finish the walk, point n at the root
+      r = groot(t.gf + base, r);
+      t.gf[base + n] = gpack(r, 0, 0);
+      o = t.occ[base + r];
     }
+    rmap[li] = r | ((o >> 2) & 1u) << 31;
+    // other tiles read only the flip bit of this byte, which stays as it is
+    if (r == n) t.occ[base + n] = (uint8_t)(o | OCC_ROOT);
   }
   __syncthreads();
 #pragma unroll
@@ -638,10 +660,9 @@ __global__ __launch_bounds__(NTH) void k3_flip(LatticeSet s, TiledScratch t) {
     const uint32_t n = (x0 + lx) * L + y0 + ly;
     const uint32_t m = rmap[lr[k]];
     const uint32_t r = m & 0x7FFFFFFFu, fl = m >> 31;
-    if (fl) s.spins[base + n] = (int8_t)-s.spins[base + n];
+    if (fl) s.spins[base + n] = (int8_t)-sv[k];
     if (lr[k] == li) {
-      const uint32_t lc = t.lst[base + n];
-      const uint64_t size = lc >> 16, np = lc & 0xFFFFu;
+      const uint64_t size = lc[k] >> 16, np = lc[k] & 0xFFFFu;
       const uint64_t np_new = fl ? size - np : np;
       atomicAdd((unsigned long long*)&t.st[base + r], (unsigned long long)((size << 32) | np_new));
     }
