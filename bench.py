@@ -15,9 +15,12 @@ Printed as one JSON line, with
                 average launch duration, vs MI355X's 8 TB/s;
   plain         the one-lattice-at-a-time tiled path at the same L;
   cpu_baseline  the oracle's C restatement of the reference sweep on all host
-                cores (one lattice per thread), on a bounded sample.
+                cores (one lattice per thread), on a bounded sample (~10 s);
+  L256          the metric's second point, the same recipe at L=256 (1280
+                lattices of 65536 sites, 200 + 2000 sweeps): multi-tile
+                union-find in both the batched and the plain mode.
 
-    python bench.py [--gpus 1] [--steps 22000] [--warmup 200]
+    python bench.py [--gpus 1] [--steps 22000] [--warmup 200] [--second-L 256]
 """
 
 from __future__ import annotations
@@ -57,17 +60,17 @@ def run_schedule(K):
     return K - n_bins * spb, n_bins, spb
 
 
-def gpu_batched(specs, K, W, engine, sweeps_per_launch=0, threads=0):
+def gpu_batched(specs, K, W, engine, sweeps_per_launch=0, threads=0, streams=0):
     from rbim.engine import Batch
 
     n_therm, n_bins, spb = run_schedule(K)
     if W > 0:
         wn, wb, ws = run_schedule(W)
         with Batch(specs, n_therm=wn, n_bins=wb, sweeps_per_bin=ws, engine=engine,
-                   sweeps_per_launch=sweeps_per_launch, threads=threads) as warm:
+                   sweeps_per_launch=sweeps_per_launch, threads=threads, streams=streams) as warm:
             warm.run_timed(W)
     with Batch(specs, n_therm=n_therm, n_bins=n_bins, sweeps_per_bin=spb, engine=engine,
-               sweeps_per_launch=sweeps_per_launch, threads=threads) as batch:
+               sweeps_per_launch=sweeps_per_launch, threads=threads, streams=streams) as batch:
         t0 = time.perf_counter()
         ms = batch.run_timed(K)
         wall = time.perf_counter() - t0
@@ -109,8 +112,10 @@ def profiled_traffic(L, n_lat, sweeps_per_launch, threads):
     return None
 
 
-def cpu_baseline(specs, sweeps, per_thread):
-    """Oracle C restatement, one lattice per thread on every host core."""
+def cpu_baseline(specs, site_updates_per_thread, per_thread):
+    """Oracle C restatement, one lattice per thread on every host core; each
+    thread runs ``per_thread`` of the workload's lattices for the same number of
+    sweeps, sized to about ``site_updates_per_thread`` site-updates."""
     from concurrent.futures import ThreadPoolExecutor
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -121,6 +126,8 @@ def cpu_baseline(specs, sweeps, per_thread):
     cores = len(os.sched_getaffinity(0))
     n = cores * per_thread
     chosen = [specs[(i * 37) % len(specs)] for i in range(n)]
+    L = chosen[0].L
+    sweeps = max(22, int(site_updates_per_thread) // (per_thread * L * L))
 
     def job(spec):
         s0 = sw_oracle.c_init_spins(spec.L, spec.seed, spec.lattice_id)
@@ -137,9 +144,67 @@ def cpu_baseline(specs, sweeps, per_thread):
         "unit": "site-updates/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"{n} of the workload's lattices (L={chosen[0].L}) x {sweeps} sweeps, "
+        "sample": f"{n} of the workload's lattices (L={L}) x {sweeps} sweeps, "
                   f"one lattice per thread, {wall:.1f} s wall",
     }
+
+
+def measure_point(L, args, steps, warmup, plain_steps, with_cpu):
+    """Batched scan at one L (p, 40 T x 32 R = 1280 lattices by default), the
+    plain one-lattice path and the CPU port beside it."""
+    temperatures = np.linspace(1.0, 3.0, args.temps)
+    specs = scan_specs(L, args.p, temperatures, args.realizations, seed=0)
+    res = gpu_batched(specs, steps, warmup, args.engine, args.sweeps_per_launch, args.threads,
+                      args.streams)
+    value = res["site_updates"] / (res["ms"] / 1e3)
+    n_lat = len(specs)
+    sites = L * L * n_lat
+    lds = L <= 128 and args.engine != 1
+    # every launch covers all lattices: one LDS kernel launch runs sweeps_per_launch
+    # sweeps; the tiled path runs 4 kernels per sweep (timed together per sweep)
+    launches = max(1, res["launches"])
+    if lds:
+        avg_launch_ms = res["ms"] / launches
+        units = sites * steps / launches
+        kernel = "lds_sweep_kernel"
+    else:
+        avg_launch_ms = res["ms"] / steps
+        units = sites
+        kernel = "tiled sweep: k1_tile_full + k2_merge + k3_flip + k5_finalize"
+    achieved = ALGO_BYTES_PER_SITE_UPDATE * units / (avg_launch_ms / 1e3) / 1e9
+    plain = gpu_plain(specs[len(specs) // 2], plain_steps)
+    point = {
+        "L": L,
+        "value": value,
+        "ms_per_step": res["ms"] / steps,
+        "steps": steps,
+        "warmup": warmup,
+        "workload": f"batched uniform scan: L={L}, p={args.p}, {args.temps} T in linspace(1,3) x "
+                    f"{args.realizations} realizations = {n_lat} lattices, {steps} sweeps "
+                    f"({run_schedule(steps)[0]} therm + {steps - run_schedule(steps)[0]} measured)",
+        "lattices": n_lat,
+        "threads_per_lattice": res["threads"],
+        "sweeps_per_launch": res["sweeps_per_launch"],
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBS,
+            "traffic": profiled_traffic(L, n_lat, res["sweeps_per_launch"], res["threads"]),
+            "kernel": kernel,
+            "avg_launch_ms": avg_launch_ms,
+            "launches": launches if lds else steps,
+        },
+        "plain": {"value": plain, "unit": "site-updates/s", "L": L,
+                  "sample": f"one lattice, {plain_steps} sweeps, tiled kernels in hipGraphs"},
+        "batched_vs_plain": value / plain,
+    }
+    if with_cpu:
+        cpu = cpu_baseline(specs, args.cpu_site_updates, args.cpu_per_thread)
+        point["cpu_baseline"] = cpu
+        point["batched_vs_cpu"] = value / cpu["value"]
+    return point
 
 
 def main():
@@ -154,9 +219,16 @@ def main():
     ap.add_argument("--engine", type=int, default=0, help="0 auto, 1 plain (tiled), 2 batched (LDS)")
     ap.add_argument("--sweeps-per-launch", type=int, default=0)
     ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--streams", type=int, default=0, help="streams per tiled set (0: default)")
     ap.add_argument("--plain-steps", type=int, default=400)
-    ap.add_argument("--cpu-sweeps", type=int, default=2500)
+    ap.add_argument("--cpu-site-updates", type=float, default=2.5e8,
+                    help="site-updates per host thread in the CPU sample (about 10 s)")
     ap.add_argument("--cpu-per-thread", type=int, default=4)
+    ap.add_argument("--second-L", type=int, default=256,
+                    help="second metric point, same recipe (0 to skip)")
+    ap.add_argument("--second-steps", type=int, default=2200)
+    ap.add_argument("--second-warmup", type=int, default=20)
+    ap.add_argument("--second-plain-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (rocprof counter "
@@ -179,62 +251,40 @@ def main():
               f"with no communication, so the N-GPU figure is N x the 1-GPU one.", file=sys.stderr)
         sys.exit(2)
 
-    temperatures = np.linspace(1.0, 3.0, args.temps)
-    specs = scan_specs(args.L, args.p, temperatures, args.realizations, seed=0)
-    res = gpu_batched(specs, args.steps, args.warmup, args.engine, args.sweeps_per_launch, args.threads)
-    value = res["site_updates"] / (res["ms"] / 1e3)
-    ms_per_step = res["ms"] / args.steps
-    n_lat = len(specs)
-    sites = args.L * args.L * n_lat
-    # dominant kernel: every launch covers all lattices for sweeps_per_launch sweeps
-    launches = max(1, res["launches"])
-    avg_launch_ms = res["ms"] / launches
-    bytes_per_launch = ALGO_BYTES_PER_SITE_UPDATE * sites * args.steps / launches
-    achieved = bytes_per_launch / (avg_launch_ms / 1e3) / 1e9
-    plain = gpu_plain(specs[len(specs) // 2], args.plain_steps)
+    main_pt = measure_point(args.L, args, args.steps, args.warmup, args.plain_steps, not args.no_cpu)
     line = {
         "metric": "SW site-updates/s on 1 MI355X, batched vs plain, at L=32 & L=256 (bit-exact)",
-        "value": value,
+        "value": main_pt["value"],
         "unit": "site-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": main_pt["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int8/int32 (FP64 moments)",
         "data": "synthetic: seeded +-J disorder (numpy, as the reference draws it), hot start",
         "config": {
-            "workload": f"batched uniform scan: L={args.L}, p={args.p}, {args.temps} T in "
-                        f"linspace(1,3) x {args.realizations} realizations = {n_lat} lattices, "
-                        f"{args.steps} sweeps ({run_schedule(args.steps)[0]} therm + "
-                        f"{args.steps - run_schedule(args.steps)[0]} measured)",
+            "workload": main_pt["workload"],
             "L": args.L,
-            "lattices": n_lat,
+            "lattices": main_pt["lattices"],
             "engine": {0: "auto", 1: "plain", 2: "batched"}[args.engine],
-            "threads_per_lattice": res["threads"],
-            "sweeps_per_launch": res["sweeps_per_launch"],
+            "threads_per_lattice": main_pt["threads_per_lattice"],
+            "sweeps_per_launch": main_pt["sweeps_per_launch"],
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": PEAK_HBM_GBS,
-            "unit": "GB/s",
-            "frac": achieved / PEAK_HBM_GBS,
-            "traffic": profiled_traffic(args.L, n_lat, res["sweeps_per_launch"], res["threads"]),
-            "kernel": "lds_sweep_kernel",
-            "avg_launch_ms": avg_launch_ms,
-            "launches": launches,
-        },
-        "plain": {"value": plain, "unit": "site-updates/s", "L": args.L,
-                  "sample": f"one lattice, {args.plain_steps} sweeps, tiled kernels in hipGraphs"},
-        "batched_vs_plain": value / plain,
+        "roofline": main_pt["roofline"],
+        "plain": main_pt["plain"],
+        "batched_vs_plain": main_pt["batched_vs_plain"],
     }
-    if not args.no_cpu:
-        cpu = cpu_baseline(specs, args.cpu_sweeps, args.cpu_per_thread)
-        line["cpu_baseline"] = cpu
-        line["batched_vs_cpu"] = value / cpu["value"]
+    if "cpu_baseline" in main_pt:
+        line["cpu_baseline"] = main_pt["cpu_baseline"]
+        line["batched_vs_cpu"] = main_pt["batched_vs_cpu"]
+    if args.second_L and args.second_L != args.L:
+        # the metric's second point (multi-tile union-find in both modes)
+        line[f"L{args.second_L}"] = measure_point(args.second_L, args, args.second_steps,
+                                                  args.second_warmup, args.second_plain_steps,
+                                                  not args.no_cpu)
     print(json.dumps(line), flush=True)
 
 

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define RBIM_ABI_VERSION 1
+#define RBIM_ABI_VERSION 2
 
 /* Moment slots per (lattice, bin); sums over the measured sweeps of a bin.
  * 128-bit sums are stored as (lo, hi) uint64 pairs.                       */
@@ -108,10 +108,14 @@ typedef struct {
   int32_t sweeps_per_launch;  /* LDS kernel sweeps per launch / tiled sweeps per
                                  graph; 0 -> default                              */
   int32_t threads;            /* LDS kernel workgroup size (64..1024); 0 -> default */
+  int32_t streams;            /* streams a tiled (L > 128) set of lattices is split
+                                 over so its latency-bound kernels overlap; every
+                                 distinct L also gets its own stream; 0 -> default */
 } rbim_batch_desc;
 
 typedef struct {
-  int32_t n_groups;           /* lattices are grouped by (L, engine)           */
+  int32_t n_groups;           /* lattices are grouped by (L, engine), tiled
+                                 groups split further; one stream per group    */
   int32_t lds_groups, tiled_groups;
   int32_t threads;            /* LDS kernel workgroup size of the first group  */
   int32_t sweeps_per_launch;

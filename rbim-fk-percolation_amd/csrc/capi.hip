@@ -83,6 +83,8 @@ struct Group {
   TiledScratch tile{};
   DevArena arena;
   hipGraphExec_t graph = nullptr;
+  hipStream_t stream = nullptr;  // batch groups run concurrently, one stream each
+  hipEvent_t done = nullptr;     // recorded after the group's work of a run
 };
 
 size_t group_bytes(int L, int n_lat, bool lds, int n_bins) {
@@ -218,10 +220,20 @@ int group_run(rbim_ctx* ctx, Group& g, int64_t n, hipStream_t stream, int64_t* l
 }
 
 void group_free(Group& g) {
+  if (g.stream) (void)hipStreamSynchronize(g.stream);
   if (g.graph) (void)hipGraphExecDestroy(g.graph);
   g.graph = nullptr;
   g.arena.release();
+  if (g.done) (void)hipEventDestroy(g.done);
+  if (g.stream) (void)hipStreamDestroy(g.stream);
+  g.done = nullptr;
+  g.stream = nullptr;
 }
+
+// Tiled sets of lattices are split over this many streams by default: their
+// kernels are alternately ALU-bound (activation + tile labelling) and
+// latency-bound (cross-tile merge, flip), so two halves interleave on the CUs.
+constexpr int kTiledStreams = 2;
 
 bool valid_L(int L) { return L >= 3 && L <= 16384; }
 
@@ -321,7 +333,7 @@ int ranks_from_canon(rbim_ctx* ctx, const int32_t* canon, int N, int32_t* labels
 struct rbim_batch {
   rbim_ctx* ctx = nullptr;
   hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, fork = nullptr;
   std::vector<Group> groups;
   int n_lat = 0;
   std::vector<int64_t> off;  // site offset of lattice i (batch order)
@@ -533,6 +545,8 @@ static int check_desc(rbim_ctx* ctx, const rbim_batch_desc* d) {
     return set_err(ctx, RBIM_E_INVAL, "more than 2^32 sweeps per lattice");
   if (d->engine < RBIM_ENGINE_AUTO || d->engine > RBIM_ENGINE_BATCHED)
     return set_err(ctx, RBIM_E_INVAL, "unknown engine %d", d->engine);
+  if (d->streams < 0 || d->streams > 16)
+    return set_err(ctx, RBIM_E_INVAL, "streams must be 0 (default) .. 16");
   if (d->threads != 0 && d->threads != 64 && d->threads != 128 && d->threads != 256 &&
       d->threads != 512 && d->threads != 1024)
     return set_err(ctx, RBIM_E_INVAL, "threads must be 64, 128, 256, 512 or 1024");
@@ -588,6 +602,7 @@ int rbim_batch_create(rbim_ctx* ctx, const rbim_batch_desc* d, rbim_batch** out)
   hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&b->ev0);
   if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
   if (e != hipSuccess) {
     rbim_batch_destroy(b);
     return set_err(ctx, RBIM_E_HIP, "stream/event creation failed: %s", hipGetErrorString(e));
@@ -609,6 +624,34 @@ int rbim_batch_create(rbim_ctx* ctx, const rbim_batch_desc* d, rbim_batch** out)
                                               : (g.lds ? default_lds_sweeps(L) : 16);
     }
     b->groups[gid[key]].idx.push_back(i);
+  }
+  // split tiled groups into contiguous chunks, one stream each
+  {
+    std::vector<Group> split;
+    for (Group& g : b->groups) {
+      const int want = g.lds ? 1 : (d->streams > 0 ? d->streams : kTiledStreams);
+      const int parts = std::max(1, std::min(want, (int)g.idx.size()));
+      const size_t n = g.idx.size();
+      for (int k = 0; k < parts; ++k) {
+        Group h;
+        h.L = g.L;
+        h.N = g.N;
+        h.lds = g.lds;
+        h.threads = g.threads;
+        h.per_launch = g.per_launch;
+        h.idx.assign(g.idx.begin() + n * k / parts, g.idx.begin() + n * (k + 1) / parts);
+        split.push_back(std::move(h));
+      }
+    }
+    b->groups = std::move(split);
+  }
+  for (Group& g : b->groups) {
+    hipError_t se = hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking);
+    if (se == hipSuccess) se = hipEventCreateWithFlags(&g.done, hipEventDisableTiming);
+    if (se != hipSuccess) {
+      rbim_batch_destroy(b);
+      return set_err(ctx, RBIM_E_HIP, "stream/event creation failed: %s", hipGetErrorString(se));
+    }
   }
   for (Group& g : b->groups) {
     g.n_lat = (int)g.idx.size();
@@ -645,9 +688,20 @@ int rbim_batch_run(rbim_batch* b, int64_t n_sweeps) {
     return set_err(ctx, RBIM_E_STATE, "run of %lld sweeps passes the end (%lld of %lld issued)",
                    (long long)n_sweeps, (long long)b->issued, (long long)b->total);
   HIP_TRY(ctx, hipSetDevice(ctx->device));
-  for (Group& g : b->groups) {
-    int rc = group_run(ctx, g, n_sweeps, b->stream, &b->launches);
+  if (b->groups.size() == 1) {
+    int rc = group_run(ctx, b->groups[0], n_sweeps, b->stream, &b->launches);
     if (rc != RBIM_OK) return rc;
+  } else {
+    // fork onto the group streams and join back, so the batch stream (what
+    // query, sync and the timing events see) completes after every group
+    HIP_TRY(ctx, hipEventRecord(b->fork, b->stream));
+    for (Group& g : b->groups) {
+      HIP_TRY(ctx, hipStreamWaitEvent(g.stream, b->fork, 0));
+      int rc = group_run(ctx, g, n_sweeps, g.stream, &b->launches);
+      if (rc != RBIM_OK) return rc;
+      HIP_TRY(ctx, hipEventRecord(g.done, g.stream));
+      HIP_TRY(ctx, hipStreamWaitEvent(b->stream, g.done, 0));
+    }
   }
   b->issued += n_sweeps;
   return RBIM_OK;
@@ -732,6 +786,7 @@ int rbim_batch_destroy(rbim_batch* b) {
   for (Group& g : b->groups) group_free(g);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->fork) (void)hipEventDestroy(b->fork);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return RBIM_OK;

@@ -92,6 +92,7 @@ class BatchDesc(ctypes.Structure):
         ("engine", ctypes.c_int32),
         ("sweeps_per_launch", ctypes.c_int32),
         ("threads", ctypes.c_int32),
+        ("streams", ctypes.c_int32),
     ]
 
 

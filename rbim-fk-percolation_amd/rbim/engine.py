@@ -106,7 +106,7 @@ class Batch:
 
     def __init__(self, lattices: Sequence[LatticeSpec], *, n_therm: int, n_bins: int,
                  sweeps_per_bin: int, start: str = "hot", engine: int = _hip.ENGINE_AUTO,
-                 sweeps_per_launch: int = 0, threads: int = 0):
+                 sweeps_per_launch: int = 0, threads: int = 0, streams: int = 0):
         from rbim.clusters import activation_threshold
 
         if start not in ("hot", "cold"):
@@ -157,6 +157,7 @@ class Batch:
         desc.engine = int(engine)
         desc.sweeps_per_launch = int(sweeps_per_launch)
         desc.threads = int(threads)
+        desc.streams = int(streams)
         self._desc = desc
         self._ctx = _hip.context()
         handle = ctypes.c_void_p()
