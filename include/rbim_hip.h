@@ -111,6 +111,10 @@ typedef struct {
   int32_t streams;            /* streams a tiled (L > 128) set of lattices is split
                                  over so its latency-bound kernels overlap; every
                                  distinct L also gets its own stream; 0 -> default */
+  int32_t series;             /* > 0: keep the per-sweep stats of the last `series`
+                                 measured sweeps of every lattice in a device ring
+                                 (time series for autocorrelation / equilibration
+                                 checks); 0 -> off                                */
 } rbim_batch_desc;
 
 typedef struct {
@@ -194,6 +198,12 @@ int rbim_batch_moments(rbim_batch* b, uint64_t* out);
 /* Current spins, concatenated in batch order. */
 int rbim_batch_spins(rbim_batch* b, int8_t* out);
 int rbim_batch_info_get(rbim_batch* b, rbim_batch_info* out);
+/* Per-sweep stats ring of a batch created with series > 0:
+ * out[n_lattices][series] in batch order; entry k of a lattice holds measured
+ * sweep j with j % series == k.  *recorded = measured sweeps so far (the
+ * oldest kept one is max(0, recorded - series)).  The per-sweep loop of
+ * scripts/measure_autocorrelation.py:69-90 reads these series. */
+int rbim_batch_series(rbim_batch* b, rbim_sweep_stats* out, int64_t* recorded);
 int rbim_batch_destroy(rbim_batch* b);
 
 #ifdef __cplusplus

@@ -87,10 +87,11 @@ struct Group {
   hipEvent_t done = nullptr;     // recorded after the group's work of a run
 };
 
-size_t group_bytes(int L, int n_lat, bool lds, int n_bins) {
+size_t group_bytes(int L, int n_lat, bool lds, int n_bins, int series) {
   const size_t N = (size_t)L * L;
   size_t b = N * n_lat * 2 + (size_t)n_lat * (4 + 8 + 4 + 8 + sizeof(rbim_sweep_stats)) +
-             (size_t)n_lat * n_bins * RBIM_ACC_SLOTS * 8;
+             (size_t)n_lat * n_bins * RBIM_ACC_SLOTS * 8 +
+             (size_t)n_lat * (size_t)std::max(series, 0) * sizeof(rbim_sweep_stats);
   if (!lds) {
     const size_t tx = (L + 31) / 32;
     b += N * n_lat * (1 + 2 + 8 + 4 + 8) + (size_t)n_lat * tx * tx * 7 * 8 + (size_t)n_lat * 12;
@@ -116,7 +117,7 @@ int default_lds_sweeps(int L) {
 int group_alloc(rbim_ctx* ctx, Group& g, const std::vector<uint8_t>& jbits,
                 const std::vector<uint32_t>& thr, const std::vector<uint64_t>& seed,
                 const std::vector<uint32_t>& lid, const int8_t* spins0, int start,
-                int64_t n_therm, int64_t spb, int64_t n_bins, int64_t g0) {
+                int64_t n_therm, int64_t spb, int64_t n_bins, int64_t g0, int series = 0) {
   const size_t NN = (size_t)g.N * g.n_lat;
   LatticeSet& s = g.set;
   s.L = g.L;
@@ -148,6 +149,13 @@ int group_alloc(rbim_ctx* ctx, Group& g, const std::vector<uint8_t>& jbits,
   HIP_TRY(ctx, hipMemsetAsync(p, 0, accb, ctx->stream));
   HIP_TRY(ctx, g.arena.alloc(&p, sizeof(rbim_sweep_stats) * g.n_lat)); s.last = (rbim_sweep_stats*)p;
   HIP_TRY(ctx, hipMemsetAsync(p, 0, sizeof(rbim_sweep_stats) * g.n_lat, ctx->stream));
+  s.series = nullptr;
+  s.series_cap = series;
+  if (series > 0) {
+    const size_t sb = (size_t)g.n_lat * series * sizeof(rbim_sweep_stats);
+    HIP_TRY(ctx, g.arena.alloc(&p, sb)); s.series = (rbim_sweep_stats*)p;
+    HIP_TRY(ctx, hipMemsetAsync(p, 0, sb, ctx->stream));
+  }
   if (!g.lds) {
     TiledScratch& t = g.tile;
     t.tile = tiled_tile_size(g.L);
@@ -545,6 +553,7 @@ static int check_desc(rbim_ctx* ctx, const rbim_batch_desc* d) {
     return set_err(ctx, RBIM_E_INVAL, "more than 2^32 sweeps per lattice");
   if (d->engine < RBIM_ENGINE_AUTO || d->engine > RBIM_ENGINE_BATCHED)
     return set_err(ctx, RBIM_E_INVAL, "unknown engine %d", d->engine);
+  if (d->series < 0) return set_err(ctx, RBIM_E_INVAL, "series must be >= 0");
   if (d->streams < 0 || d->streams > 16)
     return set_err(ctx, RBIM_E_INVAL, "streams must be 0 (default) .. 16");
   if (d->threads != 0 && d->threads != 64 && d->threads != 128 && d->threads != 256 &&
@@ -571,7 +580,8 @@ int rbim_batch_footprint(const rbim_batch_desc* d, uint64_t* bytes) {
   std::map<std::pair<int, int>, int> counts;
   for (int i = 0; i < d->n_lattices; ++i) counts[{d->L[i], use_lds(d, d->L[i]) ? 1 : 0}]++;
   uint64_t b = 0;
-  for (auto& kv : counts) b += group_bytes(kv.first.first, kv.second, kv.first.second != 0, d->n_bins);
+  for (auto& kv : counts)
+    b += group_bytes(kv.first.first, kv.second, kv.first.second != 0, d->n_bins, d->series);
   if (bytes) *bytes = b;
   return RBIM_OK;
 }
@@ -670,7 +680,7 @@ int rbim_batch_create(rbim_ctx* ctx, const rbim_batch_desc* d, rbim_batch** out)
       lid[k] = d->lattice_id[i];
     }
     rc = group_alloc(ctx, g, jb, thr, seed, lid, d->spins0 ? sp.data() : nullptr, d->start,
-                     d->n_therm, d->sweeps_per_bin, d->n_bins, 0);
+                     d->n_therm, d->sweeps_per_bin, d->n_bins, 0, d->series);
     if (rc != RBIM_OK) {
       rbim_batch_destroy(b);
       return rc;
@@ -776,6 +786,24 @@ int rbim_batch_info_get(rbim_batch* b, rbim_batch_info* o) {
   o->sweeps_done = b->issued;
   o->total_sweeps = b->total;
   o->kernel_launches = b->launches;
+  return RBIM_OK;
+}
+
+int rbim_batch_series(rbim_batch* b, rbim_sweep_stats* out, int64_t* recorded) {
+  if (!b || !out || !recorded) return set_err(nullptr, RBIM_E_INVAL, "NULL argument");
+  rbim_ctx* ctx = b->ctx;
+  if (b->groups.empty() || b->groups[0].set.series == nullptr)
+    return set_err(ctx, RBIM_E_STATE, "batch was created without a series ring");
+  HIP_TRY(ctx, hipStreamSynchronize(b->stream));
+  const int64_t cap = b->groups[0].set.series_cap;
+  for (Group& g : b->groups) {
+    std::vector<rbim_sweep_stats> h((size_t)g.n_lat * cap);
+    HIP_TRY(ctx, hipMemcpy(h.data(), g.set.series, h.size() * sizeof(rbim_sweep_stats),
+                           hipMemcpyDeviceToHost));
+    for (int k = 0; k < g.n_lat; ++k)
+      std::memcpy(out + (size_t)cap * g.idx[k], &h[(size_t)cap * k], cap * sizeof(rbim_sweep_stats));
+  }
+  *recorded = std::max<int64_t>(0, b->issued - b->n_therm);
   return RBIM_OK;
 }
 

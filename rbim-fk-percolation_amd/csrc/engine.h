@@ -22,6 +22,8 @@ struct LatticeSet {
   int64_t* sweep_ctr;       // [n_lat] sweeps completed
   uint64_t* acc;            // [n_lat][n_bins][RBIM_ACC_SLOTS]
   rbim_sweep_stats* last;   // [n_lat] stats of the last measured sweep
+  rbim_sweep_stats* series; // [n_lat][series_cap] ring of measured sweeps, or NULL
+  int64_t series_cap;
   int64_t n_therm, spb, n_bins, total;
   // given-bits mode (one-lattice parity entry points); NULL in production
   const uint8_t* given_occ; // [N] bit0 horizontal, bit1 vertical
@@ -73,6 +75,12 @@ __device__ __forceinline__ void acc_add_u128(uint64_t* lo, uint64_t* hi, uint64_
   const uint64_t nlo = *lo + vlo;
   *hi += vhi + (nlo < vlo ? 1ull : 0ull);
   *lo = nlo;
+}
+
+// Measured sweep gm of lattice lat into its series ring (when kept).
+__device__ __forceinline__ void series_put(const LatticeSet& s, int lat, int64_t gm,
+                                           const rbim_sweep_stats& st) {
+  if (s.series) s.series[(size_t)lat * s.series_cap + (gm - s.n_therm) % s.series_cap] = st;
 }
 
 __device__ __forceinline__ void acc_sweep(uint64_t* a, const rbim_sweep_stats& s) {

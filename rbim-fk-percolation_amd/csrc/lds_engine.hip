@@ -344,6 +344,7 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
     }
     acc_sweep(a, st);
     if (s.last) s.last[lat] = st;
+    series_put(s, lat, gm, st);
   };
   // satisfied flags of the pair's four bonds, from the current spins
   auto satisfied = [&](const Pair& p, uint32_t& b0, uint32_t& b1, uint32_t& sh0, uint32_t& sv0,

@@ -736,6 +736,7 @@ __device__ void finalize_sweep(const LatticeSet& s, const TiledScratch& t, int l
   const int64_t bin = (gm - s.n_therm) / s.spb;
   acc_sweep(s.acc + ((size_t)lat * s.n_bins + bin) * RBIM_ACC_SLOTS, st);
   if (s.last) s.last[lat] = st;
+  series_put(s, lat, gm, st);
 }
 
 // After K1..K3 of sweep g: finalize sweep g-1 (measured in K1 of g), mark

@@ -93,6 +93,7 @@ class BatchDesc(ctypes.Structure):
         ("sweeps_per_launch", ctypes.c_int32),
         ("threads", ctypes.c_int32),
         ("streams", ctypes.c_int32),
+        ("series", ctypes.c_int32),
     ]
 
 
@@ -133,6 +134,7 @@ SIGNATURES = {
     "rbim_batch_moments": ([_P, _P], ctypes.c_int),
     "rbim_batch_spins": ([_P, _P], ctypes.c_int),
     "rbim_batch_info_get": ([_P, _P], ctypes.c_int),
+    "rbim_batch_series": ([_P, _P, _P], ctypes.c_int),
     "rbim_batch_destroy": ([_P], ctypes.c_int),
 }
 

@@ -101,12 +101,22 @@ class LatticeSpec:
     spins0: Optional[np.ndarray] = None
 
 
+# Layout of rbim_sweep_stats (one measured sweep of one lattice).
+SWEEP_STATS_DTYPE = np.dtype([
+    ("energy", np.int64), ("magnetization", np.int64), ("n_clusters", np.int64),
+    ("largest_size", np.int64), ("largest_abs_mag", np.int64), ("second_size", np.int64),
+    ("second_abs_mag", np.int64), ("wrap_x", np.int32), ("wrap_y", np.int32),
+])
+assert SWEEP_STATS_DTYPE.itemsize == ctypes.sizeof(_hip.SweepStats)
+
+
 class Batch:
     """A set of independent lattices advanced together on the device."""
 
     def __init__(self, lattices: Sequence[LatticeSpec], *, n_therm: int, n_bins: int,
                  sweeps_per_bin: int, start: str = "hot", engine: int = _hip.ENGINE_AUTO,
-                 sweeps_per_launch: int = 0, threads: int = 0, streams: int = 0):
+                 sweeps_per_launch: int = 0, threads: int = 0, streams: int = 0,
+                 series: int = 0):
         from rbim.clusters import activation_threshold
 
         if start not in ("hot", "cold"):
@@ -158,6 +168,10 @@ class Batch:
         desc.sweeps_per_launch = int(sweeps_per_launch)
         desc.threads = int(threads)
         desc.streams = int(streams)
+        if series < 0:
+            raise ValueError("series must be >= 0")
+        self.series_capacity = int(series)
+        desc.series = self.series_capacity
         self._desc = desc
         self._ctx = _hip.context()
         handle = ctypes.c_void_p()
@@ -200,6 +214,23 @@ class Batch:
         flat = np.empty(int(self.offsets[-1]), dtype=np.int8)
         _hip.check(_hip.lib().rbim_batch_spins(self._h, _hip.ptr(flat)), self._ctx)
         return [flat[self.offsets[i]:self.offsets[i + 1]].copy() for i in range(self.n)]
+
+    def series(self) -> np.ndarray:
+        """Per-sweep stats of the measured sweeps kept in the device ring, in
+        order: a (n_lattices, k) structured array (fields of SWEEP_STATS_DTYPE),
+        k = min(measured sweeps so far, ring capacity), oldest first."""
+        if self.series_capacity <= 0:
+            raise RuntimeError("batch was created without series=")
+        cap = self.series_capacity
+        ring = np.zeros((self.n, cap), dtype=SWEEP_STATS_DTYPE)
+        recorded = ctypes.c_int64()
+        _hip.check(_hip.lib().rbim_batch_series(self._h, _hip.ptr(ring), ctypes.byref(recorded)),
+                   self._ctx)
+        n = int(recorded.value)
+        if n <= cap:
+            return ring[:, :n].copy()
+        start = n % cap
+        return np.concatenate([ring[:, start:], ring[:, :start]], axis=1)
 
     def site_updates(self, n_sweeps: int) -> int:
         return int(self.sizes.sum()) * int(n_sweeps)

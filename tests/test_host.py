@@ -248,3 +248,24 @@ def test_results_roundtrip_and_reference_fixture(tmp_path):
     with np.load(str(path), allow_pickle=False) as data:
         assert len(data.files) == 2 + 3 * len(OBSERVABLES)
         json.loads(str(data["metadata_json"]))
+
+
+def test_integrated_autocorrelation_ar1_closed_form():
+    """AR(1) x_t = r x_{t-1} + e_t has rho(t) = r^t, so tau_int = (1+r)/(2(1-r));
+    white noise gives 1/2, a constant series (0.5, 0)."""
+    from rbim.timeseries import integrated_autocorrelation
+
+    rng = np.random.default_rng(11)
+    for r in (0.0, 0.5, 0.9):
+        n = 400_000
+        e = rng.standard_normal(n)
+        x = np.empty(n)
+        x[0] = e[0] / np.sqrt(1 - r * r)
+        for t in range(1, n):
+            x[t] = r * x[t - 1] + e[t]
+        tau, window = integrated_autocorrelation(x)
+        exact = (1 + r) / (2 * (1 - r))
+        assert abs(tau - exact) < 0.05 * exact + 0.02, (r, tau, exact)
+        assert window >= 6 * tau - 1
+    assert integrated_autocorrelation(np.ones(100)) == (0.5, 0)
+    assert integrated_autocorrelation(np.arange(10.0)) == (0.5, 0)
