@@ -100,14 +100,21 @@ def gpu_plain(spec, K):
 
 
 def profiled_traffic(L, n_lat, sweeps_per_launch, threads):
-    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE runs of this same default command (calibrated
-    against a plain device copy); None when the configuration differs."""
-    path = os.path.join(ROOT, "profiles", "lds_sweep_kernel_L32_batched.md")
-    if (L, n_lat, sweeps_per_launch, threads) != (32, 1280, 2000, 256) or not os.path.exists(path):
+    """HBM bytes per launch (LDS kernel) or per sweep (tiled kernels) of the
+    dominant kernel, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
+    this benchmark's batches (calibrated against a plain device copy); None when
+    the configuration differs from the profiled one."""
+    if (L, n_lat, sweeps_per_launch, threads) == (32, 1280, 2000, 256):
+        name, key = "lds_sweep_kernel_L32_batched.md", "traffic_bytes_per_launch:"
+    elif (L, n_lat) == (256, 1280):
+        name, key = "tiled_kernels_L256_batched.md", "traffic_bytes_per_sweep:"
+    else:
+        return None
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key):
             return int(line.split(":")[1])
     return None
 
