@@ -14,7 +14,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SOURCES = ["csrc/lds_engine.hip", "csrc/tiled_engine.hip", "csrc/capi.hip"]
+SOURCES = ["csrc/lds_engine.hip", "csrc/tiled_engine.hip", "csrc/fused_engine.hip", "csrc/capi.hip"]
 OUTPUT = os.path.join(HERE, "rbim", "librbim_hip.so")
 ARCH = os.environ.get("RBIM_OFFLOAD_ARCH", "gfx950")
 

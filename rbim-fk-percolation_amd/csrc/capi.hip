@@ -76,23 +76,30 @@ struct DevArena {
 struct Group {
   int L = 0, N = 0, n_lat = 0;
   bool lds = false;
+  bool fused = false;  // tiled group on the two-kernel edge-slot sweep
   int threads = 256;
   int per_launch = 1;  // LDS sweeps per launch, or tiled sweeps per graph
   std::vector<int> idx;
   LatticeSet set{};
   TiledScratch tile{};
+  FusedScratch fz{};
   DevArena arena;
   hipGraphExec_t graph = nullptr;
   hipStream_t stream = nullptr;  // batch groups run concurrently, one stream each
   hipEvent_t done = nullptr;     // recorded after the group's work of a run
 };
 
-size_t group_bytes(int L, int n_lat, bool lds, int n_bins, int series) {
+size_t group_bytes(int L, int n_lat, bool lds, bool fused, int n_bins, int series) {
   const size_t N = (size_t)L * L;
   size_t b = N * n_lat * 2 + (size_t)n_lat * (4 + 8 + 4 + 8 + sizeof(rbim_sweep_stats)) +
              (size_t)n_lat * n_bins * RBIM_ACC_SLOTS * 8 +
              (size_t)n_lat * (size_t)std::max(series, 0) * sizeof(rbim_sweep_stats);
-  if (!lds) {
+  if (fused) {
+    const int T = fused_tile_size(L);
+    const size_t tiles = (size_t)(L / T) * (L / T), ns = tiles * 4 * T;
+    b += N * n_lat * (1 + 4) + (size_t)n_lat * (ns * 8 + tiles * (2 * T / 64) * 8 + ns / 8 +
+                                                 tiles * (2 * 5 + 2) * 8 + 2 * 6 * 8 + 8 + 4);
+  } else if (!lds) {
     const size_t tx = (L + 31) / 32;
     b += N * n_lat * (1 + 2 + 8 + 4 + 8) + (size_t)n_lat * tx * tx * 7 * 8 + (size_t)n_lat * 12;
   }
@@ -156,7 +163,33 @@ int group_alloc(rbim_ctx* ctx, Group& g, const std::vector<uint8_t>& jbits,
     HIP_TRY(ctx, g.arena.alloc(&p, sb)); s.series = (rbim_sweep_stats*)p;
     HIP_TRY(ctx, hipMemsetAsync(p, 0, sb, ctx->stream));
   }
-  if (!g.lds) {
+  if (g.fused) {
+    FusedScratch& f = g.fz;
+    f.tile = fused_tile_size(g.L);
+    f.tiles_x = g.L / f.tile;
+    const size_t tiles = (size_t)f.tiles_x * f.tiles_x;
+    f.nslots = (int32_t)(tiles * 4 * f.tile);
+    const size_t nl = (size_t)g.n_lat;
+    auto zalloc = [&](void** q, size_t bytes) -> hipError_t {
+      hipError_t e = g.arena.alloc(q, bytes);
+      return e == hipSuccess ? hipMemsetAsync(*q, 0, bytes, ctx->stream) : e;
+    };
+    HIP_TRY(ctx, zalloc(&p, NN)); f.spins_alt = (int8_t*)p;
+    HIP_TRY(ctx, zalloc(&p, 2 * NN * 2)); f.lab = (uint16_t*)p;
+    HIP_TRY(ctx, zalloc(&p, nl * f.nslots * 8)); f.slots = (uint64_t*)p;
+    HIP_TRY(ctx, zalloc(&p, nl * tiles * (2 * f.tile / 64) * 8)); f.xbits = (uint64_t*)p;
+    HIP_TRY(ctx, zalloc(&p, nl * f.nslots / 8)); f.flipbits = (uint32_t*)p;
+    HIP_TRY(ctx, zalloc(&p, 2 * nl * tiles * 5 * 8)); f.part_int = (int64_t*)p;
+    HIP_TRY(ctx, zalloc(&p, nl * tiles * 2 * 8)); f.part_em = (int64_t*)p;
+    HIP_TRY(ctx, zalloc(&p, 2 * nl * 6 * 8)); f.bst = (int64_t*)p;
+    HIP_TRY(ctx, g.arena.alloc(&p, nl * 8)); f.fin = (int64_t*)p;
+    {
+      std::vector<int64_t> c(g.n_lat, g0);
+      HIP_TRY(ctx, hipMemcpyAsync(p, c.data(), nl * 8, hipMemcpyHostToDevice, ctx->stream));
+      HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    HIP_TRY(ctx, zalloc(&p, nl * 4)); f.cur = (int32_t*)p;
+  } else if (!g.lds) {
     TiledScratch& t = g.tile;
     t.tile = tiled_tile_size(g.L);
     t.tiles_x = t.tiles_y = (g.L + t.tile - 1) / t.tile;
@@ -193,13 +226,18 @@ int group_run(rbim_ctx* ctx, Group& g, int64_t n, hipStream_t stream, int64_t* l
     }
     return RBIM_OK;
   }
+  // one sweep of a tiled group (both layouts)
+  auto sweep = [&g](hipStream_t st) {
+    return g.fused ? launch_fused_sweep(g.set, g.fz, st) : launch_tiled_sweep(g.set, g.tile, st);
+  };
+  const int per_sweep = g.fused ? fused_kernels_per_sweep() : tiled_kernels_per_sweep();
   const int S = std::max(1, g.per_launch);
   if (n >= S && S > 1) {
     if (!g.graph) {
       hipGraph_t graph;
       HIP_TRY(ctx, hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
       for (int i = 0; i < S; ++i) {
-        hipError_t e = launch_tiled_sweep(g.set, g.tile, stream);
+        hipError_t e = sweep(stream);
         if (e != hipSuccess) {
           hipGraph_t junk;
           (void)hipStreamEndCapture(stream, &junk);
@@ -212,18 +250,23 @@ int group_run(rbim_ctx* ctx, Group& g, int64_t n, hipStream_t stream, int64_t* l
     }
     while (n >= S) {
       HIP_TRY(ctx, hipGraphLaunch(g.graph, stream));
-      *launches += (int64_t)S * tiled_kernels_per_sweep();
+      *launches += (int64_t)S * per_sweep;
       n -= S;
     }
   }
   for (int64_t i = 0; i < n; ++i) {
-    HIP_TRY(ctx, launch_tiled_sweep(g.set, g.tile, stream));
-    *launches += tiled_kernels_per_sweep();
+    HIP_TRY(ctx, sweep(stream));
+    *launches += per_sweep;
   }
-  // each sweep's measurement rides in the next sweep's first kernel; the
-  // last one of the run is measured here so the moment sums are complete
-  HIP_TRY(ctx, launch_tiled_flush(g.set, g.tile, stream));
-  *launches += 2;
+  // each sweep's measurement rides in the next sweep's kernels; the last one
+  // of the run is measured here so the moment sums (and spins) are complete
+  if (g.fused) {
+    HIP_TRY(ctx, launch_fused_close(g.set, g.fz, stream));
+    *launches += 3;
+  } else {
+    HIP_TRY(ctx, launch_tiled_flush(g.set, g.tile, stream));
+    *launches += 2;
+  }
   return RBIM_OK;
 }
 
@@ -551,7 +594,7 @@ static int check_desc(rbim_ctx* ctx, const rbim_batch_desc* d) {
   if (d->n_therm < 0) return set_err(ctx, RBIM_E_INVAL, "n_therm must be non-negative");
   if (d->n_therm + (int64_t)d->n_bins * d->sweeps_per_bin > 0xFFFFFFFFll)
     return set_err(ctx, RBIM_E_INVAL, "more than 2^32 sweeps per lattice");
-  if (d->engine < RBIM_ENGINE_AUTO || d->engine > RBIM_ENGINE_BATCHED)
+  if (d->engine < RBIM_ENGINE_AUTO || d->engine > RBIM_ENGINE_TILED)
     return set_err(ctx, RBIM_E_INVAL, "unknown engine %d", d->engine);
   if (d->series < 0) return set_err(ctx, RBIM_E_INVAL, "series must be >= 0");
   if (d->streams < 0 || d->streams > 16)
@@ -574,14 +617,21 @@ static bool use_lds(const rbim_batch_desc* d, int L) {
   return d->engine == RBIM_ENGINE_BATCHED || (d->engine == RBIM_ENGINE_AUTO && L <= RBIM_LDS_MAX_L);
 }
 
+static bool use_fused(const rbim_batch_desc* d, int L, bool lds) {
+  return !lds && d->engine != RBIM_ENGINE_TILED && fused_tile_size(L) > 0;
+}
+
 int rbim_batch_footprint(const rbim_batch_desc* d, uint64_t* bytes) {
   int rc = check_desc(nullptr, d);
   if (rc != RBIM_OK) return rc;
   std::map<std::pair<int, int>, int> counts;
   for (int i = 0; i < d->n_lattices; ++i) counts[{d->L[i], use_lds(d, d->L[i]) ? 1 : 0}]++;
   uint64_t b = 0;
-  for (auto& kv : counts)
-    b += group_bytes(kv.first.first, kv.second, kv.first.second != 0, d->n_bins, d->series);
+  for (auto& kv : counts) {
+    const bool lds = kv.first.second != 0;
+    b += group_bytes(kv.first.first, kv.second, lds, use_fused(d, kv.first.first, lds), d->n_bins,
+                     d->series);
+  }
   if (bytes) *bytes = b;
   return RBIM_OK;
 }
@@ -629,6 +679,7 @@ int rbim_batch_create(rbim_ctx* ctx, const rbim_batch_desc* d, rbim_batch** out)
       g.L = L;
       g.N = L * L;
       g.lds = lds != 0;
+      g.fused = use_fused(d, L, g.lds);
       g.threads = d->threads ? d->threads : default_threads(L);
       g.per_launch = d->sweeps_per_launch > 0 ? d->sweeps_per_launch
                                               : (g.lds ? default_lds_sweeps(L) : 16);
@@ -647,6 +698,7 @@ int rbim_batch_create(rbim_ctx* ctx, const rbim_batch_desc* d, rbim_batch** out)
         h.L = g.L;
         h.N = g.N;
         h.lds = g.lds;
+        h.fused = g.fused;
         h.threads = g.threads;
         h.per_launch = g.per_launch;
         h.idx.assign(g.idx.begin() + n * k / parts, g.idx.begin() + n * (k + 1) / parts);

@@ -51,6 +51,28 @@ struct TiledScratch {
 hipError_t launch_init_spins(const LatticeSet& s, int start, hipStream_t st);
 hipError_t launch_lds_sweeps(const LatticeSet& s, int threads, int sweeps, hipStream_t st);
 size_t lds_bytes(int N);
+// Two-kernel sweep for lattices of several full tiles whose tile-edge slots fit
+// one workgroup's LDS (fused_engine.hip); device pointers, [n_lat][N] unless noted.
+#define RBIM_FUSED_MAX_SLOTS 6400
+struct FusedScratch {
+  int8_t* spins_alt;   // second spin buffer
+  uint16_t* lab;       // [2][n_lat][N] 0, or 0x8000 | rep slot of the site's edge cluster
+  uint64_t* slots;     // [n_lat][nslots] edge-slot entries of the last tile pass
+  uint64_t* xbits;     // [n_lat][tiles][2T/64] occupied bonds leaving each tile (+x, +y)
+  uint32_t* flipbits;  // [n_lat][nslots/32] flip bit of each slot's cluster
+  int64_t* part_int;   // [2][n_lat][tiles][5] interior clusters: n, key1, key2, n_plus1, n_plus2
+  int64_t* part_em;    // [n_lat][tiles][2] E, M of the state a tile pass started from
+  int64_t* bst;        // [2][n_lat][6] edge clusters: n, key1, key2, n_plus1, n_plus2, wrap bits
+  int64_t* fin;        // [n_lat] first sweep not yet folded into the moments
+  int32_t* cur;        // [n_lat] buffer (0 / 1) holding the current state
+  int32_t tile, tiles_x, nslots;
+};
+int fused_tile_size(int L);  // 0 when L does not qualify
+int fused_kernels_per_sweep();
+hipError_t launch_fused_sweep(const LatticeSet& s, const FusedScratch& f, hipStream_t st);
+// Folds the last sweep of a run and applies its pending flips into buffer 0.
+hipError_t launch_fused_close(const LatticeSet& s, const FusedScratch& f, hipStream_t st);
+
 hipError_t launch_tiled_sweep(const LatticeSet& s, const TiledScratch& t, hipStream_t st);
 // Measures and folds the pending sweep at the end of a run of tiled sweeps.
 hipError_t launch_tiled_flush(const LatticeSet& s, const TiledScratch& t, hipStream_t st);

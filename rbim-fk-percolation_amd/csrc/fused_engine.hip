@@ -1,0 +1,722 @@
+// Two-kernel Swendsen-Wang sweep for lattices of many tiles (L a multiple of
+// the tile side T, at most RBIM_FUSED_MAX_SLOTS tile-edge slots per lattice:
+// L <= 320 at T = 64, L <= 224 at T = 32), on gfx950.
+//
+// The tiled path of tiled_engine.hip keeps a forest word, counters and a
+// local-root index for every site in HBM (about 49 B of traffic per
+// site-update).  Here only the spins (1 B) and a 16-bit label per site cross
+// HBM; everything that couples tiles is reduced to the tile-edge sites, which
+// one workgroup per lattice merges in LDS:
+//
+//   KA tile    one workgroup per T x T tile.  Applies the previous sweep's
+//              pending flips (below) to its sites and to the row / column of
+//              sites just beyond its upper edges, measures E and M of that
+//              state, draws the sweep's bonds (Philox4x32-10, as everywhere),
+//              labels the tile's interior bonds in LDS (ballot runs + plaquette
+//              skip + hook-and-compress, no windings: a tile never crosses the
+//              periodic boundary) and classifies every tile-local cluster:
+//                interior  no site on the tile edge: the cluster is final.  Its
+//                          flip (flip bit of its minimum site) is applied at
+//                          once and it is measured here (size, n_plus after the
+//                          flip, two largest keys).
+//                edge      touches the tile edge: its sites keep their spin and
+//                          get label 0x8000 | rep, rep = the cluster's smallest
+//                          edge slot (4T slots per tile: x = x0, x = x0+T-1,
+//                          y = y0, y = y0+T-1).  Every slot records its rep and,
+//                          at reps, the cluster's size, n_plus before the flip,
+//                          minimum site and that site's flip bit.
+//              The occupation of the 2T bonds leaving the tile (+x, +y) goes out
+//              as a bitmask.
+//   KB lattice one workgroup per lattice.  Union-find over the lattice's edge
+//              slots in LDS: slots start under their rep (winding 0: same
+//              tile), every occupied tile-crossing bond unites its two slots
+//              with its periodic winding; a bond that finds one root closes a
+//              loop and c + w(b) - w(a) decides wrap_x / wrap_y exactly
+//              (_unionfind.py:101-124).  Each merged cluster sums its reps'
+//              sizes and n_plus and takes the smallest (min site << 1 | flip):
+//              its flip bit is the flip bit of its minimum site, the rule of
+//              every engine here.  Flip bits go out per slot; the merged
+//              clusters are measured; the previous sweep is folded into its
+//              bin (E, M from this sweep's KA; interior clusters from the
+//              previous KA; edge clusters and wraps from the previous KB).
+//
+// Spins and labels are double-buffered (KA reads one buffer and writes the
+// other), so a tile reading its neighbours' edge sites never races their
+// writes.  After a run of sweeps three small kernels measure the last state,
+// apply the pending flips into buffer 0 and fold the last sweep, so the spins
+// a caller reads are final and a later run starts from a clean state.
+#include "engine.h"
+#include "tile_common.h"
+
+namespace {
+
+constexpr uint32_t LAB_EDGE = 0x8000u;
+constexpr uint32_t NONE16 = 0xFFFFu;
+
+// Slot entry: rep (0..11) | min local index (12..23) | flip (24) |
+// size (25..37) | n_plus before the flip (38..50).
+__device__ __forceinline__ uint64_t slot_entry(uint32_t rep, uint32_t root, uint32_t flip,
+                                               uint32_t size, uint32_t np) {
+  return (uint64_t)rep | ((uint64_t)root << 12) | ((uint64_t)flip << 24) |
+         ((uint64_t)size << 25) | ((uint64_t)np << 38);
+}
+
+// Local index of edge slot k of a T x T tile: side k / T (0: lx = 0,
+// 1: lx = T-1, 2: ly = 0, 3: ly = T-1), position k % T along it.
+template <int T>
+__device__ __forceinline__ uint32_t slot_li(uint32_t k) {
+  const uint32_t side = k / T, i = k % T;
+  const uint32_t lx = side == 0 ? 0u : side == 1 ? (uint32_t)T - 1u : i;
+  const uint32_t ly = side == 2 ? 0u : side == 3 ? (uint32_t)T - 1u : i;
+  return lx * T + ly;
+}
+
+__device__ __forceinline__ uint32_t flip_of(const uint32_t* fb, uint32_t slot) {
+  return (fb[slot >> 5] >> (slot & 31u)) & 1u;
+}
+
+// Buffers of the current state of lattice lat.
+struct FusedView {
+  const int8_t* sin;
+  int8_t* sout;
+  const uint16_t* lin;
+  uint16_t* lout;
+};
+
+__device__ __forceinline__ FusedView fused_view(const LatticeSet& s, const FusedScratch& f, int lat) {
+  const size_t base = (size_t)lat * s.N;
+  const int cur = f.cur[lat];
+  FusedView v;
+  v.sin = (cur ? f.spins_alt : s.spins) + base;
+  v.sout = (cur ? s.spins : f.spins_alt) + base;
+  v.lin = f.lab + ((size_t)cur * s.n_lat + lat) * s.N;
+  v.lout = f.lab + ((size_t)(1 - cur) * s.n_lat + lat) * s.N;
+  return v;
+}
+
+// Spin bit (1: +1) of site n of tile `tile` in the current state, pending
+// edge-cluster flip applied.
+template <int T>
+__device__ __forceinline__ uint32_t true_spin(int8_t v, uint32_t lb, uint32_t tile,
+                                              const uint32_t* fb) {
+  uint32_t b = v > 0 ? 1u : 0u;
+  if (lb & LAB_EDGE) b ^= flip_of(fb, tile * 4u * T + (lb & 0x7FFFu));
+  return b;
+}
+
+// Stage the lattice's flip bits (previous sweep) into LDS.
+__device__ __forceinline__ void load_flipbits(const FusedScratch& f, int lat, uint32_t* fb, int nth) {
+  const uint32_t words = (uint32_t)f.nslots / 32u;
+  const uint32_t* src = f.flipbits + (size_t)lat * words;
+  for (uint32_t i = threadIdx.x; i < words; i += nth) fb[i] = src[i];
+}
+
+// Spin bits of the column beyond x0+T-1 (hsp[0..T)) and the row beyond
+// y0+T-1 (hsp[T..2T)), pending flips applied (sites of the neighbour tiles).
+template <int T>
+__device__ __forceinline__ void load_halo(const LatticeSet& s, const FusedScratch& f,
+                                          const FusedView& v, const uint32_t* fb, uint32_t tx,
+                                          uint32_t ty, uint8_t* hsp) {
+  const uint32_t tid = threadIdx.x;
+  if (tid >= 2u * T) return;
+  const uint32_t L = (uint32_t)s.L, nt = (uint32_t)f.tiles_x;
+  const uint32_t x0 = tx * T, y0 = ty * T;
+  const bool right = tid < (uint32_t)T;
+  const uint32_t i = right ? tid : tid - T;
+  const uint32_t hx = right ? (x0 + T == L ? 0u : x0 + T) : x0 + i;
+  const uint32_t hy = right ? y0 + i : (y0 + T == L ? 0u : y0 + T);
+  const uint32_t ntile = right ? ty * nt + (tx + 1u == nt ? 0u : tx + 1u)
+                               : (ty + 1u == nt ? 0u : ty + 1u) * nt + tx;
+  const uint32_t hn = hx * L + hy;
+  hsp[tid] = (uint8_t)true_spin<T>(v.sin[hn], v.lin[hn], ntile, fb);
+}
+
+// ------------------------------------------------------------------- KA
+template <int T, int NTH>
+__global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
+  constexpr int ITERS = T * T / 2 / NTH;
+  constexpr uint32_t NS = 4u * T;  // edge slots per tile
+  __shared__ uint32_t par[T * T];
+  __shared__ uint32_t lcnt[T * T];
+  __shared__ uint8_t bits[T * T];  // bit0 spin, bit1 occ_h, bit2 occ_v, bit3 flip draw, bit4/5 J<0
+  __shared__ uint8_t hsp[2 * T];
+  __shared__ uint32_t fb[RBIM_FUSED_MAX_SLOTS / 32];
+  __shared__ int64_t red[NTH / 64][PARTS];
+  const int lat = blockIdx.y;
+  if (!lattice_active(s, lat)) return;
+  const uint32_t L = (uint32_t)s.L, N = (uint32_t)s.N;
+  const uint32_t nt = (uint32_t)f.tiles_x, tiles = nt * nt;
+  const uint32_t tile = blockIdx.x, tx = tile % nt, ty = tile / nt;
+  const uint32_t x0 = tx * T, y0 = ty * T;
+  const size_t base = (size_t)lat * N;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint64_t below = (1ull << lane) - 1ull;
+  const int64_t g = s.sweep_ctr[lat];
+  const uint32_t thr = s.thr31[lat];
+  const uint64_t seed = s.seed[lat];
+  const uint32_t lid = s.lat_id[lat];
+  const FusedView v = fused_view(s, f, lat);
+  const uint8_t* jb = s.jbits + base;
+  // E, M of the state this sweep starts from close sweep g-1 (unless a run's
+  // closing pass has already folded it); the clusters of sweep g are measured
+  // when g is past thermalisation
+  const bool meas_em = g >= 1 && f.fin[lat] == g - 1 && g - 1 >= s.n_therm;
+  const bool meas_cl = g >= s.n_therm;
+
+  load_flipbits(f, lat, fb, NTH);
+  __syncthreads();
+  load_halo<T>(s, f, v, fb, tx, ty, hsp);
+  // the tile's own spins (pending flips applied) and couplings into LDS
+#pragma unroll
+  for (int it = 0; it < ITERS; ++it) {
+    const uint32_t ql = (uint32_t)(it * NTH + tid);
+    const uint32_t lx = ql / (T / 2), ly0 = (ql % (T / 2)) * 2u;
+    const uint32_t li0 = ql << 1;
+    const uint32_t n0 = (x0 + lx) * L + y0 + ly0;  // even: the pair loads are aligned
+    const uint32_t sp2 = *reinterpret_cast<const uint16_t*>(v.sin + n0);
+    const uint32_t lb2 = *reinterpret_cast<const uint32_t*>(v.lin + n0);
+    const uint32_t j2 = *reinterpret_cast<const uint16_t*>(jb + n0);
+    const uint32_t t0 = true_spin<T>((int8_t)(sp2 & 0xFFu), lb2 & 0xFFFFu, tile, fb);
+    const uint32_t t1 = true_spin<T>((int8_t)(sp2 >> 8), lb2 >> 16, tile, fb);
+    bits[li0] = (uint8_t)(t0 | (j2 & 3u) << 4);
+    bits[li0 + 1u] = (uint8_t)(t1 | ((j2 >> 8) & 3u) << 4);
+  }
+  __syncthreads();
+
+  // activation (+ E, M of the starting state) and run pre-labels
+  int32_t me = 0, mm = 0;
+#pragma unroll 1
+  for (int it = 0; it < ITERS; ++it) {
+    const uint32_t ql = (uint32_t)(it * NTH + tid);
+    const uint32_t lx = ql / (T / 2), ly0 = (ql % (T / 2)) * 2u;
+    const uint32_t li0 = ql << 1, li1 = li0 + 1u;
+    const uint32_t n0 = (x0 + lx) * L + y0 + ly0;
+    const uint32_t b0 = bits[li0], b1 = bits[li1];
+    const uint32_t sr0 = lx + 1u < T ? bits[li0 + T] & 1u : hsp[ly0];
+    const uint32_t sr1 = lx + 1u < T ? bits[li1 + T] & 1u : hsp[ly0 + 1u];
+    const uint32_t su1 = ly0 + 2u < T ? bits[li1 + 1u] & 1u : hsp[T + lx];
+    // satisfied: J s_a s_b = +1, i.e. equal spins xor J < 0
+    const uint32_t sh0 = ~(b0 ^ sr0 ^ (b0 >> 4)) & 1u;
+    const uint32_t sv0 = ~(b0 ^ b1 ^ (b0 >> 5)) & 1u;
+    const uint32_t sh1 = ~(b1 ^ sr1 ^ (b1 >> 4)) & 1u;
+    const uint32_t sv1 = ~(b1 ^ su1 ^ (b1 >> 5)) & 1u;
+    if (meas_em) {
+      me += 4 - 2 * (int32_t)(sh0 + sv0 + sh1 + sv1);
+      mm += 2 * (int32_t)((b0 & 1u) + (b1 & 1u)) - 2;
+    }
+    const rbim_u32x4 r = rbim_sweep_draws(seed, lid, (uint32_t)g, n0 >> 1);
+    const uint32_t oh0 = sh0 & ((r.x >> 1) < thr ? 1u : 0u);
+    const uint32_t ov0 = sv0 & ((r.y >> 1) < thr ? 1u : 0u);
+    const uint32_t oh1 = sh1 & ((r.z >> 1) < thr ? 1u : 0u);
+    const uint32_t ov1 = sv1 & ((r.w >> 1) < thr ? 1u : 0u);
+    // bit 0 (the spin other lanes read as a neighbour) is rewritten unchanged
+    bits[li0] = (uint8_t)((b0 & 1u) | (oh0 << 1) | (ov0 << 2) | ((r.x & 1u) << 3));
+    bits[li1] = (uint8_t)((b1 & 1u) | (oh1 << 1) | (ov1 << 2) | ((r.z & 1u) << 3));
+    const bool internal = ov0 != 0u;                       // n0 -> n1, same row and tile
+    const bool outlink = ov1 != 0u && ly0 + 1u != T - 1u;  // n1 -> n1+1 inside the tile
+    const uint64_t thru = __ballot(internal && outlink);
+    const uint64_t ol = __ballot(outlink);
+    const uint64_t stop = ~thru & below;
+    const uint32_t chunk = 2u * (uint32_t)(it * NTH + wave * 64);
+    uint32_t start0 = chunk;
+    if (stop != 0ull) {
+      const uint32_t j = 63u - (uint32_t)__clzll(stop);
+      start0 = chunk + 2u * j + (((ol >> j) & 1ull) ? 1u : 2u);
+    }
+    par[li0] = start0;
+    par[li1] = internal ? start0 : li1;
+    lcnt[li0] = 0u;
+    lcnt[li1] = 0u;
+  }
+  if (meas_em) wave_spins(me, mm, red);
+  __syncthreads();
+  if (meas_em && tid == 0) {
+    int64_t e = 0, m = 0;
+    for (int w = 0; w < NTH / 64; ++w) {
+      e += red[w][0];
+      m += red[w][1];
+    }
+    int64_t* o = f.part_em + ((size_t)lat * tiles + tile) * 2;
+    o[0] = e;
+    o[1] = m;
+  }
+
+  // remaining in-tile bonds: x bonds (plaquette skip) and the y bond leaving a
+  // wave's chunk
+#pragma unroll 1
+  for (int it = 0; it < ITERS; ++it) {
+    const uint32_t ql = (uint32_t)(it * NTH + tid);
+    const uint32_t lx = ql / (T / 2), ly0 = (ql % (T / 2)) * 2u;
+    const uint32_t li0 = ql << 1, li1 = li0 + 1u;
+    const uint32_t b0 = bits[li0], b1 = bits[li1];
+    uint32_t req = 0u;
+    if (lx + 1u < T) {
+      uint32_t h0 = (b0 >> 1) & 1u;
+      if (h0 && ly0 != 0u) {
+        const uint32_t left = bits[li0 - 1u], left_up = bits[li0 - 1u + T];
+        h0 &= ~((left >> 1) & (left >> 2) & (left_up >> 2)) & 1u;
+      }
+      uint32_t h1 = (b1 >> 1) & 1u;
+      if (h1) h1 &= ~((b0 >> 1) & (b0 >> 2) & (bits[li0 + T] >> 2)) & 1u;
+      req = h0 | (h1 << 1);
+    }
+    req |= ((b1 >> 2) & 1u & ((lane == 63 && ly0 + 1u != T - 1u) ? 1u : 0u)) << 2;
+    while (__ballot(req != 0u)) {
+      if (req) {
+        const uint32_t k = (uint32_t)__builtin_ctz(req);
+        req &= req - 1u;
+        tunion(par, k == 0u ? li0 : li1, k == 0u ? li0 + T : k == 1u ? li1 + T : li1 + 1u);
+      }
+    }
+  }
+  __syncthreads();
+  // roots: counts (size << 16 | n_plus before the flip); every site is pointed
+  // straight at its root (the forest is settled, so this is only compression)
+#pragma unroll 1
+  for (int it = 0; it < ITERS; ++it) {
+    const uint32_t li0 = (uint32_t)(it * NTH + tid) << 1, li1 = li0 + 1u;
+    const uint32_t b0 = bits[li0], b1 = bits[li1];
+    const uint32_t r0 = troot(par, li0);
+    const uint32_t r1 = (b0 & 4u) ? r0 : troot(par, li1);
+    if (r0 == r1) {
+      atomicAdd(&lcnt[r0], 0x20000u | ((b0 & 1u) + (b1 & 1u)));
+    } else {
+      atomicAdd(&lcnt[r0], 0x10000u | (b0 & 1u));
+      atomicAdd(&lcnt[r1], 0x10000u | (b1 & 1u));
+    }
+    if (r0 != li0) par[li0] = r0;
+    if (r1 != li1) par[li1] = r1;
+  }
+  __syncthreads();
+  // rep slot of every edge cluster: par[root] = root | min slot << 16
+  for (uint32_t li = tid; li < T * T; li += NTH)
+    if (par[li] == li) par[li] = li | (NONE16 << 16);
+  __syncthreads();
+  for (uint32_t k = tid; k < NS; k += NTH) {
+    const uint32_t r = par[slot_li<T>(k)] & 0xFFFFu;
+    atomicMin(&par[r], r | (k << 16));
+  }
+  __syncthreads();
+
+  // new state: interior clusters flipped now, edge clusters labelled
+  TileMeas ms = {0, 0, 0, 0ull, 0ull, 0u, 0u};
+#pragma unroll 1
+  for (int it = 0; it < ITERS; ++it) {
+    const uint32_t ql = (uint32_t)(it * NTH + tid);
+    const uint32_t lx = ql / (T / 2), ly0 = (ql % (T / 2)) * 2u;
+    const uint32_t li0 = ql << 1;
+    const uint32_t n0 = (x0 + lx) * L + y0 + ly0;
+    uint32_t sp2 = 0u, lb2 = 0u;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const uint32_t li = li0 + (uint32_t)j;
+      const uint32_t r = par[li] & 0xFFFFu;
+      const uint32_t rp = par[r];
+      const uint32_t rep = rp >> 16;
+      const uint32_t br = bits[r];
+      const uint32_t fl = (br >> 3) & 1u;
+      const uint32_t sbit = bits[li] & 1u;
+      const bool edge = rep != NONE16;
+      const uint32_t ns = edge ? sbit : sbit ^ fl;
+      sp2 |= (ns ? 0x01u : 0xFFu) << (8 * j);
+      lb2 |= (edge ? (LAB_EDGE | rep) : 0u) << (16 * j);
+      if (meas_cl && !edge && r == li) {  // an interior cluster, at its root
+        const uint32_t c = lcnt[r];
+        const uint32_t size = c >> 16, np = c & 0xFFFFu;
+        const uint32_t site = n0 + (uint32_t)j;
+        ++ms.ncl;
+        top2_insert(ms, ((uint64_t)size << 32) | (uint64_t)(0xFFFFFFFFu - site), fl ? size - np : np);
+      }
+    }
+    *reinterpret_cast<uint16_t*>(v.sout + n0) = (uint16_t)sp2;
+    *reinterpret_cast<uint32_t*>(v.lout + n0) = lb2;
+  }
+  // edge slots for KB, and the occupation of the bonds leaving the tile
+  uint64_t* sl = f.slots + (size_t)lat * f.nslots + (size_t)tile * NS;
+  for (uint32_t k = tid; k < NS; k += NTH) {
+    const uint32_t r = par[slot_li<T>(k)] & 0xFFFFu;
+    const uint32_t rep = par[r] >> 16;
+    const uint32_t c = lcnt[r];
+    sl[k] = slot_entry(rep, r, (bits[r] >> 3) & 1u, c >> 16, c & 0xFFFFu);
+  }
+  if (tid < 2 * T) {
+    // tid < T: +x bond of (x0+T-1, y0+tid); else +y bond of (x0+tid-T, y0+T-1)
+    const uint32_t occ = tid < T ? (bits[(T - 1) * T + tid] >> 1) & 1u
+                                 : (bits[(tid - T) * T + T - 1] >> 2) & 1u;
+    const uint64_t word = __ballot(occ != 0u);
+    if (lane == 0) f.xbits[((size_t)lat * tiles + tile) * (2 * T / 64) + wave] = word;
+  }
+  if (meas_cl) {
+    wave_clusters(ms, red);
+    __syncthreads();
+    if (tid == 0) {
+      TileMeas a = {0, 0, 0, 0ull, 0ull, 0u, 0u};
+      int64_t ncl = 0;
+      for (int w = 0; w < NTH / 64; ++w) {
+        ncl += red[w][2];
+        top2_insert(a, (uint64_t)red[w][3], (uint32_t)red[w][5]);
+        top2_insert(a, (uint64_t)red[w][4], (uint32_t)red[w][6]);
+      }
+      int64_t* o = f.part_int + (((size_t)(g & 1) * s.n_lat + lat) * tiles + tile) * 5;
+      o[0] = ncl;
+      o[1] = (int64_t)a.k1;
+      o[2] = (int64_t)a.k2;
+      o[3] = a.p1;
+      o[4] = a.p2;
+    }
+  }
+}
+
+// ------------------------------------------------------------------- KB
+__device__ __forceinline__ uint64_t kpack(uint32_t p, int wx, int wy) {
+  return (uint64_t)p | ((uint64_t)(uint16_t)(int16_t)wx << 32) |
+         ((uint64_t)(uint16_t)(int16_t)wy << 48);
+}
+__device__ __forceinline__ uint32_t kpar(uint64_t v) { return (uint32_t)v; }
+__device__ __forceinline__ int kwx(uint64_t v) { return (int)(int16_t)(uint16_t)(v >> 32); }
+__device__ __forceinline__ int kwy(uint64_t v) { return (int)(int16_t)(uint16_t)(v >> 48); }
+
+// Root of x in the slot forest with the winding of the path x -> root; path
+// halving (stores only touch non-root words, which hooks never target).
+__device__ __forceinline__ uint32_t kfind(uint64_t* uf, uint32_t x, int& wx, int& wy) {
+  uint64_t vx = uf[x];
+  while (true) {
+    const uint32_t p = kpar(vx);
+    if (p == x) return x;
+    const uint64_t vp = uf[p];
+    const uint32_t gp = kpar(vp);
+    if (gp == p) {
+      wx += kwx(vx);
+      wy += kwy(vx);
+      return p;
+    }
+    const int hx = kwx(vx) + kwx(vp), hy = kwy(vx) + kwy(vp);
+    uf[x] = kpack(gp, hx, hy);
+    wx += hx;
+    wy += hy;
+    x = gp;
+    vx = uf[gp];
+  }
+}
+
+__device__ __forceinline__ uint32_t kroot(const uint64_t* uf, uint32_t x) {
+  uint32_t p = kpar(uf[x]);
+  while (p != x) {
+    x = p;
+    p = kpar(uf[x]);
+  }
+  return x;
+}
+
+// Occupied bond a -> b crossing the periodic boundary (cx, cy) times; wrap
+// bits (1: x, 2: y) when it closes a loop.
+__device__ uint32_t kunion(uint64_t* uf, uint32_t a, uint32_t b, int cx, int cy) {
+  int ax = 0, ay = 0, bx = 0, by = 0;
+  while (true) {
+    a = kfind(uf, a, ax, ay);
+    b = kfind(uf, b, bx, by);
+    if (a == b) return ((cx + bx - ax) != 0 ? 1u : 0u) | ((cy + by - ay) != 0 ? 2u : 0u);
+    if (a < b) {
+      if (atomicCAS((unsigned long long*)&uf[b], (unsigned long long)kpack(b, 0, 0),
+                    (unsigned long long)kpack(a, ax - bx - cx, ay - by - cy)) == kpack(b, 0, 0))
+        return 0u;
+    } else {
+      if (atomicCAS((unsigned long long*)&uf[a], (unsigned long long)kpack(a, 0, 0),
+                    (unsigned long long)kpack(b, bx - ax + cx, by - ay + cy)) == kpack(a, 0, 0))
+        return 0u;
+    }
+  }
+}
+
+// Fold measured sweep gm of lattice lat into its bin: E, M from part_em,
+// interior clusters from part_int[gm & 1], edge clusters and wraps from
+// bst[gm & 1].  Single thread.
+__device__ void fused_finalize(const LatticeSet& s, const FusedScratch& f, int lat, int64_t gm) {
+  const int tiles = f.tiles_x * f.tiles_x;
+  const int64_t* em = f.part_em + (size_t)lat * tiles * 2;
+  const int64_t* pi = f.part_int + ((size_t)(gm & 1) * s.n_lat + lat) * tiles * 5;
+  const int64_t* eb = f.bst + ((size_t)(gm & 1) * s.n_lat + lat) * 6;
+  int64_t e = 0, m = 0, ncl = eb[0];
+  TileMeas a = {0, 0, 0, 0ull, 0ull, 0u, 0u};
+  top2_insert(a, (uint64_t)eb[1], (uint32_t)eb[3]);
+  top2_insert(a, (uint64_t)eb[2], (uint32_t)eb[4]);
+  for (int i = 0; i < tiles; ++i) {
+    e += em[2 * i];
+    m += em[2 * i + 1];
+    ncl += pi[5 * i];
+    top2_insert(a, (uint64_t)pi[5 * i + 1], (uint32_t)pi[5 * i + 3]);
+    top2_insert(a, (uint64_t)pi[5 * i + 2], (uint32_t)pi[5 * i + 4]);
+  }
+  rbim_sweep_stats st;
+  st.energy = e;
+  st.magnetization = m;
+  st.n_clusters = ncl;
+  const int64_t s1 = (int64_t)(a.k1 >> 32), mc1 = 2 * (int64_t)a.p1 - s1;
+  st.largest_size = s1;
+  st.largest_abs_mag = mc1 < 0 ? -mc1 : mc1;
+  if (a.k2 != 0ull) {
+    const int64_t s2 = (int64_t)(a.k2 >> 32), mc2 = 2 * (int64_t)a.p2 - s2;
+    st.second_size = s2;
+    st.second_abs_mag = mc2 < 0 ? -mc2 : mc2;
+  } else {
+    st.second_size = 0;
+    st.second_abs_mag = 0;
+  }
+  st.wrap_x = (int32_t)(eb[5] & 1);
+  st.wrap_y = (int32_t)((eb[5] >> 1) & 1);
+  const int64_t bin = (gm - s.n_therm) / s.spb;
+  acc_sweep(s.acc + ((size_t)lat * s.n_bins + bin) * RBIM_ACC_SLOTS, st);
+  if (s.last) s.last[lat] = st;
+  series_put(s, lat, gm, st);
+}
+
+constexpr int KB_THREADS = 1024;
+
+template <int T>
+__global__ __launch_bounds__(KB_THREADS) void kb_merge(LatticeSet s, FusedScratch f) {
+  constexpr uint32_t NS = 4u * T;
+  extern __shared__ uint64_t kbsm[];
+  __shared__ int64_t red[KB_THREADS / 64][PARTS];
+  __shared__ uint32_t wrapbits;
+  const int lat = blockIdx.x;
+  if (!lattice_active(s, lat)) return;
+  const uint32_t ns = (uint32_t)f.nslots, nt = (uint32_t)f.tiles_x, L = (uint32_t)s.L;
+  uint64_t* uf = kbsm;                                 // parent | wx:16 | wy:16
+  uint64_t* cnt = kbsm + ns;                           // size << 32 | n_plus, at reps
+  uint32_t* mn = reinterpret_cast<uint32_t*>(kbsm + 2 * ns);  // min site << 1 | flip, at reps
+  const int tid = threadIdx.x;
+  const int64_t g = s.sweep_ctr[lat];
+  const uint64_t* ent = f.slots + (size_t)lat * ns;
+  if (tid == 0) wrapbits = 0u;
+  for (uint32_t k = tid; k < ns; k += KB_THREADS) {
+    const uint64_t e = ent[k];
+    const uint32_t t = k / NS, loc = k % NS, rep = (uint32_t)(e & 0xFFFu);
+    uf[k] = kpack(t * NS + rep, 0, 0);
+    if (rep == loc) {
+      const uint32_t r = (uint32_t)(e >> 12) & 0xFFFu, fl = (uint32_t)(e >> 24) & 1u;
+      const uint32_t size = (uint32_t)(e >> 25) & 0x1FFFu, np = (uint32_t)(e >> 38) & 0x1FFFu;
+      const uint32_t site = ((t % nt) * T + r / T) * L + (t / nt) * T + r % T;
+      cnt[k] = ((uint64_t)size << 32) | np;
+      mn[k] = (site << 1) | fl;
+    } else {
+      mn[k] = 0xFFFFFFFFu;
+    }
+  }
+  __syncthreads();
+  // tile-crossing bonds: tid-th bond of tile t: i < T is the +x bond of slot
+  // (side 1, i) into the right neighbour's (side 0, i); else the +y bond of
+  // (side 3, i-T) into the upper neighbour's (side 2, i-T)
+  const uint32_t nb = (ns / NS) * 2u * T;
+  uint32_t wraps = 0u;
+  for (uint32_t b = tid; b < nb; b += KB_THREADS) {
+    const uint32_t t = b / (2u * T), i = b % (2u * T);
+    const uint64_t word = f.xbits[((size_t)lat * (ns / NS) + t) * (2 * T / 64) + i / 64];
+    if (!((word >> (i % 64)) & 1ull)) continue;
+    const uint32_t tx = t % nt, ty = t / nt;
+    if (i < (uint32_t)T) {
+      const uint32_t tr = ty * nt + (tx + 1u == nt ? 0u : tx + 1u);
+      wraps |= kunion(uf, t * NS + T + i, tr * NS + i, tx + 1u == nt ? 1 : 0, 0);
+    } else {
+      const uint32_t j = i - T;
+      const uint32_t tu = (ty + 1u == nt ? 0u : ty + 1u) * nt + tx;
+      wraps |= kunion(uf, t * NS + 3u * T + j, tu * NS + 2u * T + j, 0, ty + 1u == nt ? 1 : 0);
+    }
+  }
+  if (wraps) atomicOr(&wrapbits, wraps);
+  __syncthreads();
+  // reps fold into their roots
+  for (uint32_t k = tid; k < ns; k += KB_THREADS) {
+    if (mn[k] == 0xFFFFFFFFu) continue;
+    const uint32_t r = kroot(uf, k);
+    if (r != k) {
+      atomicAdd((unsigned long long*)&cnt[r], (unsigned long long)cnt[k]);
+      atomicMin(&mn[r], mn[k]);
+    }
+  }
+  __syncthreads();
+  // flip bits per slot; merged clusters measured at their roots
+  const bool meas_cl = g >= s.n_therm;
+  TileMeas ms = {0, 0, 0, 0ull, 0ull, 0u, 0u};
+  uint32_t* fbo = f.flipbits + (size_t)lat * (ns / 32u);
+  for (uint32_t k0 = 0; k0 < ns; k0 += KB_THREADS) {
+    const uint32_t k = k0 + tid;
+    uint32_t fl = 0u;
+    if (k < ns) {
+      const uint32_t r = kroot(uf, k);
+      const uint32_t m = mn[r];
+      fl = m & 1u;
+      if (meas_cl && r == k) {
+        const uint64_t c = cnt[r];
+        const uint32_t size = (uint32_t)(c >> 32), np = (uint32_t)c;
+        ++ms.ncl;
+        top2_insert(ms, ((uint64_t)size << 32) | (uint64_t)(0xFFFFFFFFu - (m >> 1)),
+                    fl ? size - np : np);
+      }
+    }
+    const uint64_t word = __ballot(fl != 0u);
+    if ((tid & 63) == 0 && k < ns) {
+      fbo[k / 32] = (uint32_t)word;
+      if (k + 32 < ns) fbo[k / 32 + 1] = (uint32_t)(word >> 32);
+    }
+  }
+  if (meas_cl) wave_clusters(ms, red);
+  __syncthreads();
+  if (tid == 0) {
+    if (meas_cl) {
+      TileMeas a = {0, 0, 0, 0ull, 0ull, 0u, 0u};
+      int64_t ncl = 0;
+      for (int w = 0; w < KB_THREADS / 64; ++w) {
+        ncl += red[w][2];
+        top2_insert(a, (uint64_t)red[w][3], (uint32_t)red[w][5]);
+        top2_insert(a, (uint64_t)red[w][4], (uint32_t)red[w][6]);
+      }
+      int64_t* o = f.bst + ((size_t)(g & 1) * s.n_lat + lat) * 6;
+      o[0] = ncl;
+      o[1] = (int64_t)a.k1;
+      o[2] = (int64_t)a.k2;
+      o[3] = a.p1;
+      o[4] = a.p2;
+      o[5] = wrapbits;
+    }
+    // sweep g-1 is complete now (its E, M came from this sweep's KA)
+    if (g >= 1 && f.fin[lat] == g - 1) {
+      if (g - 1 >= s.n_therm) fused_finalize(s, f, lat, g - 1);
+      f.fin[lat] = g;
+    }
+    f.cur[lat] ^= 1;
+    s.sweep_ctr[lat] = g + 1;
+  }
+}
+
+// ------------------------------------------------------------ run closing
+// E, M of the current state (pending flips applied), when the last sweep
+// still has to be folded.
+template <int T, int NTH>
+__global__ __launch_bounds__(NTH) void kf_measure(LatticeSet s, FusedScratch f) {
+  __shared__ uint8_t bits[T * T];
+  __shared__ uint8_t hsp[2 * T];
+  __shared__ uint32_t fb[RBIM_FUSED_MAX_SLOTS / 32];
+  __shared__ int64_t red[NTH / 64][PARTS];
+  const int lat = blockIdx.y;
+  const int64_t G = s.sweep_ctr[lat] - 1;
+  if (!(G >= 0 && f.fin[lat] == G && G >= s.n_therm)) return;
+  const uint32_t L = (uint32_t)s.L, N = (uint32_t)s.N;
+  const uint32_t nt = (uint32_t)f.tiles_x, tiles = nt * nt;
+  const uint32_t tile = blockIdx.x, tx = tile % nt, ty = tile / nt;
+  const uint32_t x0 = tx * T, y0 = ty * T;
+  const FusedView v = fused_view(s, f, lat);
+  const uint8_t* jb = s.jbits + (size_t)lat * N;
+  load_flipbits(f, lat, fb, NTH);
+  __syncthreads();
+  load_halo<T>(s, f, v, fb, tx, ty, hsp);
+  for (uint32_t li = threadIdx.x; li < T * T; li += NTH) {
+    const uint32_t n = (x0 + li / T) * L + y0 + li % T;
+    bits[li] = (uint8_t)(true_spin<T>(v.sin[n], v.lin[n], tile, fb) | (jb[n] & 3u) << 4);
+  }
+  __syncthreads();
+  int32_t e = 0, m = 0;
+  for (uint32_t li = threadIdx.x; li < T * T; li += NTH) {
+    const uint32_t lx = li / T, ly = li % T;
+    const uint32_t b = bits[li];
+    const uint32_t sr = lx + 1u < T ? bits[li + T] & 1u : hsp[ly];
+    const uint32_t su = ly + 1u < T ? bits[li + 1u] & 1u : hsp[T + lx];
+    const uint32_t sh = ~(b ^ sr ^ (b >> 4)) & 1u, sv = ~(b ^ su ^ (b >> 5)) & 1u;
+    e += 2 - 2 * (int32_t)(sh + sv);
+    m += 2 * (int32_t)(b & 1u) - 1;
+  }
+  wave_spins(e, m, red);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t es = 0, msum = 0;
+    for (int w = 0; w < NTH / 64; ++w) {
+      es += red[w][0];
+      msum += red[w][1];
+    }
+    int64_t* o = f.part_em + ((size_t)lat * tiles + tile) * 2;
+    o[0] = es;
+    o[1] = msum;
+  }
+}
+
+// Pending flips applied into buffer 0; labels cleared.
+template <int T, int NTH>
+__global__ __launch_bounds__(NTH) void kf_settle(LatticeSet s, FusedScratch f) {
+  __shared__ uint32_t fb[RBIM_FUSED_MAX_SLOTS / 32];
+  const int lat = blockIdx.y;
+  const uint32_t L = (uint32_t)s.L, N = (uint32_t)s.N;
+  const uint32_t nt = (uint32_t)f.tiles_x;
+  const uint32_t tile = blockIdx.x, tx = tile % nt, ty = tile / nt;
+  const uint32_t x0 = tx * T, y0 = ty * T;
+  const FusedView v = fused_view(s, f, lat);
+  int8_t* s0 = s.spins + (size_t)lat * N;
+  uint16_t* l0 = f.lab + (size_t)lat * N;
+  load_flipbits(f, lat, fb, NTH);
+  __syncthreads();
+  for (uint32_t li = threadIdx.x; li < T * T; li += NTH) {
+    const uint32_t n = (x0 + li / T) * L + y0 + li % T;
+    const uint32_t b = true_spin<T>(v.sin[n], v.lin[n], tile, fb);
+    s0[n] = b ? (int8_t)1 : (int8_t)-1;
+    l0[n] = 0u;
+  }
+}
+
+__global__ void kf_close(LatticeSet s, FusedScratch f) {
+  const int lat = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lat >= s.n_lat) return;
+  const int64_t G = s.sweep_ctr[lat] - 1;
+  if (G >= 0 && f.fin[lat] == G) {
+    if (G >= s.n_therm) fused_finalize(s, f, lat, G);
+    f.fin[lat] = G + 1;
+  }
+  f.cur[lat] = 0;
+}
+
+size_t kb_lds_bytes(int nslots) { return (size_t)nslots * (8 + 8 + 4); }
+
+template <int T>
+hipError_t launch_fused_t(const LatticeSet& s, const FusedScratch& f, hipStream_t st) {
+  constexpr int NTH = T == 64 ? 512 : 256;
+  static bool configured = false;
+  if (!configured) {
+    hipError_t e = hipFuncSetAttribute((const void*)kb_merge<T>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)kb_lds_bytes(RBIM_FUSED_MAX_SLOTS));
+    if (e != hipSuccess) return e;
+    configured = true;
+  }
+  const int tiles = f.tiles_x * f.tiles_x;
+  hipLaunchKernelGGL((ka_tile<T, NTH>), dim3(tiles, s.n_lat), dim3(NTH), 0, st, s, f);
+  hipLaunchKernelGGL((kb_merge<T>), dim3(s.n_lat), dim3(KB_THREADS), kb_lds_bytes(f.nslots), st, s, f);
+  return hipGetLastError();
+}
+
+template <int T>
+hipError_t launch_fused_close_t(const LatticeSet& s, const FusedScratch& f, hipStream_t st) {
+  constexpr int NTH = T == 64 ? 512 : 256;
+  const int tiles = f.tiles_x * f.tiles_x;
+  hipLaunchKernelGGL((kf_measure<T, NTH>), dim3(tiles, s.n_lat), dim3(NTH), 0, st, s, f);
+  hipLaunchKernelGGL((kf_settle<T, NTH>), dim3(tiles, s.n_lat), dim3(NTH), 0, st, s, f);
+  hipLaunchKernelGGL(kf_close, dim3((s.n_lat + 63) / 64), dim3(64), 0, st, s, f);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+int fused_tile_size(int L) {
+  for (int T : {64, 32}) {
+    if (L % T != 0) continue;
+    const int nt = L / T;
+    if ((size_t)nt * nt * 4 * T <= RBIM_FUSED_MAX_SLOTS) return T;
+  }
+  return 0;
+}
+
+int fused_kernels_per_sweep() { return 2; }
+
+hipError_t launch_fused_sweep(const LatticeSet& s, const FusedScratch& f, hipStream_t st) {
+  return f.tile == 64 ? launch_fused_t<64>(s, f, st) : launch_fused_t<32>(s, f, st);
+}
+
+hipError_t launch_fused_close(const LatticeSet& s, const FusedScratch& f, hipStream_t st) {
+  return f.tile == 64 ? launch_fused_close_t<64>(s, f, st) : launch_fused_close_t<32>(s, f, st);
+}

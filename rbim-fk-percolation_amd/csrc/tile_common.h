@@ -1,0 +1,124 @@
+// Device helpers shared by the tiled kernels (tiled_engine.hip) and the
+// fused tile kernels (fused_engine.hip): tile-local union-find in LDS and the
+// per-tile measurement partials.
+#pragma once
+#include "engine.h"
+
+namespace {
+
+constexpr int PARTS = 7;  // e, m, n_clusters, key1, key2, n_plus1, n_plus2
+
+__device__ __forceinline__ bool lattice_active(const LatticeSet& s, int lat) {
+  return s.sweep_ctr[lat] < s.total;
+}
+
+// Per-tile partial of one sweep's measurement: E, M, n_clusters and the two
+// largest clusters (key = size << 32 | ~root, ties to the smaller root) with
+// their n_plus carried along, so the partial is final even after the next
+// sweep has reused the cluster counters.
+struct TileMeas {
+  int32_t e, m, ncl;  // one tile: |e| <= 2 T^2
+  uint64_t k1, k2;
+  uint32_t p1, p2;
+};
+
+__device__ __forceinline__ void top2_insert(TileMeas& a, uint64_t k, uint32_t p) {
+  if (k > a.k1) {
+    a.k2 = a.k1;
+    a.p2 = a.p1;
+    a.k1 = k;
+    a.p1 = p;
+  } else if (k > a.k2) {
+    a.k2 = k;
+    a.p2 = p;
+  }
+}
+
+// Workgroup reduction of the lanes' partials in two halves, so the cluster
+// part (read before the sweep overwrites the roots) and the spin part (summed
+// while the sweep reads the spins) need not be live at the same time: each
+// wave reduces into red[wave][.] and, after a barrier, thread 0 combines.
+__device__ void wave_clusters(TileMeas a, int64_t (*red)[PARTS]) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a.ncl += __shfl_xor(a.ncl, off);
+    const uint64_t o1 = __shfl_xor(a.k1, off), o2 = __shfl_xor(a.k2, off);
+    const uint32_t q1 = __shfl_xor(a.p1, off), q2 = __shfl_xor(a.p2, off);
+    top2_insert(a, o1, q1);
+    top2_insert(a, o2, q2);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    int64_t* r = red[threadIdx.x >> 6];
+    r[2] = a.ncl;
+    r[3] = (int64_t)a.k1;
+    r[4] = (int64_t)a.k2;
+    r[5] = a.p1;
+    r[6] = a.p2;
+  }
+}
+
+__device__ void wave_spins(int32_t e, int32_t m, int64_t (*red)[PARTS]) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    e += __shfl_xor(e, off);
+    m += __shfl_xor(m, off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[threadIdx.x >> 6][0] = e;
+    red[threadIdx.x >> 6][1] = m;
+  }
+}
+
+// Tile-local union-find in LDS over tile indices (no windings: a tile never
+// crosses the periodic boundary).  Two finds walked together with path
+// halving; the larger root hooks under the smaller, so every root is the
+// minimum index (= minimum site) of its tile-local cluster.
+__device__ __forceinline__ void tfind2(uint32_t* par, uint32_t& a, uint32_t& b) {
+  uint32_t pa = par[a], pb = par[b];
+  while (pa != a || pb != b) {
+    const uint32_t ga = pa != a ? par[pa] : pa;
+    const uint32_t gb = pb != b ? par[pb] : pb;
+    if (pa != a) {
+      if (ga == pa) {
+        a = pa;
+      } else {
+        par[a] = ga;
+        a = ga;
+        pa = par[ga];
+      }
+    }
+    if (pb != b) {
+      if (gb == pb) {
+        b = pb;
+      } else {
+        par[b] = gb;
+        b = gb;
+        pb = par[gb];
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void tunion(uint32_t* par, uint32_t a, uint32_t b) {
+  while (true) {
+    tfind2(par, a, b);
+    if (a == b) return;
+    const uint32_t hi = max(a, b), lo = min(a, b);
+    if (atomicCAS(&par[hi], hi, lo) == hi) return;
+    a = lo;
+    b = hi;
+  }
+}
+
+__device__ __forceinline__ uint32_t troot(const uint32_t* par, uint32_t x) {
+  uint32_t p = par[x];
+  while (p != x) {
+    const uint32_t g = par[p];
+    if (g == p) return p;
+    x = g;
+    p = par[x];
+  }
+  return x;
+}
+
+}  // namespace

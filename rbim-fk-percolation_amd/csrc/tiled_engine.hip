@@ -29,10 +29,9 @@
 // local roots.  Every kernel reads the lattice's own sweep counter from HBM,
 // so a captured graph of sweeps replays without host arguments changing.
 #include "engine.h"
+#include "tile_common.h"
 
 namespace {
-
-constexpr int PARTS = 7;  // e, m, n_clusters, key1, key2, n_plus1, n_plus2
 
 __device__ __forceinline__ uint64_t gpack(uint32_t p, int wx, int wy) {
   return (uint64_t)p | ((uint64_t)(uint16_t)(int16_t)wx << 32) |
@@ -88,33 +87,7 @@ __device__ __forceinline__ uint32_t groot(uint64_t* gf, uint32_t x) {
   return x;
 }
 
-__device__ __forceinline__ bool lattice_active(const LatticeSet& s, int lat) {
-  return s.sweep_ctr[lat] < s.total;
-}
-
 // ------------------------------------------------------------ measurement
-// Per-tile partial of one sweep's measurement: E, M, n_clusters and the two
-// largest clusters (key = size << 32 | ~root, ties to the smaller root) with
-// their n_plus carried along, so the partial is final even after the next
-// sweep has reused the cluster counters.
-struct TileMeas {
-  int32_t e, m, ncl;  // one tile: |e| <= 2 T^2
-  uint64_t k1, k2;
-  uint32_t p1, p2;
-};
-
-__device__ __forceinline__ void top2_insert(TileMeas& a, uint64_t k, uint32_t p) {
-  if (k > a.k1) {
-    a.k2 = a.k1;
-    a.p2 = a.p1;
-    a.k1 = k;
-    a.p1 = p;
-  } else if (k > a.k2) {
-    a.k2 = k;
-    a.p2 = p;
-  }
-}
-
 // Bits of the per-site occ byte: bonds and flip bit of the sweep, and -- set by
 // K3 at the root of every cluster -- the root flag the next measurement reads.
 constexpr uint32_t OCC_ROOT = 8u;
@@ -129,41 +102,6 @@ __device__ __forceinline__ int bond_energy(uint32_t j, int s0, int sr, int su) {
 __device__ __forceinline__ void meas_root(TileMeas& a, uint64_t c, uint32_t n) {
   ++a.ncl;
   top2_insert(a, ((c >> 32) << 32) | (uint64_t)(0xFFFFFFFFu - n), (uint32_t)c);
-}
-
-// Workgroup reduction of the lanes' partials in two halves, so the cluster
-// part (read before the sweep overwrites the roots) and the spin part (summed
-// while the sweep reads the spins) need not be live at the same time: each
-// wave reduces into red[wave][.] and, after a barrier, thread 0 combines.
-__device__ void wave_clusters(TileMeas a, int64_t (*red)[PARTS]) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    a.ncl += __shfl_xor(a.ncl, off);
-    const uint64_t o1 = __shfl_xor(a.k1, off), o2 = __shfl_xor(a.k2, off);
-    const uint32_t q1 = __shfl_xor(a.p1, off), q2 = __shfl_xor(a.p2, off);
-    top2_insert(a, o1, q1);
-    top2_insert(a, o2, q2);
-  }
-  if ((threadIdx.x & 63) == 0) {
-    int64_t* r = red[threadIdx.x >> 6];
-    r[2] = a.ncl;
-    r[3] = (int64_t)a.k1;
-    r[4] = (int64_t)a.k2;
-    r[5] = a.p1;
-    r[6] = a.p2;
-  }
-}
-
-__device__ void wave_spins(int32_t e, int32_t m, int64_t (*red)[PARTS]) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    e += __shfl_xor(e, off);
-    m += __shfl_xor(m, off);
-  }
-  if ((threadIdx.x & 63) == 0) {
-    red[threadIdx.x >> 6][0] = e;
-    red[threadIdx.x >> 6][1] = m;
-  }
 }
 
 template <int NTH>
@@ -317,63 +255,15 @@ __global__ __launch_bounds__(NTH) void k1_tile(LatticeSet s, TiledScratch t) {
   }
 }
 
-// Full 32x32 tiles (L a multiple of 32): the LDS kernel's scheme on an open
+// Full T x T tiles (L a multiple of T): the LDS kernel's scheme on an open
 // tile.  Each lane owns a site pair (one Philox call); occupied in-row bonds
-// inside a wavefront's 128-site chunk (four tile rows) are merged by two
-// ballots (every site points at the start of its run); the remaining in-tile
-// bonds go through a winding-free union-find in LDS, skipping any x bond that
-// closes an occupied plaquette with its left neighbour (a contractible loop:
-// connectivity and windings are unchanged).
-__device__ __forceinline__ void tfind2(uint32_t* par, uint32_t& a, uint32_t& b) {
-  uint32_t pa = par[a], pb = par[b];
-  while (pa != a || pb != b) {
-    const uint32_t ga = pa != a ? par[pa] : pa;
-    const uint32_t gb = pb != b ? par[pb] : pb;
-    if (pa != a) {
-      if (ga == pa) {
-        a = pa;
-      } else {
-        par[a] = ga;
-        a = ga;
-        pa = par[ga];
-      }
-    }
-    if (pb != b) {
-      if (gb == pb) {
-        b = pb;
-      } else {
-        par[b] = gb;
-        b = gb;
-        pb = par[gb];
-      }
-    }
-  }
-}
-
-__device__ __forceinline__ void tunion(uint32_t* par, uint32_t a, uint32_t b) {
-  while (true) {
-    tfind2(par, a, b);
-    if (a == b) return;
-    const uint32_t hi = max(a, b), lo = min(a, b);
-    if (atomicCAS(&par[hi], hi, lo) == hi) return;
-    a = lo;
-    b = hi;
-  }
-}
-
-__device__ __forceinline__ uint32_t troot(const uint32_t* par, uint32_t x) {
-  uint32_t p = par[x];
-  while (p != x) {
-    const uint32_t g = par[p];
-    if (g == p) return p;
-    x = g;
-    p = par[x];
-  }
-  return x;
-}
-
+// inside a wavefront's 128-site chunk are merged by two ballots (every site
+// points at the start of its run); the remaining in-tile bonds go through a
+// winding-free union-find in LDS, skipping any x bond that closes an occupied
+// plaquette with its left neighbour (a contractible loop: connectivity and
+// windings are unchanged).  Bounded to 8 waves per SIMD: the LDS of four
+// 512-thread 64 x 64 tiles fits a CU.
 template <int T, int NTH>
-// (8 waves per SIMD: the LDS of four 512-thread tiles fits a CU)
 __global__ __launch_bounds__(NTH, 8) void k1_tile_full(LatticeSet s, TiledScratch t) {
   __shared__ int64_t red[NTH / 64][PARTS];
   __shared__ uint32_t par[T * T];

@@ -20,6 +20,7 @@ __all__ = [
     "ENGINE_AUTO",
     "ENGINE_PLAIN",
     "ENGINE_BATCHED",
+    "ENGINE_TILED",
     "EngineError",
     "EngineUnavailable",
     "SweepStats",
@@ -33,7 +34,7 @@ __all__ = [
 ]
 
 ACC_SLOTS = 20
-ENGINE_AUTO, ENGINE_PLAIN, ENGINE_BATCHED = 0, 1, 2
+ENGINE_AUTO, ENGINE_PLAIN, ENGINE_BATCHED, ENGINE_TILED = 0, 1, 2, 3
 START_HOT, START_COLD = 0, 1
 
 RBIM_OK = 0

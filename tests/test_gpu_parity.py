@@ -248,3 +248,36 @@ def test_run_past_end_is_refused(hip):
         batch.run(3)
         with pytest.raises(EngineError):
             batch.run(1)
+
+
+@pytest.mark.parametrize("Ls", [(32, 64, 96), (128, 160, 256)])
+def test_edge_slot_sweep_equals_hbm_forest_sweep(hip, coracle, Ls):
+    """The two-kernel edge-slot sweep (ENGINE_PLAIN where L qualifies) and the
+    multi-kernel HBM-forest sweep (ENGINE_TILED) give identical trajectories,
+    moment sums and per-sweep series, across split runs; and both equal the
+    oracle."""
+    from rbim._hip import ENGINE_TILED
+    from rbim.engine import Batch
+
+    specs = _specs(Ls, seed=12, temps=(1.3, 2.27, 3.1))
+    n_therm, spb, n_bins = 5, 4, 3
+    total = n_therm + spb * n_bins
+    out = []
+    for engine, per_launch in [(PLAIN, 3), (ENGINE_TILED, 4), (PLAIN, 1)]:
+        with Batch(specs, n_therm=n_therm, n_bins=n_bins, sweeps_per_bin=spb, engine=engine,
+                   sweeps_per_launch=per_launch, series=spb * n_bins) as batch:
+            batch.run(2)   # the first run ends inside thermalisation
+            batch.run(7)   # this one inside a bin
+            batch.run()
+            out.append((batch.moments(), batch.spins(), batch.series(), batch.info()))
+    assert out[0][3].tiled_groups >= 1
+    for acc, spins, series, _ in out[1:]:
+        assert np.array_equal(acc, out[0][0])
+        assert np.array_equal(series, out[0][2])
+        for a, b in zip(spins, out[0][1]):
+            assert np.array_equal(a, b)
+    for i, spec in enumerate(specs[::3]):
+        s_ref, acc_ref = _oracle_trajectory(coracle, spec, total, n_therm, spb, n_bins)
+        k = 3 * i
+        assert np.array_equal(out[0][1][k], s_ref), f"L={spec.L}"
+        assert np.array_equal(out[0][0][k], acc_ref), f"L={spec.L}"
