@@ -136,6 +136,8 @@ template <int T, int NTH>
 __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
   constexpr int ITERS = T * T / 2 / NTH;
   constexpr uint32_t NS = 4u * T;  // edge slots per tile
+  constexpr uint32_t QW = T * T / (NTH / 64);  // union queue entries per wave (>= 192)
+  static_assert(QW >= 192, "a wave queues up to 3 unions per lane");
   __shared__ uint32_t par[T * T];
   __shared__ uint32_t lcnt[T * T];
   __shared__ uint8_t bits[T * T];  // bit0 spin, bit1 occ_h, bit2 occ_v, bit3 flip draw, bit4/5 J<0
@@ -225,8 +227,6 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
     }
     par[li0] = start0;
     par[li1] = internal ? start0 : li1;
-    lcnt[li0] = 0u;
-    lcnt[li1] = 0u;
   }
   if (meas_em) wave_spins(me, mm, red);
   __syncthreads();
@@ -261,14 +261,12 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
       req = h0 | (h1 << 1);
     }
     req |= ((b1 >> 2) & 1u & ((lane == 63 && ly0 + 1u != T - 1u) ? 1u : 0u)) << 2;
-    while (__ballot(req != 0u)) {
-      if (req) {
-        const uint32_t k = (uint32_t)__builtin_ctz(req);
-        req &= req - 1u;
-        tunion(par, k == 0u ? li0 : li1, k == 0u ? li0 + T : k == 1u ? li1 + T : li1 + 1u);
-      }
-    }
+    // the wave's queue lives in lcnt, which is cleared before it is used
+    const uint32_t qa[3] = {li0, li1, li1};
+    const uint32_t qb[3] = {li0 + T, li1 + T, li1 + 1u};
+    wave_union_queue(par, lcnt + wave * QW, req, qa, qb, below);
   }
+  for (uint32_t i = lane; i < QW; i += 64) lcnt[wave * QW + i] = 0u;  // this wave's queue
   __syncthreads();
   // roots: counts (size << 16 | n_plus before the flip); every site is pointed
   // straight at its root (the forest is settled, so this is only compression)
@@ -298,8 +296,11 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
   }
   __syncthreads();
 
-  // new state: interior clusters flipped now, edge clusters labelled
-  TileMeas ms = {0, 0, 0, 0ull, 0ull, 0u, 0u};
+  // new state: interior clusters flipped now, edge clusters labelled.  Interior
+  // clusters are ranked by a 32-bit key size << 12 | (4095 - root index): inside
+  // one tile the local index orders like the site; the winners' n_plus is
+  // looked up afterwards.
+  uint32_t ik1 = 0u, ik2 = 0u, incl = 0u;
 #pragma unroll 1
   for (int it = 0; it < ITERS; ++it) {
     const uint32_t ql = (uint32_t)(it * NTH + tid);
@@ -321,11 +322,10 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
       sp2 |= (ns ? 0x01u : 0xFFu) << (8 * j);
       lb2 |= (edge ? (LAB_EDGE | rep) : 0u) << (16 * j);
       if (meas_cl && !edge && r == li) {  // an interior cluster, at its root
-        const uint32_t c = lcnt[r];
-        const uint32_t size = c >> 16, np = c & 0xFFFFu;
-        const uint32_t site = n0 + (uint32_t)j;
-        ++ms.ncl;
-        top2_insert(ms, ((uint64_t)size << 32) | (uint64_t)(0xFFFFFFFFu - site), fl ? size - np : np);
+        const uint32_t key = ((lcnt[r] >> 16) << 12) | (0xFFFu - li);
+        ++incl;
+        ik2 = max(ik2, min(ik1, key));
+        ik1 = max(ik1, key);
       }
     }
     *reinterpret_cast<uint16_t*>(v.sout + n0) = (uint16_t)sp2;
@@ -347,22 +347,43 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
     if (lane == 0) f.xbits[((size_t)lat * tiles + tile) * (2 * T / 64) + wave] = word;
   }
   if (meas_cl) {
-    wave_clusters(ms, red);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const uint32_t o1 = __shfl_xor(ik1, off), o2 = __shfl_xor(ik2, off);
+      incl += __shfl_xor(incl, off);
+      ik2 = max(min(ik1, o1), max(ik2, o2));
+      ik1 = max(ik1, o1);
+    }
+    if (lane == 0) {
+      red[wave][2] = incl;
+      red[wave][3] = ik1;
+      red[wave][4] = ik2;
+    }
     __syncthreads();
     if (tid == 0) {
-      TileMeas a = {0, 0, 0, 0ull, 0ull, 0u, 0u};
+      uint32_t k1 = 0u, k2 = 0u;
       int64_t ncl = 0;
       for (int w = 0; w < NTH / 64; ++w) {
         ncl += red[w][2];
-        top2_insert(a, (uint64_t)red[w][3], (uint32_t)red[w][5]);
-        top2_insert(a, (uint64_t)red[w][4], (uint32_t)red[w][6]);
+        const uint32_t o1 = (uint32_t)red[w][3], o2 = (uint32_t)red[w][4];
+        k2 = max(min(k1, o1), max(k2, o2));
+        k1 = max(k1, o1);
       }
       int64_t* o = f.part_int + (((size_t)(g & 1) * s.n_lat + lat) * tiles + tile) * 5;
       o[0] = ncl;
-      o[1] = (int64_t)a.k1;
-      o[2] = (int64_t)a.k2;
-      o[3] = a.p1;
-      o[4] = a.p2;
+      const uint32_t ks[2] = {k1, k2};
+      for (int q = 0; q < 2; ++q) {
+        int64_t key = 0, np_after = 0;
+        if (ks[q] != 0u) {
+          const uint32_t li = 0xFFFu - (ks[q] & 0xFFFu);
+          const uint32_t c = lcnt[li], size = c >> 16, np = c & 0xFFFFu;
+          const uint32_t site = (x0 + li / T) * L + y0 + li % T;
+          key = (int64_t)(((uint64_t)size << 32) | (uint64_t)(0xFFFFFFFFu - site));
+          np_after = ((bits[li] >> 3) & 1u) ? size - np : np;
+        }
+        o[1 + q] = key;
+        o[3 + q] = np_after;
+      }
     }
   }
 }

@@ -30,6 +30,7 @@
 //                  folds the sweep into its bin's moment sums
 //                  (simulation.py:212-236).
 #include "engine.h"
+#include "tile_common.h"
 
 namespace {
 
@@ -258,6 +259,10 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
   const uint64_t below = (1ull << lane) - 1ull;
   const uint32_t npairs = (N + 1) >> 1;
   const int iters = (int)((npairs + NT - 1) / NT);
+  // B1 unions through a per-wave queue in cnt when every wave's share of cnt
+  // holds the up to 3 x 64 requests of one pass
+  const uint32_t qw = N / NW;
+  const bool compact = qw >= 192u;
 
   int8_t* gsp = s.spins + (size_t)lat * N;
   const uint8_t* gj = s.jbits + (size_t)lat * N;
@@ -455,14 +460,23 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
         req |= h1 << 1;
         req |= ((b1 >> 4) & 1u & ((lane == 63 && p.y1 != L - 1u) ? 1u : 0u)) << 2;
       }
-      while (__ballot(req != 0u)) {
-        if (req) {
-          const uint32_t k = (uint32_t)__builtin_ctz(req);
-          req &= req - 1u;
-          lds_union_plain(uf, k == 0u ? p.n0 : p.n1, k == 0u ? p.nr0 : k == 1u ? p.nr1 : p.nu1);
+      if (compact) {
+        // the wave's requests go through a dense queue (in cnt, cleared below)
+        const uint32_t qa[3] = {p.n0, p.n1, p.n1};
+        const uint32_t qb[3] = {p.nr0, p.nr1, p.nu1};
+        wave_union_queue(uf, cnt + wave * qw, req, qa, qb, below);
+      } else {
+        while (__ballot(req != 0u)) {
+          if (req) {
+            const uint32_t k = (uint32_t)__builtin_ctz(req);
+            req &= req - 1u;
+            lds_union_plain(uf, k == 0u ? p.n0 : p.n1, k == 0u ? p.nr0 : k == 1u ? p.nr1 : p.nu1);
+          }
         }
       }
     }
+    if (compact)
+      for (uint32_t i = (uint32_t)lane; i < qw; i += 64u) cnt[wave * qw + i] = 0u;
     __syncthreads();
     // ----------------------------------------------------------- B2
     // The at most 2L bonds across the periodic boundary, with windings: x

@@ -99,6 +99,14 @@ __device__ __forceinline__ void tfind2(uint32_t* par, uint32_t& a, uint32_t& b) 
   }
 }
 
+// Unions of one wavefront, compacted: every lane holds up to three requests
+// (bit k of req: the pair (a_k, b_k)); they are written densely into the
+// wave's queue q (>= 192 entries, packed a | b << 16) and processed 64 at a
+// time, so a round has no idle lanes waiting on a neighbour's second request.
+__device__ __forceinline__ void wave_union_queue(uint32_t* par, uint32_t* q, uint32_t req,
+                                                 const uint32_t (&a)[3], const uint32_t (&b)[3],
+                                                 uint64_t below);
+
 __device__ __forceinline__ void tunion(uint32_t* par, uint32_t a, uint32_t b) {
   while (true) {
     tfind2(par, a, b);
@@ -119,6 +127,29 @@ __device__ __forceinline__ uint32_t troot(const uint32_t* par, uint32_t x) {
     p = par[x];
   }
   return x;
+}
+
+__device__ __forceinline__ void wave_union_queue(uint32_t* par, uint32_t* q, uint32_t req,
+                                                 const uint32_t (&a)[3], const uint32_t (&b)[3],
+                                                 uint64_t below) {
+  const uint32_t c = (uint32_t)__popc(req);
+  const uint64_t c0 = __ballot(c & 1u), c1 = __ballot(c & 2u);
+  uint32_t pos = (uint32_t)(__popcll(c0 & below) + 2 * __popcll(c1 & below));
+  const uint32_t total = (uint32_t)(__popcll(c0) + 2 * __popcll(c1));
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    if ((req >> k) & 1u) q[pos++] = a[k] | (b[k] << 16);
+  // the queue is read by other lanes of this wavefront only
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (uint32_t i = threadIdx.x & 63u; i < total; i += 64u) {
+    const uint32_t e = q[i];
+    tunion(par, e & 0xFFFFu, e >> 16);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 }  // namespace
