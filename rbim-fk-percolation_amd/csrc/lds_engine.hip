@@ -186,15 +186,33 @@ __device__ __forceinline__ void lds_union_plain(uint32_t* uf, uint32_t a, uint32
   }
 }
 
-// Cluster key: size (bits 32..), ~root (16..29), n_plus (0..14).  The max key is
-// the largest cluster, ties to the smaller root; n_plus rides along.
-__device__ __forceinline__ uint64_t cluster_key(uint32_t c, uint32_t root) {
-  return ((uint64_t)(c >> 16) << 32) | ((uint64_t)(PMASK - root) << 16) | (uint64_t)(c & 0xFFFFu);
-}
+// Cluster keys from the counter c = size << 16 | n_plus of a root.  The max key
+// is the largest cluster, ties to the smaller root; n_plus rides along.
+//   wide   (any L):   size (bits 32..), ~root (16..29), n_plus (0..14)
+//   narrow (N <= 1024): size (21..31), 1023 - root (11..20), n_plus (0..10)
+template <bool NARROW>
+struct ClusterKey {
+  using T = uint64_t;
+  static __device__ __forceinline__ T make(uint32_t c, uint32_t root) {
+    return ((uint64_t)(c >> 16) << 32) | ((uint64_t)(PMASK - root) << 16) | (uint64_t)(c & 0xFFFFu);
+  }
+  static __device__ __forceinline__ int64_t size(T k) { return (int64_t)(k >> 32); }
+  static __device__ __forceinline__ int64_t n_plus(T k) { return (int64_t)(k & 0xFFFFu); }
+};
+template <>
+struct ClusterKey<true> {
+  using T = uint32_t;
+  static __device__ __forceinline__ T make(uint32_t c, uint32_t root) {
+    return ((c >> 16) << 21) | ((1023u - root) << 11) | (c & 0x7FFu);
+  }
+  static __device__ __forceinline__ int64_t size(T k) { return (int64_t)(k >> 21); }
+  static __device__ __forceinline__ int64_t n_plus(T k) { return (int64_t)(k & 0x7FFu); }
+};
 
-__device__ __forceinline__ void top2_merge(uint64_t& k1, uint64_t& k2, uint64_t o1, uint64_t o2) {
-  const uint64_t hi = max(k1, o1);
-  const uint64_t lo = max(min(k1, o1), max(k2, o2));
+template <typename K>
+__device__ __forceinline__ void top2_merge(K& k1, K& k2, K o1, K o2) {
+  const K hi = max(k1, o1);
+  const K lo = max(min(k1, o1), max(k2, o2));
   k1 = hi;
   k2 = lo;
 }
@@ -239,9 +257,11 @@ template <int NT, int LC>
 __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t sweeps) {
   constexpr int NW = NT / 64;
   constexpr bool ALL = LC > 0 && (LC % 2) == 0 && ((LC * LC / 2) % NT) == 0;
+  using Key = ClusterKey<(LC > 0 && LC <= 32)>;
+  using KT = typename Key::T;
   extern __shared__ uint32_t smem[];
   __shared__ uint32_t red_c[NW][3];  // satisfied bonds, up spins, roots
-  __shared__ uint64_t red_k[NW][2];  // two largest cluster keys
+  __shared__ KT red_k[NW][2];        // two largest cluster keys
   __shared__ uint32_t wrapflag[2];   // by sweep parity
   __shared__ uint64_t a[RBIM_ACC_SLOTS];  // running sums of the open bin (lane 0)
 
@@ -286,19 +306,19 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
 
   // Measurement bookkeeping for one pass (counts are wave-uniform).
   uint32_t m_sat = 0u, m_up = 0u, m_root = 0u;
-  uint64_t k1 = 0ull, k2 = 0ull;
+  KT k1 = 0u, k2 = 0u;
   auto measure_site = [&](bool valid, uint32_t n, uint32_t b, uint32_t sat_h, uint32_t sat_v) {
     m_sat += popc_ballot(valid && sat_h) + popc_ballot(valid && sat_v);
     m_up += popc_ballot(valid && (b & 1u));
     const bool root = valid && (uf[n] & PMASK) == n;
     m_root += popc_ballot(root);
-    if (root) top2_merge(k1, k2, cluster_key(cnt[n], n), 0ull);
+    if (root) top2_merge(k1, k2, Key::make(cnt[n], n), (KT)0u);
   };
   auto publish = [&]() {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
-      const uint64_t o1 = __shfl_xor(k1, off);
-      const uint64_t o2 = __shfl_xor(k2, off);
+      const KT o1 = __shfl_xor(k1, off);
+      const KT o2 = __shfl_xor(k2, off);
       top2_merge(k1, k2, o1, o2);
     }
     if (lane == 0) {
@@ -309,12 +329,12 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
       red_k[wave][1] = k2;
     }
     m_sat = m_up = m_root = 0u;
-    k1 = k2 = 0ull;
+    k1 = k2 = 0u;
   };
   // Lane 0 of wave 0: fold the published partials of sweep gm into its bin.
   auto finalize = [&](int64_t gm) {
     uint32_t sat = 0u, up = 0u, roots = 0u;
-    uint64_t f1 = 0ull, f2 = 0ull;
+    KT f1 = 0u, f2 = 0u;
     for (int w = 0; w < NW; ++w) {
       sat += red_c[w][0];
       up += red_c[w][1];
@@ -325,11 +345,11 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
     st.energy = 2 * (int64_t)N - 2 * (int64_t)sat;
     st.magnetization = 2 * (int64_t)up - (int64_t)N;
     st.n_clusters = roots;
-    const int64_t s1 = (int64_t)(f1 >> 32), mc1 = 2 * (int64_t)(f1 & 0xFFFFu) - s1;
+    const int64_t s1 = Key::size(f1), mc1 = 2 * Key::n_plus(f1) - s1;
     st.largest_size = s1;
     st.largest_abs_mag = mc1 < 0 ? -mc1 : mc1;
-    if (f2 != 0ull) {
-      const int64_t s2 = (int64_t)(f2 >> 32), mc2 = 2 * (int64_t)(f2 & 0xFFFFu) - s2;
+    if (f2 != 0u) {
+      const int64_t s2 = Key::size(f2), mc2 = 2 * Key::n_plus(f2) - s2;
       st.second_size = s2;
       st.second_abs_mag = mc2 < 0 ? -mc2 : mc2;
     } else {
