@@ -107,7 +107,7 @@ def profiled_traffic(L, n_lat, sweeps_per_launch, threads):
     if (L, n_lat, sweeps_per_launch, threads) == (32, 1280, 2000, 256):
         name, key = "lds_sweep_kernel_L32_batched.md", "traffic_bytes_per_launch:"
     elif (L, n_lat) == (256, 1280):
-        name, key = "tiled_kernels_L256_batched.md", "traffic_bytes_per_sweep:"
+        name, key = "edge_slot_sweep_L256_batched.md", "traffic_bytes_per_sweep:"
     else:
         return None
     path = os.path.join(ROOT, "profiles", name)
@@ -156,6 +156,15 @@ def cpu_baseline(specs, site_updates_per_thread, per_thread):
     }
 
 
+def fused_tile_size(L):
+    """Tile side of the two-kernel edge-slot sweep for L (0: L uses the
+    multi-kernel tiled sweep); mirrors fused_tile_size() of the library."""
+    for T in (64, 32):
+        if L % T == 0 and (L // T) ** 2 * 4 * T <= 6400:
+            return T
+    return 0
+
+
 def measure_point(L, args, steps, warmup, plain_steps, with_cpu):
     """Batched scan at one L (p, 40 T x 32 R = 1280 lattices by default), the
     plain one-lattice path and the CPU port beside it."""
@@ -177,7 +186,8 @@ def measure_point(L, args, steps, warmup, plain_steps, with_cpu):
     else:
         avg_launch_ms = res["ms"] / steps
         units = sites
-        kernel = "tiled sweep: k1_tile_full + k2_merge + k3_flip + k5_finalize"
+        kernel = ("edge-slot sweep: ka_tile + kb_merge" if fused_tile_size(L) and args.engine != 3
+                  else "tiled sweep: k1_tile_full + k2_merge + k3_flip + k5_finalize")
     achieved = ALGO_BYTES_PER_SITE_UPDATE * units / (avg_launch_ms / 1e3) / 1e9
     plain = gpu_plain(specs[len(specs) // 2], plain_steps)
     point = {
@@ -223,7 +233,8 @@ def main():
     ap.add_argument("--p", type=float, default=0.05)
     ap.add_argument("--temps", type=int, default=40)
     ap.add_argument("--realizations", type=int, default=32)
-    ap.add_argument("--engine", type=int, default=0, help="0 auto, 1 plain (tiled), 2 batched (LDS)")
+    ap.add_argument("--engine", type=int, default=0,
+                    help="0 auto, 1 plain (tiled), 2 batched (LDS), 3 multi-kernel tiled")
     ap.add_argument("--sweeps-per-launch", type=int, default=0)
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--streams", type=int, default=0, help="streams per tiled set (0: default)")
@@ -276,7 +287,7 @@ def main():
             "workload": main_pt["workload"],
             "L": args.L,
             "lattices": main_pt["lattices"],
-            "engine": {0: "auto", 1: "plain", 2: "batched"}[args.engine],
+            "engine": {0: "auto", 1: "plain", 2: "batched", 3: "tiled"}[args.engine],
             "threads_per_lattice": main_pt["threads_per_lattice"],
             "sweeps_per_launch": main_pt["sweeps_per_launch"],
         },
