@@ -99,3 +99,20 @@ def test_run_sweep_cli(tmp_path):
     assert res["metadata"]["n_realizations"] == 3 and len(res["temperatures"]) == 4
     assert set(res["observables"]) == set(OBSERVABLES)
     assert np.all(np.isfinite(res["observables"]["energy"]["value"]))
+
+
+def test_critical_temperature_from_crossings_at_scale():
+    """p = 0, L = 32 ... 256: the Binder-cumulant and FK wrap_y crossings (all
+    size pairs, mean) land within 0.04 % of T_c = 2/ln(1+sqrt 2), and agree
+    with each other (Coniglio-Klein).  The engine is deterministic for a seed,
+    so this run is the same on every GPU."""
+    sys.path.insert(0, os.path.join(ROOT, "rbim-fk-percolation_amd", "scripts"))
+    import validate_pure_ising as v
+
+    binder_tc, wrap_tc = v.main(["--sizes", "32", "64", "128", "256", "--t-min", "2.24",
+                                 "--t-max", "2.30", "--n-temps", "13", "--sweeps", "20000",
+                                 "--replicas", "8"])
+    assert binder_tc is not None and wrap_tc is not None
+    assert abs(binder_tc - v.TC_EXACT) / v.TC_EXACT < 4e-4, binder_tc
+    assert abs(wrap_tc - v.TC_EXACT) / v.TC_EXACT < 4e-4, wrap_tc
+    assert abs(binder_tc - wrap_tc) < 1e-3

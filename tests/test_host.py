@@ -269,3 +269,23 @@ def test_integrated_autocorrelation_ar1_closed_form():
         assert window >= 6 * tau - 1
     assert integrated_autocorrelation(np.ones(100)) == (0.5, 0)
     assert integrated_autocorrelation(np.arange(10.0)) == (0.5, 0)
+
+
+def test_crossing_interpolation():
+    """Linear crossing of two lines and of sampled curves with a known
+    intersection; no crossing -> None; a grid point where they meet."""
+    import os
+    import sys
+
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                    "rbim-fk-percolation_amd", "scripts"))
+    import validate_pure_ising as v
+
+    t = np.linspace(2.0, 2.5, 11)
+    assert abs(v.crossing(t, 2.0 * (t - 2.2691), -0.5 * (t - 2.2691)) - 2.2691) < 1e-12
+    assert v.crossing(t, t, t + 1.0) is None
+    assert v.crossing(t, np.where(t == t[3], 0.0, t - t[3]), np.zeros_like(t)) == t[3]
+    curves = {8: 1.0 - (t - 2.27), 16: 1.0 - 2.0 * (t - 2.27), 32: 1.0 - 4.0 * (t - 2.27)}
+    found = v.pairwise_crossings(t, curves)
+    assert [(a, b) for a, b, _ in found] == [(8, 16), (8, 32), (16, 32)]
+    assert all(abs(x - 2.27) < 1e-12 for _, _, x in found)
