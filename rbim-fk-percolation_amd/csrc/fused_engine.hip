@@ -276,11 +276,20 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
     const uint32_t b0 = bits[li0], b1 = bits[li1];
     const uint32_t r0 = troot(par, li0);
     const uint32_t r1 = (b0 & 4u) ? r0 : troot(par, li1);
-    if (r0 == r1) {
+    // the first lane's cluster (often most of the wave) is summed by ballots
+    // and added by one atomic
+    const uint32_t R = __builtin_amdgcn_readfirstlane(r0);
+    const bool in0 = r0 == R, in1 = r1 == R;
+    const uint64_t m0 = __ballot(in0), m1 = __ballot(in1);
+    const uint64_t u0 = __ballot(in0 && (b0 & 1u)), u1 = __ballot(in1 && (b1 & 1u));
+    if (lane == 0)
+      atomicAdd(&lcnt[R], (uint32_t)(__popcll(m0) + __popcll(m1)) << 16 |
+                              (uint32_t)(__popcll(u0) + __popcll(u1)));
+    if (!in0 && !in1 && r0 == r1) {
       atomicAdd(&lcnt[r0], 0x20000u | ((b0 & 1u) + (b1 & 1u)));
     } else {
-      atomicAdd(&lcnt[r0], 0x10000u | (b0 & 1u));
-      atomicAdd(&lcnt[r1], 0x10000u | (b1 & 1u));
+      if (!in0) atomicAdd(&lcnt[r0], 0x10000u | (b0 & 1u));
+      if (!in1) atomicAdd(&lcnt[r1], 0x10000u | (b1 & 1u));
     }
     if (r0 != li0) par[li0] = r0;
     if (r1 != li1) par[li1] = r1;

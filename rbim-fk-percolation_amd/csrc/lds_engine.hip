@@ -514,32 +514,43 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
     if (wraps) atomicOr(&wrapflag[g & 1], wraps);
     __syncthreads();
     // ------------------------------------------------------------ C
+    // Counts go to cnt[root] by LDS atomics; the sites of the first lane's
+    // cluster (at low temperature usually most of the wave, one giant cluster)
+    // are summed by ballots and added by a single atomic.
 #pragma unroll 1
     for (int it = 0; it < iters; ++it) {
       const uint32_t q = (uint32_t)(it * NT + tid);
       const Pair p = make_pair<LC, ALL>(q, N, L, magic);
-      if (!p.v0) continue;
-      const uint32_t r0 = lds_root(uf, p.n0);
-      const uint32_t b0 = sb[p.n0];
-      const uint32_t ns0 = (b0 ^ (sb[r0] >> 5)) & 1u;
-      sb[p.n0] = (uint8_t)((b0 & ~1u) | ns0);
-      if (s.canon_out) s.canon_out[p.n0] = (int32_t)r0;
-      if (p.v1) {
-        // n1 shares n0's run (and root) when the in-row bond n0 -> n1 is occupied
-        const bool linked = ((b0 >> 4) & 1u) && p.y0 != L - 1u;
-        const uint32_t r1 = linked ? r0 : lds_root(uf, p.n1);
-        const uint32_t b1 = sb[p.n1];
-        const uint32_t ns1 = (b1 ^ (sb[r1] >> 5)) & 1u;
-        sb[p.n1] = (uint8_t)((b1 & ~1u) | ns1);
-        if (s.canon_out) s.canon_out[p.n1] = (int32_t)r1;
-        if (r1 == r0) {
-          atomicAdd(&cnt[r0], 0x20000u | (ns0 + ns1));
-        } else {
-          atomicAdd(&cnt[r0], 0x10000u | ns0);
-          atomicAdd(&cnt[r1], 0x10000u | ns1);
+      uint32_t r0 = 0u, r1 = 0u, ns0 = 0u, ns1 = 0u;
+      if (p.v0) {
+        r0 = lds_root(uf, p.n0);
+        const uint32_t b0 = sb[p.n0];
+        ns0 = (b0 ^ (sb[r0] >> 5)) & 1u;
+        sb[p.n0] = (uint8_t)((b0 & ~1u) | ns0);
+        if (s.canon_out) s.canon_out[p.n0] = (int32_t)r0;
+        if (p.v1) {
+          // n1 shares n0's run (and root) when the in-row bond n0 -> n1 is occupied
+          const bool linked = ((b0 >> 4) & 1u) && p.y0 != L - 1u;
+          r1 = linked ? r0 : lds_root(uf, p.n1);
+          const uint32_t b1 = sb[p.n1];
+          ns1 = (b1 ^ (sb[r1] >> 5)) & 1u;
+          sb[p.n1] = (uint8_t)((b1 & ~1u) | ns1);
+          if (s.canon_out) s.canon_out[p.n1] = (int32_t)r1;
         }
+      }
+      const uint32_t R = __builtin_amdgcn_readfirstlane(r0);
+      const bool in0 = p.v0 && r0 == R, in1 = p.v1 && r1 == R;
+      const uint64_t m0 = __ballot(in0), m1 = __ballot(in1);
+      const uint64_t u0 = __ballot(in0 && ns0), u1 = __ballot(in1 && ns1);
+      if (__builtin_amdgcn_readfirstlane((uint32_t)lane) == (uint32_t)lane && (m0 | m1))
+        atomicAdd(&cnt[R], (uint32_t)(__popcll(m0) + __popcll(m1)) << 16 |
+                               (uint32_t)(__popcll(u0) + __popcll(u1)));
+      const bool rest0 = p.v0 && !in0, rest1 = p.v1 && !in1;
+      if (rest0 && rest1 && r0 == r1) {
+        atomicAdd(&cnt[r0], 0x20000u | (ns0 + ns1));
       } else {
-        atomicAdd(&cnt[r0], 0x10000u | ns0);
+        if (rest0) atomicAdd(&cnt[r0], 0x10000u | ns0);
+        if (rest1) atomicAdd(&cnt[r1], 0x10000u | ns1);
       }
     }
     __syncthreads();
