@@ -224,6 +224,35 @@ def measure_point(L, args, steps, warmup, plain_steps, with_cpu):
     return point
 
 
+def workload_specs(name, seed=0):
+    """Lattices of the ragged and large-lattice scans (one Batch each):
+    ragged  L in {16, 32, 64, 128} x p in {0, 0.05, 0.10, 0.12} x 16 realizations x
+            24 T in linspace(1.4, 2.8)  -> 6144 lattices, one stream per L;
+    large   L in {512, 1024}, p = 0.10, 24 T in linspace(1.6, 2.6) x 8 realizations."""
+    from rbim.engine import LatticeSpec
+
+    specs = []
+    if name == "ragged":
+        for L in (16, 32, 64, 128):
+            for i, p in enumerate((0.0, 0.05, 0.10, 0.12)):
+                specs += scan_specs(L, p, np.linspace(1.4, 2.8, 24), 16, seed=seed + 10 * L + i)
+    else:
+        for L in (512, 1024):
+            specs += scan_specs(L, 0.10, np.linspace(1.6, 2.6, 24), 8, seed=seed + L)
+    # distinct lattice ids across the sub-scans
+    return [LatticeSpec(L=s.L, J_h=s.J_h, J_v=s.J_v, beta=s.beta, seed=s.seed, lattice_id=k)
+            for k, s in enumerate(specs)]
+
+
+def measure_workload(name, steps):
+    specs = workload_specs(name)
+    res = gpu_batched(specs, steps, 0, 0)
+    value = res["site_updates"] / (res["ms"] / 1e3)
+    return {"workload": name, "lattices": len(specs), "steps": steps, "value": value,
+            "unit": "site-updates/s", "ms_per_step": res["ms"] / steps,
+            "Ls": sorted({s.L for s in specs})}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -248,6 +277,8 @@ def main():
     ap.add_argument("--second-warmup", type=int, default=20)
     ap.add_argument("--second-plain-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["ragged", "large"], default=None,
+                    help="only time the ragged (L 16..128 mix) or large (L 512, 1024) scan")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (rocprof counter "
                          "calibration) and exit")
@@ -262,6 +293,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_per_dispatch": 4 * n, "dispatches": 3}))
+        return
+    if args.workload:
+        print(json.dumps(measure_workload(args.workload, args.steps)), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box in this build has one MI355X "

@@ -216,15 +216,8 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
     bits[li1] = (uint8_t)((b1 & 1u) | (oh1 << 1) | (ov1 << 2) | ((r.z & 1u) << 3));
     const bool internal = ov0 != 0u;                       // n0 -> n1, same row and tile
     const bool outlink = ov1 != 0u && ly0 + 1u != T - 1u;  // n1 -> n1+1 inside the tile
-    const uint64_t thru = __ballot(internal && outlink);
-    const uint64_t ol = __ballot(outlink);
-    const uint64_t stop = ~thru & below;
     const uint32_t chunk = 2u * (uint32_t)(it * NTH + wave * 64);
-    uint32_t start0 = chunk;
-    if (stop != 0ull) {
-      const uint32_t j = 63u - (uint32_t)__clzll(stop);
-      start0 = chunk + 2u * j + (((ol >> j) & 1ull) ? 1u : 2u);
-    }
+    const uint32_t start0 = run_start(__ballot(internal && outlink), __ballot(outlink), below, chunk);
     par[li0] = start0;
     par[li1] = internal ? start0 : li1;
   }

@@ -423,16 +423,7 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
       // run pre-labelling of in-row bonds inside this wave's 128-site chunk
       const bool internal = p.v1 && ov0 && p.y0 != L - 1u;
       const bool outlink = p.v1 && ov1 && p.y1 != L - 1u;
-      const uint64_t thru = __ballot(internal && outlink);
-      const uint64_t ol = __ballot(outlink);
-      const uint64_t stop = ~thru & below;
-      uint32_t start0;
-      if (stop == 0ull) {
-        start0 = chunk;
-      } else {
-        const uint32_t j = 63u - (uint32_t)__clzll(stop);
-        start0 = chunk + 2u * j + (((ol >> j) & 1ull) ? 1u : 2u);
-      }
+      const uint32_t start0 = run_start(__ballot(internal && outlink), __ballot(outlink), below, chunk);
       if (p.v0) {
         sb[p.n0] = (uint8_t)((b0 & 7u) | (oh0 << 3) | (ov0 << 4) | (fl0 << 5));
         uf[p.n0] = start0;

@@ -69,6 +69,18 @@ __device__ void wave_spins(int32_t e, int32_t m, int64_t (*red)[PARTS]) {
   }
 }
 
+// Start of the in-row run holding a lane's even site.  Lane j's pair is
+// (chunk + 2j, chunk + 2j + 1); thru: lanes whose pair is linked inside and to
+// the next pair, ol: lanes whose odd site links to the next pair.  The run
+// starts after the nearest lane below that does not pass the run through.
+__device__ __forceinline__ uint32_t run_start(uint64_t thru, uint64_t ol, uint64_t below,
+                                              uint32_t chunk) {
+  const uint64_t stop = ~thru & below;
+  if (stop == 0ull) return chunk;
+  const uint32_t j = 63u - (uint32_t)__clzll(stop);
+  return chunk + 2u * j + (((ol >> j) & 1ull) ? 1u : 2u);
+}
+
 // Tile-local union-find in LDS over tile indices (no windings: a tile never
 // crosses the periodic boundary).  Two finds walked together with path
 // halving; the larger root hooks under the smaller, so every root is the
