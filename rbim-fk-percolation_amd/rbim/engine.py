@@ -253,7 +253,8 @@ class Batch:
             pass
 
 
-def footprint(lattices: Sequence[LatticeSpec], *, n_bins: int, engine: int = _hip.ENGINE_AUTO) -> int:
+def footprint(lattices: Sequence[LatticeSpec], *, n_bins: int, engine: int = _hip.ENGINE_AUTO,
+              series: int = 0) -> int:
     """Device bytes a batch of these lattices would hold (admission control)."""
     Ls = np.array([s.L for s in lattices], dtype=np.int32)
     n = int(Ls.size)
@@ -273,6 +274,7 @@ def footprint(lattices: Sequence[LatticeSpec], *, n_bins: int, engine: int = _hi
     desc.n_bins = int(n_bins)
     desc.sweeps_per_bin = 1
     desc.engine = int(engine)
+    desc.series = int(series)
     out = ctypes.c_uint64()
     _hip.check(_hip.lib().rbim_batch_footprint(ctypes.byref(desc), ctypes.byref(out)))
     return int(out.value)

@@ -71,3 +71,28 @@ def test_fails_loudly_without_a_gpu():
     lattice = rbim.build_lattice(4)
     with pytest.raises(_hip.EngineUnavailable):
         rbim.label_clusters(np.zeros(lattice.n_bonds, bool), lattice)
+
+
+def test_footprint_by_layout():
+    """rbim_batch_footprint is host arithmetic (no GPU): the edge-slot layout
+    holds less than the multi-kernel tiled layout, the LDS layout least, and a
+    series ring adds n_lattices x capacity x sizeof(rbim_sweep_stats)."""
+    import numpy as np
+
+    from rbim import _hip
+    from rbim.engine import LatticeSpec, footprint
+
+    def specs(L, n):
+        J = np.ones((L, L), np.int8)
+        return [LatticeSpec(L=L, J_h=J, J_v=J, beta=0.4, seed=1, lattice_id=i) for i in range(n)]
+
+    big = specs(256, 8)
+    fused = footprint(big, n_bins=10, engine=_hip.ENGINE_PLAIN)
+    tiled = footprint(big, n_bins=10, engine=_hip.ENGINE_TILED)
+    assert 0 < fused < tiled
+    small = specs(32, 8)
+    assert footprint(small, n_bins=10) < footprint(small, n_bins=10, engine=_hip.ENGINE_PLAIN)
+    ring = footprint(small, n_bins=10, series=100) - footprint(small, n_bins=10)
+    assert ring == 8 * 100 * ctypes.sizeof(_hip.SweepStats)
+    with pytest.raises(ValueError):
+        footprint(small, n_bins=10, series=-1)

@@ -203,3 +203,39 @@ def test_binder_cumulant_brackets_its_limits():
     disordered = run_temperature_scan(12, 0.0, [5.0], **kw)
     assert ordered["observables"]["binder"]["value"][0] == pytest.approx(2.0 / 3.0, abs=0.03)
     assert abs(disordered["observables"]["binder"]["value"][0]) < 0.1
+
+
+@pytest.mark.parametrize("engine", [0, 1, 3])
+def test_allocation_stays_within_footprint(engine):
+    """The admission figure covers what a batch really holds, for a ragged mix
+    across all layouts (LDS, edge-slot, multi-kernel tiled), with a series ring."""
+    from rbim.engine import Batch, LatticeSpec, footprint
+
+    rng = np.random.default_rng(4)
+    specs = []
+    for k, L in enumerate((16, 32, 100, 128, 160, 256, 320, 336)):
+        J_h = np.where(rng.random((L, L)) < 0.1, -1, 1).astype(np.int8)
+        specs.append(LatticeSpec(L=L, J_h=J_h, J_v=J_h.T.copy(), beta=0.45, seed=3, lattice_id=k))
+    need = footprint(specs, n_bins=4, engine=engine, series=16)
+    with Batch(specs, n_therm=2, n_bins=4, sweeps_per_bin=3, engine=engine, series=16) as batch:
+        batch.run()
+        info = batch.info()
+        assert batch.series().shape == (len(specs), 12)
+    assert 0 < info.device_bytes <= need
+
+
+def test_descriptor_validation():
+    from rbim.engine import Batch, LatticeSpec
+
+    J = np.ones((32, 32), np.int8)
+    spec = [LatticeSpec(L=32, J_h=J, J_v=J, beta=0.4, seed=1, lattice_id=0)]
+    with pytest.raises(ValueError):
+        Batch(spec, n_therm=1, n_bins=1, sweeps_per_bin=1, streams=17)
+    with pytest.raises(ValueError):
+        Batch(spec, n_therm=1, n_bins=1, sweeps_per_bin=1, series=-1)
+    with pytest.raises(ValueError):
+        Batch(spec, n_therm=1, n_bins=1, sweeps_per_bin=1, engine=4)
+    with Batch(spec, n_therm=1, n_bins=1, sweeps_per_bin=1) as batch:
+        batch.run()
+        with pytest.raises(RuntimeError):
+            batch.series()
