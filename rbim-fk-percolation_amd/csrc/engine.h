@@ -1,5 +1,6 @@
 // Internal declarations shared by the gfx950 kernels and the C ABI.
 #pragma once
+#include <atomic>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -97,6 +98,19 @@ __device__ __forceinline__ void acc_add_u128(uint64_t* lo, uint64_t* hi, uint64_
   const uint64_t nlo = *lo + vlo;
   *hi += vhi + (nlo < vlo ? 1ull : 0ull);
   *lo = nlo;
+}
+
+// Raises a kernel's dynamic-LDS limit once per device (the attribute is per
+// device; `done` is the kernel's own bitmask of devices already configured).
+inline hipError_t ensure_dynamic_lds(const void* kernel, int bytes, std::atomic<uint64_t>& done) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  const uint64_t bit = 1ull << (dev & 63);
+  if (done.load(std::memory_order_acquire) & bit) return hipSuccess;
+  e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e == hipSuccess) done.fetch_or(bit, std::memory_order_release);
+  return e;
 }
 
 // Measured sweep gm of lattice lat into its series ring (when kept).

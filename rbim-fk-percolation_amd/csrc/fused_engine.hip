@@ -699,14 +699,10 @@ size_t kb_lds_bytes(int nslots) { return (size_t)nslots * (8 + 8 + 4); }
 template <int T>
 hipError_t launch_fused_t(const LatticeSet& s, const FusedScratch& f, hipStream_t st) {
   constexpr int NTH = T == 64 ? 512 : 256;
-  static bool configured = false;
-  if (!configured) {
-    hipError_t e = hipFuncSetAttribute((const void*)kb_merge<T>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)kb_lds_bytes(RBIM_FUSED_MAX_SLOTS));
-    if (e != hipSuccess) return e;
-    configured = true;
-  }
+  static std::atomic<uint64_t> configured{0};
+  hipError_t e = ensure_dynamic_lds((const void*)kb_merge<T>,
+                                    (int)kb_lds_bytes(RBIM_FUSED_MAX_SLOTS), configured);
+  if (e != hipSuccess) return e;
   const int tiles = f.tiles_x * f.tiles_x;
   hipLaunchKernelGGL((ka_tile<T, NTH>), dim3(tiles, s.n_lat), dim3(NTH), 0, st, s, f);
   hipLaunchKernelGGL((kb_merge<T>), dim3(s.n_lat), dim3(KB_THREADS), kb_lds_bytes(f.nslots), st, s, f);

@@ -584,14 +584,10 @@ __global__ void init_spins_kernel(LatticeSet s, int start) {
 
 template <int NT, int LC>
 hipError_t launch_lds_t(const LatticeSet& s, int sweeps, hipStream_t st) {
-  static bool configured = false;
-  if (!configured) {
-    hipError_t e = hipFuncSetAttribute((const void*)lds_sweep_kernel<NT, LC>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)lds_bytes(RBIM_LDS_MAX_N));
-    if (e != hipSuccess) return e;
-    configured = true;
-  }
+  static std::atomic<uint64_t> configured{0};
+  hipError_t e = ensure_dynamic_lds((const void*)lds_sweep_kernel<NT, LC>,
+                                    (int)lds_bytes(RBIM_LDS_MAX_N), configured);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL((lds_sweep_kernel<NT, LC>), dim3(s.n_lat), dim3(NT), lds_bytes(s.N), st, s,
                      sweeps);
   return hipGetLastError();
