@@ -97,8 +97,12 @@ size_t group_bytes(int L, int n_lat, bool lds, bool fused, int n_bins, int serie
   if (fused) {
     const int T = fused_tile_size(L);
     const size_t tiles = (size_t)(L / T) * (L / T), ns = tiles * 4 * T;
-    b += N * n_lat * (1 + 4) + (size_t)n_lat * (ns * 8 + tiles * (2 * T / 64) * 8 + ns / 8 +
+    b += N * n_lat * (1 + 4) + (size_t)n_lat * (tiles * (2 * T / 64) * 8 + ns / 8 +
                                                  tiles * (2 * 5 + 2) * 8 + 2 * 6 * 8 + 8 + 4);
+    if (fused_merge_in_hbm(L))
+      b += (size_t)n_lat * (ns * (8 + 8 + 4) + (ns + 255) / 256 * 5 * 8 + 4);
+    else
+      b += (size_t)n_lat * ns * 8;
   } else if (!lds) {
     const size_t tx = (L + 31) / 32;
     b += N * n_lat * (1 + 2 + 8 + 4 + 8) + (size_t)n_lat * tx * tx * 7 * 8 + (size_t)n_lat * 12;
@@ -176,7 +180,16 @@ int group_alloc(rbim_ctx* ctx, Group& g, const std::vector<uint8_t>& jbits,
     };
     HIP_TRY(ctx, zalloc(&p, NN)); f.spins_alt = (int8_t*)p;
     HIP_TRY(ctx, zalloc(&p, 2 * NN * 2)); f.lab = (uint16_t*)p;
-    HIP_TRY(ctx, zalloc(&p, nl * f.nslots * 8)); f.slots = (uint64_t*)p;
+    f.big = fused_merge_in_hbm(g.L) ? 1 : 0;
+    if (f.big) {
+      HIP_TRY(ctx, zalloc(&p, nl * f.nslots * 8)); f.guf = (uint64_t*)p;
+      HIP_TRY(ctx, zalloc(&p, nl * f.nslots * 8)); f.gcnt = (uint64_t*)p;
+      HIP_TRY(ctx, zalloc(&p, nl * f.nslots * 4)); f.gmn = (uint32_t*)p;
+      HIP_TRY(ctx, zalloc(&p, nl * ((f.nslots + 255) / 256) * 5 * 8)); f.gpart = (int64_t*)p;
+      HIP_TRY(ctx, zalloc(&p, nl * 4)); f.wrapw = (uint32_t*)p;
+    } else {
+      HIP_TRY(ctx, zalloc(&p, nl * f.nslots * 8)); f.slots = (uint64_t*)p;
+    }
     HIP_TRY(ctx, zalloc(&p, nl * tiles * (2 * f.tile / 64) * 8)); f.xbits = (uint64_t*)p;
     HIP_TRY(ctx, zalloc(&p, nl * f.nslots / 8)); f.flipbits = (uint32_t*)p;
     HIP_TRY(ctx, zalloc(&p, 2 * nl * tiles * 5 * 8)); f.part_int = (int64_t*)p;
@@ -230,7 +243,7 @@ int group_run(rbim_ctx* ctx, Group& g, int64_t n, hipStream_t stream, int64_t* l
   auto sweep = [&g](hipStream_t st) {
     return g.fused ? launch_fused_sweep(g.set, g.fz, st) : launch_tiled_sweep(g.set, g.tile, st);
   };
-  const int per_sweep = g.fused ? fused_kernels_per_sweep() : tiled_kernels_per_sweep();
+  const int per_sweep = g.fused ? fused_kernels_per_sweep(g.fz) : tiled_kernels_per_sweep();
   const int S = std::max(1, g.per_launch);
   if (n >= S && S > 1) {
     if (!g.graph) {

@@ -54,7 +54,7 @@ hipError_t launch_lds_sweeps(const LatticeSet& s, int threads, int sweeps, hipSt
 size_t lds_bytes(int N);
 // Two-kernel sweep for lattices of several full tiles whose tile-edge slots fit
 // one workgroup's LDS (fused_engine.hip); device pointers, [n_lat][N] unless noted.
-#define RBIM_FUSED_MAX_SLOTS 6400
+#define RBIM_FUSED_MAX_SLOTS 6400  // slots merged in one workgroup's LDS
 struct FusedScratch {
   int8_t* spins_alt;   // second spin buffer
   uint16_t* lab;       // [2][n_lat][N] 0, or 0x8000 | rep slot of the site's edge cluster
@@ -66,10 +66,17 @@ struct FusedScratch {
   int64_t* bst;        // [2][n_lat][6] edge clusters: n, key1, key2, n_plus1, n_plus2, wrap bits
   int64_t* fin;        // [n_lat] first sweep not yet folded into the moments
   int32_t* cur;        // [n_lat] buffer (0 / 1) holding the current state
-  int32_t tile, tiles_x, nslots;
+  // slot merge in HBM (big != 0: the slots do not fit one workgroup's LDS)
+  uint64_t* guf;       // [n_lat][nslots] slot forest: parent | wx:16 | wy:16
+  uint64_t* gcnt;      // [n_lat][nslots] size << 32 | n_plus before the flip, at reps
+  uint32_t* gmn;       // [n_lat][nslots] min site << 1 | its flip bit, at reps (else ~0)
+  int64_t* gpart;      // [n_lat][nslots / 256][5] merged-cluster partials
+  uint32_t* wrapw;     // [n_lat] wrap bits of the sweep
+  int32_t tile, tiles_x, nslots, big;
 };
 int fused_tile_size(int L);  // 0 when L does not qualify
-int fused_kernels_per_sweep();
+bool fused_merge_in_hbm(int L);
+int fused_kernels_per_sweep(const FusedScratch& f);
 hipError_t launch_fused_sweep(const LatticeSet& s, const FusedScratch& f, hipStream_t st);
 // Folds the last sweep of a run and applies its pending flips into buffer 0.
 hipError_t launch_fused_close(const LatticeSet& s, const FusedScratch& f, hipStream_t st);

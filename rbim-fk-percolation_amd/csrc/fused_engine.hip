@@ -1,6 +1,8 @@
-// Two-kernel Swendsen-Wang sweep for lattices of many tiles (L a multiple of
-// the tile side T, at most RBIM_FUSED_MAX_SLOTS tile-edge slots per lattice:
-// L <= 320 at T = 64, L <= 224 at T = 32), on gfx950.
+// Edge-slot Swendsen-Wang sweep for lattices of whole tiles (L a multiple of
+// the tile side T = 64, else 32), on gfx950.  Two kernels while a lattice's
+// tile-edge slots fit one workgroup's LDS (RBIM_FUSED_MAX_SLOTS: L <= 320 at
+// T = 64, L <= 224 at T = 32); beyond that the slot merge runs in HBM
+// (kg_* kernels, five kernels per sweep).
 //
 // The tiled path of tiled_engine.hip keeps a forest word, counters and a
 // local-root index for every site in HBM (about 49 B of traffic per
@@ -104,12 +106,21 @@ __device__ __forceinline__ uint32_t true_spin(int8_t v, uint32_t lb, uint32_t ti
   return b;
 }
 
-// Stage the lattice's flip bits (previous sweep) into LDS.
-__device__ __forceinline__ void load_flipbits(const FusedScratch& f, int lat, uint32_t* fb, int nth) {
+// The lattice's flip bits of the previous sweep: staged into LDS (lds) when
+// the lattice's slots merge in LDS, read in place from HBM (L2) otherwise.
+template <bool BIG>
+__device__ __forceinline__ const uint32_t* flipbits_of(const FusedScratch& f, int lat, uint32_t* lds,
+                                                       int nth) {
   const uint32_t words = (uint32_t)f.nslots / 32u;
   const uint32_t* src = f.flipbits + (size_t)lat * words;
-  for (uint32_t i = threadIdx.x; i < words; i += nth) fb[i] = src[i];
+  if (BIG) return src;
+  for (uint32_t i = threadIdx.x; i < words; i += nth) lds[i] = src[i];
+  return lds;
 }
+
+// LDS words for staged flip bits (none when they stay in HBM).
+template <bool BIG>
+constexpr int fb_words() { return BIG ? 1 : RBIM_FUSED_MAX_SLOTS / 32; }
 
 // Spin bits of the column beyond x0+T-1 (hsp[0..T)) and the row beyond
 // y0+T-1 (hsp[T..2T)), pending flips applied (sites of the neighbour tiles).
@@ -132,7 +143,7 @@ __device__ __forceinline__ void load_halo(const LatticeSet& s, const FusedScratc
 }
 
 // ------------------------------------------------------------------- KA
-template <int T, int NTH>
+template <int T, int NTH, bool BIG>
 __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
   constexpr int ITERS = T * T / 2 / NTH;
   constexpr uint32_t NS = 4u * T;  // edge slots per tile
@@ -142,7 +153,7 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
   __shared__ uint32_t lcnt[T * T];
   __shared__ uint8_t bits[T * T];  // bit0 spin, bit1 occ_h, bit2 occ_v, bit3 flip draw, bit4/5 J<0
   __shared__ uint8_t hsp[2 * T];
-  __shared__ uint32_t fb[RBIM_FUSED_MAX_SLOTS / 32];
+  __shared__ uint32_t fb_lds[fb_words<BIG>()];
   __shared__ int64_t red[NTH / 64][PARTS];
   const int lat = blockIdx.y;
   if (!lattice_active(s, lat)) return;
@@ -165,7 +176,7 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
   const bool meas_em = g >= 1 && f.fin[lat] == g - 1 && g - 1 >= s.n_therm;
   const bool meas_cl = g >= s.n_therm;
 
-  load_flipbits(f, lat, fb, NTH);
+  const uint32_t* fb = flipbits_of<BIG>(f, lat, fb_lds, NTH);
   __syncthreads();
   load_halo<T>(s, f, v, fb, tx, ty, hsp);
   // the tile's own spins (pending flips applied) and couplings into LDS
@@ -334,12 +345,30 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
     *reinterpret_cast<uint32_t*>(v.lout + n0) = lb2;
   }
   // edge slots for KB, and the occupation of the bonds leaving the tile
-  uint64_t* sl = f.slots + (size_t)lat * f.nslots + (size_t)tile * NS;
-  for (uint32_t k = tid; k < NS; k += NTH) {
-    const uint32_t r = par[slot_li<T>(k)] & 0xFFFFu;
-    const uint32_t rep = par[r] >> 16;
-    const uint32_t c = lcnt[r];
-    sl[k] = slot_entry(rep, r, (bits[r] >> 3) & 1u, c >> 16, c & 0xFFFFu);
+  if (BIG) {
+    // the slot forest in HBM starts here: every slot under its rep, reps
+    // carrying their cluster's counts and (min site << 1 | flip)
+    const size_t sb = (size_t)lat * f.nslots + (size_t)tile * NS;
+    for (uint32_t k = tid; k < NS; k += NTH) {
+      const uint32_t r = par[slot_li<T>(k)] & 0xFFFFu;
+      const uint32_t rep = par[r] >> 16;
+      f.guf[sb + k] = gpack(tile * NS + rep, 0, 0);
+      if (rep == k) {
+        const uint32_t c = lcnt[r];
+        f.gcnt[sb + k] = ((uint64_t)(c >> 16) << 32) | (c & 0xFFFFu);
+        f.gmn[sb + k] = (((x0 + r / T) * L + y0 + r % T) << 1) | ((bits[r] >> 3) & 1u);
+      } else {
+        f.gmn[sb + k] = 0xFFFFFFFFu;
+      }
+    }
+  } else {
+    uint64_t* sl = f.slots + (size_t)lat * f.nslots + (size_t)tile * NS;
+    for (uint32_t k = tid; k < NS; k += NTH) {
+      const uint32_t r = par[slot_li<T>(k)] & 0xFFFFu;
+      const uint32_t rep = par[r] >> 16;
+      const uint32_t c = lcnt[r];
+      sl[k] = slot_entry(rep, r, (bits[r] >> 3) & 1u, c >> 16, c & 0xFFFFu);
+    }
   }
   if (tid < 2 * T) {
     // tid < T: +x bond of (x0+T-1, y0+tid); else +y bond of (x0+tid-T, y0+T-1)
@@ -611,14 +640,168 @@ __global__ __launch_bounds__(KB_THREADS) void kb_merge(LatticeSet s, FusedScratc
   }
 }
 
+// ------------------------------------------- slot merge in HBM (large L)
+// The same merge as kb_merge for lattices whose slots do not fit one
+// workgroup's LDS: the slot forest (parent | wx | wy), the rep counts and the
+// (min site << 1 | flip) words live in HBM, one thread per bond / slot.
+template <int T>
+__global__ __launch_bounds__(256) void kg_union(LatticeSet s, FusedScratch f) {
+  constexpr uint32_t NS = 4u * T;
+  const int lat = blockIdx.y;
+  if (!lattice_active(s, lat)) return;
+  const uint32_t ns = (uint32_t)f.nslots, nt = (uint32_t)f.tiles_x;
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= ns / 2u) return;  // 2T bonds leave each tile: ns / 2 per lattice
+  const uint32_t t = b / (2u * T), i = b % (2u * T);
+  const uint64_t word = f.xbits[((size_t)lat * (ns / NS) + t) * (2 * T / 64) + i / 64];
+  if (!((word >> (i % 64)) & 1ull)) return;
+  const uint32_t tx = t % nt, ty = t / nt;
+  uint32_t a, c;
+  int cx = 0, cy = 0;
+  if (i < (uint32_t)T) {
+    a = t * NS + T + i;
+    c = (ty * nt + (tx + 1u == nt ? 0u : tx + 1u)) * NS + i;
+    cx = tx + 1u == nt ? 1 : 0;
+  } else {
+    const uint32_t j = i - T;
+    a = t * NS + 3u * T + j;
+    c = ((ty + 1u == nt ? 0u : ty + 1u) * nt + tx) * NS + 2u * T + j;
+    cy = ty + 1u == nt ? 1 : 0;
+  }
+  uint64_t* uf = f.guf + (size_t)lat * ns;
+  int ax = 0, ay = 0, bx = 0, by = 0;
+  while (true) {
+    int dax = 0, day = 0, dbx = 0, dby = 0;
+    a = gfind(uf, a, dax, day);
+    c = gfind(uf, c, dbx, dby);
+    ax += dax;
+    ay += day;
+    bx += dbx;
+    by += dby;
+    if (a == c) {
+      const uint32_t w = ((cx + bx - ax) != 0 ? 1u : 0u) | ((cy + by - ay) != 0 ? 2u : 0u);
+      if (w) atomicOr(&f.wrapw[lat], w);
+      return;
+    }
+    if (a < c) {
+      if (atomicCAS((unsigned long long*)&uf[c], (unsigned long long)gpack(c, 0, 0),
+                    (unsigned long long)gpack(a, ax - bx - cx, ay - by - cy)) == gpack(c, 0, 0))
+        return;
+    } else {
+      if (atomicCAS((unsigned long long*)&uf[a], (unsigned long long)gpack(a, 0, 0),
+                    (unsigned long long)gpack(c, bx - ax + cx, by - ay + cy)) == gpack(a, 0, 0))
+        return;
+    }
+  }
+}
+
+// Reps fold their counts and (min site << 1 | flip) into their roots.
+__global__ __launch_bounds__(256) void kg_fold(LatticeSet s, FusedScratch f) {
+  const int lat = blockIdx.y;
+  if (!lattice_active(s, lat)) return;
+  const uint32_t ns = (uint32_t)f.nslots;
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ns) return;
+  const size_t base = (size_t)lat * ns;
+  const uint32_t m = f.gmn[base + k];
+  if (m == 0xFFFFFFFFu) return;
+  const uint32_t r = groot(f.guf + base, k);
+  if (r == k) return;
+  atomicAdd((unsigned long long*)&f.gcnt[base + r], (unsigned long long)f.gcnt[base + k]);
+  atomicMin(&f.gmn[base + r], m);
+}
+
+// Flip bit of every slot's cluster; merged clusters measured at their roots
+// into per-block partials (n, key1, key2, n_plus1, n_plus2).
+__global__ __launch_bounds__(256) void kg_flip(LatticeSet s, FusedScratch f) {
+  __shared__ int64_t red[256 / 64][PARTS];
+  const int lat = blockIdx.y;
+  if (!lattice_active(s, lat)) return;
+  const uint32_t ns = (uint32_t)f.nslots;
+  // ns is a multiple of 128: a wavefront's 64 slots are all in or all out
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = k < ns;
+  const size_t base = (size_t)lat * ns;
+  const bool meas_cl = s.sweep_ctr[lat] >= s.n_therm;
+  uint32_t r = 0u, m = 0u;
+  if (valid) {
+    r = groot(f.guf + base, k);
+    m = f.gmn[base + r];
+  }
+  const uint32_t fl = m & 1u;
+  const uint64_t word = __ballot(fl != 0u);
+  if ((threadIdx.x & 63) == 0 && valid) {
+    uint32_t* fbo = f.flipbits + (size_t)lat * (ns / 32u) + k / 32u;
+    fbo[0] = (uint32_t)word;
+    fbo[1] = (uint32_t)(word >> 32);
+  }
+  if (!meas_cl) return;  // block-uniform
+  TileMeas ms = {0, 0, 0, 0ull, 0ull, 0u, 0u};
+  if (valid && r == k) {  // roots are reps
+    const uint64_t c = f.gcnt[base + k];
+    const uint32_t size = (uint32_t)(c >> 32), np = (uint32_t)c;
+    ++ms.ncl;
+    top2_insert(ms, ((uint64_t)size << 32) | (uint64_t)(0xFFFFFFFFu - (m >> 1)), fl ? size - np : np);
+  }
+  wave_clusters(ms, red);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    TileMeas a = {0, 0, 0, 0ull, 0ull, 0u, 0u};
+    int64_t ncl = 0;
+    for (int w = 0; w < 4; ++w) {
+      ncl += red[w][2];
+      top2_insert(a, (uint64_t)red[w][3], (uint32_t)red[w][5]);
+      top2_insert(a, (uint64_t)red[w][4], (uint32_t)red[w][6]);
+    }
+    int64_t* o = f.gpart + ((size_t)lat * gridDim.x + blockIdx.x) * 5;
+    o[0] = ncl;
+    o[1] = (int64_t)a.k1;
+    o[2] = (int64_t)a.k2;
+    o[3] = a.p1;
+    o[4] = a.p2;
+  }
+}
+
+// Per lattice: the merged clusters' partials and the wrap bits of sweep g,
+// then the same bookkeeping as the end of kb_merge.
+__global__ void kg_sweep_end(LatticeSet s, FusedScratch f, int nblocks) {
+  const int lat = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lat >= s.n_lat || !lattice_active(s, lat)) return;
+  const int64_t g = s.sweep_ctr[lat];
+  if (g >= s.n_therm) {
+    TileMeas a = {0, 0, 0, 0ull, 0ull, 0u, 0u};
+    int64_t ncl = 0;
+    const int64_t* p = f.gpart + (size_t)lat * nblocks * 5;
+    for (int i = 0; i < nblocks; ++i) {
+      ncl += p[5 * i];
+      top2_insert(a, (uint64_t)p[5 * i + 1], (uint32_t)p[5 * i + 3]);
+      top2_insert(a, (uint64_t)p[5 * i + 2], (uint32_t)p[5 * i + 4]);
+    }
+    int64_t* o = f.bst + ((size_t)(g & 1) * s.n_lat + lat) * 6;
+    o[0] = ncl;
+    o[1] = (int64_t)a.k1;
+    o[2] = (int64_t)a.k2;
+    o[3] = a.p1;
+    o[4] = a.p2;
+    o[5] = f.wrapw[lat];
+  }
+  f.wrapw[lat] = 0u;
+  if (g >= 1 && f.fin[lat] == g - 1) {
+    if (g - 1 >= s.n_therm) fused_finalize(s, f, lat, g - 1);
+    f.fin[lat] = g;
+  }
+  f.cur[lat] ^= 1;
+  s.sweep_ctr[lat] = g + 1;
+}
+
 // ------------------------------------------------------------ run closing
 // E, M of the current state (pending flips applied), when the last sweep
 // still has to be folded.
-template <int T, int NTH>
+template <int T, int NTH, bool BIG>
 __global__ __launch_bounds__(NTH) void kf_measure(LatticeSet s, FusedScratch f) {
   __shared__ uint8_t bits[T * T];
   __shared__ uint8_t hsp[2 * T];
-  __shared__ uint32_t fb[RBIM_FUSED_MAX_SLOTS / 32];
+  __shared__ uint32_t fb_lds[fb_words<BIG>()];
   __shared__ int64_t red[NTH / 64][PARTS];
   const int lat = blockIdx.y;
   const int64_t G = s.sweep_ctr[lat] - 1;
@@ -629,7 +812,7 @@ __global__ __launch_bounds__(NTH) void kf_measure(LatticeSet s, FusedScratch f) 
   const uint32_t x0 = tx * T, y0 = ty * T;
   const FusedView v = fused_view(s, f, lat);
   const uint8_t* jb = s.jbits + (size_t)lat * N;
-  load_flipbits(f, lat, fb, NTH);
+  const uint32_t* fb = flipbits_of<BIG>(f, lat, fb_lds, NTH);
   __syncthreads();
   load_halo<T>(s, f, v, fb, tx, ty, hsp);
   for (uint32_t li = threadIdx.x; li < T * T; li += NTH) {
@@ -662,9 +845,9 @@ __global__ __launch_bounds__(NTH) void kf_measure(LatticeSet s, FusedScratch f) 
 }
 
 // Pending flips applied into buffer 0; labels cleared.
-template <int T, int NTH>
+template <int T, int NTH, bool BIG>
 __global__ __launch_bounds__(NTH) void kf_settle(LatticeSet s, FusedScratch f) {
-  __shared__ uint32_t fb[RBIM_FUSED_MAX_SLOTS / 32];
+  __shared__ uint32_t fb_lds[fb_words<BIG>()];
   const int lat = blockIdx.y;
   const uint32_t L = (uint32_t)s.L, N = (uint32_t)s.N;
   const uint32_t nt = (uint32_t)f.tiles_x;
@@ -673,7 +856,7 @@ __global__ __launch_bounds__(NTH) void kf_settle(LatticeSet s, FusedScratch f) {
   const FusedView v = fused_view(s, f, lat);
   int8_t* s0 = s.spins + (size_t)lat * N;
   uint16_t* l0 = f.lab + (size_t)lat * N;
-  load_flipbits(f, lat, fb, NTH);
+  const uint32_t* fb = flipbits_of<BIG>(f, lat, fb_lds, NTH);
   __syncthreads();
   for (uint32_t li = threadIdx.x; li < T * T; li += NTH) {
     const uint32_t n = (x0 + li / T) * L + y0 + li % T;
@@ -699,38 +882,54 @@ size_t kb_lds_bytes(int nslots) { return (size_t)nslots * (8 + 8 + 4); }
 template <int T>
 hipError_t launch_fused_t(const LatticeSet& s, const FusedScratch& f, hipStream_t st) {
   constexpr int NTH = T == 64 ? 512 : 256;
+  const int tiles = f.tiles_x * f.tiles_x;
+  if (f.big) {
+    const int nb = (f.nslots + 255) / 256;
+    hipLaunchKernelGGL((ka_tile<T, NTH, true>), dim3(tiles, s.n_lat), dim3(NTH), 0, st, s, f);
+    hipLaunchKernelGGL((kg_union<T>), dim3((f.nslots / 2 + 255) / 256, s.n_lat), dim3(256), 0, st, s, f);
+    hipLaunchKernelGGL(kg_fold, dim3(nb, s.n_lat), dim3(256), 0, st, s, f);
+    hipLaunchKernelGGL(kg_flip, dim3(nb, s.n_lat), dim3(256), 0, st, s, f);
+    hipLaunchKernelGGL(kg_sweep_end, dim3((s.n_lat + 63) / 64), dim3(64), 0, st, s, f, nb);
+    return hipGetLastError();
+  }
   static std::atomic<uint64_t> configured{0};
   hipError_t e = ensure_dynamic_lds((const void*)kb_merge<T>,
                                     (int)kb_lds_bytes(RBIM_FUSED_MAX_SLOTS), configured);
   if (e != hipSuccess) return e;
-  const int tiles = f.tiles_x * f.tiles_x;
-  hipLaunchKernelGGL((ka_tile<T, NTH>), dim3(tiles, s.n_lat), dim3(NTH), 0, st, s, f);
+  hipLaunchKernelGGL((ka_tile<T, NTH, false>), dim3(tiles, s.n_lat), dim3(NTH), 0, st, s, f);
   hipLaunchKernelGGL((kb_merge<T>), dim3(s.n_lat), dim3(KB_THREADS), kb_lds_bytes(f.nslots), st, s, f);
+  return hipGetLastError();
+}
+
+template <int T, bool BIG>
+hipError_t launch_fused_close_tb(const LatticeSet& s, const FusedScratch& f, hipStream_t st) {
+  constexpr int NTH = T == 64 ? 512 : 256;
+  const int tiles = f.tiles_x * f.tiles_x;
+  hipLaunchKernelGGL((kf_measure<T, NTH, BIG>), dim3(tiles, s.n_lat), dim3(NTH), 0, st, s, f);
+  hipLaunchKernelGGL((kf_settle<T, NTH, BIG>), dim3(tiles, s.n_lat), dim3(NTH), 0, st, s, f);
+  hipLaunchKernelGGL(kf_close, dim3((s.n_lat + 63) / 64), dim3(64), 0, st, s, f);
   return hipGetLastError();
 }
 
 template <int T>
 hipError_t launch_fused_close_t(const LatticeSet& s, const FusedScratch& f, hipStream_t st) {
-  constexpr int NTH = T == 64 ? 512 : 256;
-  const int tiles = f.tiles_x * f.tiles_x;
-  hipLaunchKernelGGL((kf_measure<T, NTH>), dim3(tiles, s.n_lat), dim3(NTH), 0, st, s, f);
-  hipLaunchKernelGGL((kf_settle<T, NTH>), dim3(tiles, s.n_lat), dim3(NTH), 0, st, s, f);
-  hipLaunchKernelGGL(kf_close, dim3((s.n_lat + 63) / 64), dim3(64), 0, st, s, f);
-  return hipGetLastError();
+  return f.big ? launch_fused_close_tb<T, true>(s, f, st) : launch_fused_close_tb<T, false>(s, f, st);
 }
 
 }  // namespace
 
 int fused_tile_size(int L) {
-  for (int T : {64, 32}) {
-    if (L % T != 0) continue;
-    const int nt = L / T;
-    if ((size_t)nt * nt * 4 * T <= RBIM_FUSED_MAX_SLOTS) return T;
-  }
+  if (L >= 64 && L % 64 == 0) return 64;
+  if (L >= 32 && L % 32 == 0) return 32;
   return 0;
 }
 
-int fused_kernels_per_sweep() { return 2; }
+bool fused_merge_in_hbm(int L) {
+  const int T = fused_tile_size(L);
+  return T > 0 && (size_t)(L / T) * (L / T) * 4 * T > RBIM_FUSED_MAX_SLOTS;
+}
+
+int fused_kernels_per_sweep(const FusedScratch& f) { return f.big ? 5 : 2; }
 
 hipError_t launch_fused_sweep(const LatticeSet& s, const FusedScratch& f, hipStream_t st) {
   return f.tile == 64 ? launch_fused_t<64>(s, f, st) : launch_fused_t<32>(s, f, st);

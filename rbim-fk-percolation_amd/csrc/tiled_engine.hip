@@ -33,60 +33,6 @@
 
 namespace {
 
-__device__ __forceinline__ uint64_t gpack(uint32_t p, int wx, int wy) {
-  return (uint64_t)p | ((uint64_t)(uint16_t)(int16_t)wx << 32) |
-         ((uint64_t)(uint16_t)(int16_t)wy << 48);
-}
-__device__ __forceinline__ uint32_t gpar(uint64_t v) { return (uint32_t)v; }
-__device__ __forceinline__ int gwx(uint64_t v) { return (int)(int16_t)(uint16_t)(v >> 32); }
-__device__ __forceinline__ int gwy(uint64_t v) { return (int)(int16_t)(uint16_t)(v >> 48); }
-
-__device__ __forceinline__ uint64_t gload(const uint64_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void gstore(uint64_t* p, uint64_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Root and tree-path winding in the HBM forest while hooks are in flight (K2):
-// agent-scope loads so a retry sees other workgroups' hooks; path halving.
-__device__ uint32_t gfind(uint64_t* uf, uint32_t x, int& wx, int& wy) {
-  wx = 0;
-  wy = 0;
-  uint64_t v = gload(&uf[x]);
-  while (true) {
-    const uint32_t p = gpar(v);
-    if (p == x) return x;
-    const uint64_t vp = gload(&uf[p]);
-    const uint32_t gp = gpar(vp);
-    if (gp == p) {
-      wx += gwx(v);
-      wy += gwy(v);
-      return p;
-    }
-    const int hx = gwx(v) + gwx(vp), hy = gwy(v) + gwy(vp);
-    gstore(&uf[x], gpack(gp, hx, hy));
-    wx += hx;
-    wy += hy;
-    x = gp;
-    v = gload(&uf[gp]);
-  }
-}
-
-// Root in the settled forest (K3), then the start node is pointed straight at
-// it.  Windings are no longer needed once K2 has decided wrapping, and a
-// concurrent reader sees either the old ancestor chain or the root.
-__device__ __forceinline__ uint32_t groot(uint64_t* gf, uint32_t x) {
-  const uint32_t start = x;
-  uint32_t p = gpar(gf[x]);
-  while (p != x) {
-    x = p;
-    p = gpar(gf[x]);
-  }
-  if (x != start) gf[start] = gpack(x, 0, 0);
-  return x;
-}
-
 // ------------------------------------------------------------ measurement
 // Bits of the per-site occ byte: bonds and flip bit of the sweep, and -- set by
 // K3 at the root of every cluster -- the root flag the next measurement reads.

@@ -213,7 +213,7 @@ def test_allocation_stays_within_footprint(engine):
 
     rng = np.random.default_rng(4)
     specs = []
-    for k, L in enumerate((16, 32, 100, 128, 160, 256, 320, 336)):
+    for k, L in enumerate((16, 32, 100, 128, 160, 256, 320, 336, 384)):
         J_h = np.where(rng.random((L, L)) < 0.1, -1, 1).astype(np.int8)
         specs.append(LatticeSpec(L=L, J_h=J_h, J_v=J_h.T.copy(), beta=0.45, seed=3, lattice_id=k))
     need = footprint(specs, n_bins=4, engine=engine, series=16)
