@@ -60,17 +60,19 @@ def run_schedule(K):
     return K - n_bins * spb, n_bins, spb
 
 
-def gpu_batched(specs, K, W, engine, sweeps_per_launch=0, threads=0, streams=0):
+def gpu_batched(specs, K, W, engine, sweeps_per_launch=0, threads=0, streams=0, tile=0):
     from rbim.engine import Batch
 
     n_therm, n_bins, spb = run_schedule(K)
     if W > 0:
         wn, wb, ws = run_schedule(W)
         with Batch(specs, n_therm=wn, n_bins=wb, sweeps_per_bin=ws, engine=engine,
-                   sweeps_per_launch=sweeps_per_launch, threads=threads, streams=streams) as warm:
+                   sweeps_per_launch=sweeps_per_launch, threads=threads, streams=streams,
+                   tile=tile) as warm:
             warm.run_timed(W)
     with Batch(specs, n_therm=n_therm, n_bins=n_bins, sweeps_per_bin=spb, engine=engine,
-               sweeps_per_launch=sweeps_per_launch, threads=threads, streams=streams) as batch:
+               sweeps_per_launch=sweeps_per_launch, threads=threads, streams=streams,
+               tile=tile) as batch:
         t0 = time.perf_counter()
         ms = batch.run_timed(K)
         wall = time.perf_counter() - t0
@@ -168,7 +170,7 @@ def measure_point(L, args, steps, warmup, plain_steps, with_cpu):
     temperatures = np.linspace(1.0, 3.0, args.temps)
     specs = scan_specs(L, args.p, temperatures, args.realizations, seed=0)
     res = gpu_batched(specs, steps, warmup, args.engine, args.sweeps_per_launch, args.threads,
-                      args.streams)
+                      args.streams, args.tile)
     value = res["site_updates"] / (res["ms"] / 1e3)
     n_lat = len(specs)
     sites = L * L * n_lat
@@ -264,6 +266,7 @@ def main():
     ap.add_argument("--sweeps-per-launch", type=int, default=0)
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--streams", type=int, default=0, help="streams per tiled set (0: default)")
+    ap.add_argument("--tile", type=int, default=0, help="tile side of the tiled engines (0: by L)")
     ap.add_argument("--plain-steps", type=int, default=400)
     ap.add_argument("--cpu-site-updates", type=float, default=2.5e8,
                     help="site-updates per host thread in the CPU sample (about 10 s)")

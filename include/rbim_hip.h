@@ -119,6 +119,8 @@ typedef struct {
                                  measured sweeps of every lattice in a device ring
                                  (time series for autocorrelation / equilibration
                                  checks); 0 -> off                                */
+  int32_t tile;               /* tile side of the tiled engines: 32 or 64 (used when
+                                 it divides L); 0 -> 64 when it divides L, else 32 */
 } rbim_batch_desc;
 
 typedef struct {

@@ -77,6 +77,7 @@ struct Group {
   int L = 0, N = 0, n_lat = 0;
   bool lds = false;
   bool fused = false;  // tiled group on the two-kernel edge-slot sweep
+  int tile_pref = 0;   // requested tile side (0: by L)
   int threads = 256;
   int per_launch = 1;  // LDS sweeps per launch, or tiled sweeps per graph
   std::vector<int> idx;
@@ -89,17 +90,17 @@ struct Group {
   hipEvent_t done = nullptr;     // recorded after the group's work of a run
 };
 
-size_t group_bytes(int L, int n_lat, bool lds, bool fused, int n_bins, int series) {
+size_t group_bytes(int L, int n_lat, bool lds, bool fused, int n_bins, int series, int tile_pref) {
   const size_t N = (size_t)L * L;
   size_t b = N * n_lat * 2 + (size_t)n_lat * (4 + 8 + 4 + 8 + sizeof(rbim_sweep_stats)) +
              (size_t)n_lat * n_bins * RBIM_ACC_SLOTS * 8 +
              (size_t)n_lat * (size_t)std::max(series, 0) * sizeof(rbim_sweep_stats);
   if (fused) {
-    const int T = fused_tile_size(L);
+    const int T = fused_tile_size(L, tile_pref);
     const size_t tiles = (size_t)(L / T) * (L / T), ns = tiles * 4 * T;
     b += N * n_lat * (1 + 4) + (size_t)n_lat * (tiles * (2 * T / 64) * 8 + ns / 8 +
                                                  tiles * (2 * 5 + 2) * 8 + 2 * 6 * 8 + 8 + 4);
-    if (fused_merge_in_hbm(L))
+    if (fused_merge_in_hbm(L, T))
       b += (size_t)n_lat * (ns * (8 + 8 + 4) + (ns + 255) / 256 * 5 * 8 + 4);
     else
       b += (size_t)n_lat * ns * 8;
@@ -169,7 +170,7 @@ int group_alloc(rbim_ctx* ctx, Group& g, const std::vector<uint8_t>& jbits,
   }
   if (g.fused) {
     FusedScratch& f = g.fz;
-    f.tile = fused_tile_size(g.L);
+    f.tile = fused_tile_size(g.L, g.tile_pref);
     f.tiles_x = g.L / f.tile;
     const size_t tiles = (size_t)f.tiles_x * f.tiles_x;
     f.nslots = (int32_t)(tiles * 4 * f.tile);
@@ -180,7 +181,7 @@ int group_alloc(rbim_ctx* ctx, Group& g, const std::vector<uint8_t>& jbits,
     };
     HIP_TRY(ctx, zalloc(&p, NN)); f.spins_alt = (int8_t*)p;
     HIP_TRY(ctx, zalloc(&p, 2 * NN * 2)); f.lab = (uint16_t*)p;
-    f.big = fused_merge_in_hbm(g.L) ? 1 : 0;
+    f.big = fused_merge_in_hbm(g.L, f.tile) ? 1 : 0;
     if (f.big) {
       HIP_TRY(ctx, zalloc(&p, nl * f.nslots * 8)); f.guf = (uint64_t*)p;
       HIP_TRY(ctx, zalloc(&p, nl * f.nslots * 8)); f.gcnt = (uint64_t*)p;
@@ -204,7 +205,7 @@ int group_alloc(rbim_ctx* ctx, Group& g, const std::vector<uint8_t>& jbits,
     HIP_TRY(ctx, zalloc(&p, nl * 4)); f.cur = (int32_t*)p;
   } else if (!g.lds) {
     TiledScratch& t = g.tile;
-    t.tile = tiled_tile_size(g.L);
+    t.tile = tiled_tile_size(g.L, g.tile_pref);
     t.tiles_x = t.tiles_y = (g.L + t.tile - 1) / t.tile;
     HIP_TRY(ctx, g.arena.alloc(&p, NN)); t.occ = (uint8_t*)p;
     HIP_TRY(ctx, g.arena.alloc(&p, NN * 2)); t.lroot = (uint16_t*)p;
@@ -610,6 +611,8 @@ static int check_desc(rbim_ctx* ctx, const rbim_batch_desc* d) {
   if (d->engine < RBIM_ENGINE_AUTO || d->engine > RBIM_ENGINE_TILED)
     return set_err(ctx, RBIM_E_INVAL, "unknown engine %d", d->engine);
   if (d->series < 0) return set_err(ctx, RBIM_E_INVAL, "series must be >= 0");
+  if (d->tile != 0 && d->tile != 32 && d->tile != 64)
+    return set_err(ctx, RBIM_E_INVAL, "tile must be 0 (by L), 32 or 64");
   if (d->streams < 0 || d->streams > 16)
     return set_err(ctx, RBIM_E_INVAL, "streams must be 0 (default) .. 16");
   if (d->threads != 0 && d->threads != 64 && d->threads != 128 && d->threads != 256 &&
@@ -643,7 +646,7 @@ int rbim_batch_footprint(const rbim_batch_desc* d, uint64_t* bytes) {
   for (auto& kv : counts) {
     const bool lds = kv.first.second != 0;
     b += group_bytes(kv.first.first, kv.second, lds, use_fused(d, kv.first.first, lds), d->n_bins,
-                     d->series);
+                     d->series, d->tile);
   }
   if (bytes) *bytes = b;
   return RBIM_OK;
@@ -693,6 +696,7 @@ int rbim_batch_create(rbim_ctx* ctx, const rbim_batch_desc* d, rbim_batch** out)
       g.N = L * L;
       g.lds = lds != 0;
       g.fused = use_fused(d, L, g.lds);
+      g.tile_pref = d->tile;
       g.threads = d->threads ? d->threads : default_threads(L);
       g.per_launch = d->sweeps_per_launch > 0 ? d->sweeps_per_launch
                                               : (g.lds ? default_lds_sweeps(L) : 16);
@@ -712,6 +716,7 @@ int rbim_batch_create(rbim_ctx* ctx, const rbim_batch_desc* d, rbim_batch** out)
         h.N = g.N;
         h.lds = g.lds;
         h.fused = g.fused;
+        h.tile_pref = g.tile_pref;
         h.threads = g.threads;
         h.per_launch = g.per_launch;
         h.idx.assign(g.idx.begin() + n * k / parts, g.idx.begin() + n * (k + 1) / parts);

@@ -74,8 +74,10 @@ struct FusedScratch {
   uint32_t* wrapw;     // [n_lat] wrap bits of the sweep
   int32_t tile, tiles_x, nslots, big;
 };
-int fused_tile_size(int L);  // 0 when L does not qualify
-bool fused_merge_in_hbm(int L);
+// Tile side for L (pref: 32 or 64 when it divides L, else the default: 64
+// when it divides L, else 32); 0 when L is not a multiple of 32.
+int fused_tile_size(int L, int pref = 0);
+bool fused_merge_in_hbm(int L, int T);
 int fused_kernels_per_sweep(const FusedScratch& f);
 hipError_t launch_fused_sweep(const LatticeSet& s, const FusedScratch& f, hipStream_t st);
 // Folds the last sweep of a run and applies its pending flips into buffer 0.
@@ -92,7 +94,7 @@ hipError_t launch_cluster_stats(const int32_t* labels, const int8_t* spins, int6
 hipError_t launch_energy_mag(const int8_t* spins, const uint8_t* jbits, int32_t L,
                              unsigned long long* em, hipStream_t st);
 int tiled_kernels_per_sweep();
-int tiled_tile_size(int L);
+int tiled_tile_size(int L, int pref = 0);
 
 // ---------------------------------------------------------------- helpers
 __device__ __forceinline__ uint32_t site_x(uint32_t n, uint32_t magic) {

@@ -918,14 +918,14 @@ hipError_t launch_fused_close_t(const LatticeSet& s, const FusedScratch& f, hipS
 
 }  // namespace
 
-int fused_tile_size(int L) {
+int fused_tile_size(int L, int pref) {
+  if ((pref == 32 || pref == 64) && L >= pref && L % pref == 0) return pref;
   if (L >= 64 && L % 64 == 0) return 64;
   if (L >= 32 && L % 32 == 0) return 32;
   return 0;
 }
 
-bool fused_merge_in_hbm(int L) {
-  const int T = fused_tile_size(L);
+bool fused_merge_in_hbm(int L, int T) {
   return T > 0 && (size_t)(L / T) * (L / T) * 4 * T > RBIM_FUSED_MAX_SLOTS;
 }
 

@@ -663,7 +663,9 @@ __global__ void energy_mag_kernel(const int8_t* spins, const uint8_t* jb, int32_
 
 int tiled_kernels_per_sweep() { return 4; }
 
-int tiled_tile_size(int L) {
+int tiled_tile_size(int L, int pref) {
+  if (pref == 32) return 32;
+  if (pref == 64 && L % 64 == 0) return 64;
   return (L % 64 == 0 && L >= 128) ? 64 : 32;
 }
 

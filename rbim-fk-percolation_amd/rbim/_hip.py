@@ -95,6 +95,7 @@ class BatchDesc(ctypes.Structure):
         ("threads", ctypes.c_int32),
         ("streams", ctypes.c_int32),
         ("series", ctypes.c_int32),
+        ("tile", ctypes.c_int32),
     ]
 
 

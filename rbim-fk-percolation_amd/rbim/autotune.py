@@ -6,7 +6,8 @@ keeps the fastest:
 
 * LDS-resident kernel (L <= 128): workgroup size, i.e. wavefronts per lattice
   (64 ... 1024 threads), and sweeps per launch;
-* tiled kernels (L > 128): sweeps per captured hipGraph.
+* tiled kernels (L > 128): tile side (32, or 64 when it divides L) and sweeps
+  per captured hipGraph.
 
 Results go to a JSON cache keyed by GPU architecture, L and a lattice-count
 bucket (``$RBIM_AUTOTUNE_CACHE``, default ``~/.cache/rbim-hip/autotune.json``);
@@ -60,7 +61,7 @@ def _load() -> Dict[str, dict]:
 
 
 def lookup(L: int, n_lattices: int) -> Optional[dict]:
-    """Tuned {"threads", "sweeps_per_launch"} for this shape, or None."""
+    """Tuned {"threads", "sweeps_per_launch", "tile"} for this shape, or None."""
     with _lock:
         return _load().get(_key(L, n_lattices))
 
@@ -98,21 +99,25 @@ def tune(L: int, n_lattices: int = 1280, sweeps: int = 2000, store: bool = True)
     sites = L * L * n_lattices
     lds = L <= 128
     if lds:
-        shapes = [(t, k) for t in LDS_THREADS for k in LDS_SWEEPS if k <= sweeps]
+        shapes = [(t, k, 0) for t in LDS_THREADS for k in LDS_SWEEPS if k <= sweeps]
     else:
-        shapes = [(0, k) for k in GRAPH_SWEEPS if k <= sweeps]
+        tiles = [T for T in (32, 64) if L % T == 0] or [0]
+        shapes = [(0, k, T) for T in tiles for k in GRAPH_SWEEPS if k <= sweeps]
     trials = []
-    for threads, per_launch in shapes:
+    for threads, per_launch, tile in shapes:
         with Batch(specs, n_therm=sweeps // 10, n_bins=1, sweeps_per_bin=sweeps - sweeps // 10,
-                   engine=_hip.ENGINE_AUTO, threads=threads, sweeps_per_launch=per_launch) as batch:
+                   engine=_hip.ENGINE_AUTO, threads=threads, sweeps_per_launch=per_launch,
+                   tile=tile) as batch:
             warm = min(50, sweeps // 4)
             batch.run_timed(warm)
             ms = batch.run_timed(sweeps - warm)
         rate = sites * (sweeps - warm) / (max(ms, 1e-6) / 1e3)
-        trials.append({"threads": threads, "sweeps_per_launch": per_launch, "site_updates_per_s": rate})
+        trials.append({"threads": threads, "sweeps_per_launch": per_launch, "tile": tile,
+                       "site_updates_per_s": rate})
     best = max(trials, key=lambda t: t["site_updates_per_s"])
     entry = {"threads": best["threads"], "sweeps_per_launch": best["sweeps_per_launch"],
-             "site_updates_per_s": best["site_updates_per_s"], "trials": trials}
+             "tile": best["tile"], "site_updates_per_s": best["site_updates_per_s"],
+             "trials": trials}
     if store:
         _store(L, n_lattices, entry)
     return entry
@@ -127,7 +132,7 @@ def main(argv=None) -> None:
     for L in args.L:
         entry = tune(L, args.lattices, args.sweeps)
         print(json.dumps({"L": L, "threads": entry["threads"],
-                          "sweeps_per_launch": entry["sweeps_per_launch"],
+                          "sweeps_per_launch": entry["sweeps_per_launch"], "tile": entry["tile"],
                           "site_updates_per_s": entry["site_updates_per_s"]}))
 
 

@@ -116,7 +116,7 @@ class Batch:
     def __init__(self, lattices: Sequence[LatticeSpec], *, n_therm: int, n_bins: int,
                  sweeps_per_bin: int, start: str = "hot", engine: int = _hip.ENGINE_AUTO,
                  sweeps_per_launch: int = 0, threads: int = 0, streams: int = 0,
-                 series: int = 0):
+                 series: int = 0, tile: int = 0):
         from rbim.clusters import activation_threshold
 
         if start not in ("hot", "cold"):
@@ -144,12 +144,13 @@ class Batch:
             self._spins0 = np.ascontiguousarray(np.concatenate([
                 (np.asarray(s.spins0, np.int8).reshape(-1) if s.spins0 is not None
                  else np.ones(s.L * s.L, np.int8)) for s in lattices]))
-        if threads == 0 and sweeps_per_launch == 0 and len(set(self.Ls.tolist())) == 1:
+        if threads == 0 and sweeps_per_launch == 0 and tile == 0 and len(set(self.Ls.tolist())) == 1:
             from rbim import autotune
 
             tuned = autotune.lookup(int(self.Ls[0]), self.n)
             if tuned:
                 threads, sweeps_per_launch = int(tuned["threads"]), int(tuned["sweeps_per_launch"])
+                tile = int(tuned.get("tile", 0))
         self.threads, self.sweeps_per_launch = int(threads), int(sweeps_per_launch)
         desc = _hip.BatchDesc()
         desc.n_lattices = self.n
@@ -168,6 +169,7 @@ class Batch:
         desc.sweeps_per_launch = int(sweeps_per_launch)
         desc.threads = int(threads)
         desc.streams = int(streams)
+        desc.tile = int(tile)
         if series < 0:
             raise ValueError("series must be >= 0")
         self.series_capacity = int(series)
@@ -254,7 +256,7 @@ class Batch:
 
 
 def footprint(lattices: Sequence[LatticeSpec], *, n_bins: int, engine: int = _hip.ENGINE_AUTO,
-              series: int = 0) -> int:
+              series: int = 0, tile: int = 0) -> int:
     """Device bytes a batch of these lattices would hold (admission control)."""
     Ls = np.array([s.L for s in lattices], dtype=np.int32)
     n = int(Ls.size)
@@ -275,6 +277,7 @@ def footprint(lattices: Sequence[LatticeSpec], *, n_bins: int, engine: int = _hi
     desc.sweeps_per_bin = 1
     desc.engine = int(engine)
     desc.series = int(series)
+    desc.tile = int(tile)
     out = ctypes.c_uint64()
     _hip.check(_hip.lib().rbim_batch_footprint(ctypes.byref(desc), ctypes.byref(out)))
     return int(out.value)
