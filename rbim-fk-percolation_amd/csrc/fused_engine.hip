@@ -142,6 +142,19 @@ __device__ __forceinline__ void load_halo(const LatticeSet& s, const FusedScratc
   hsp[tid] = (uint8_t)true_spin<T>(v.sin[hn], v.lin[hn], ntile, fb);
 }
 
+// Root in the settled tile forest, reading parents through a 16-bit mask (a
+// root's word may already carry its rep slot above bit 16).
+__device__ __forceinline__ uint32_t troot16(const uint32_t* par, uint32_t x) {
+  uint32_t p = par[x] & 0xFFFFu;
+  while (p != x) {
+    const uint32_t g = par[p] & 0xFFFFu;
+    if (g == p) return p;
+    x = g;
+    p = par[x] & 0xFFFFu;
+  }
+  return x;
+}
+
 // ------------------------------------------------------------------- KA
 template <int T, int NTH, bool BIG>
 __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
@@ -278,8 +291,8 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
   for (int it = 0; it < ITERS; ++it) {
     const uint32_t li0 = (uint32_t)(it * NTH + tid) << 1, li1 = li0 + 1u;
     const uint32_t b0 = bits[li0], b1 = bits[li1];
-    const uint32_t r0 = troot(par, li0);
-    const uint32_t r1 = (b0 & 4u) ? r0 : troot(par, li1);
+    const uint32_t r0 = troot16(par, li0);
+    const uint32_t r1 = (b0 & 4u) ? r0 : troot16(par, li1);
     // the first lane's cluster (often most of the wave) is summed by ballots
     // and added by one atomic
     const uint32_t R = __builtin_amdgcn_readfirstlane(r0);
@@ -295,14 +308,14 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
       if (!in0) atomicAdd(&lcnt[r0], 0x10000u | (b0 & 1u));
       if (!in1) atomicAdd(&lcnt[r1], 0x10000u | (b1 & 1u));
     }
-    if (r0 != li0) par[li0] = r0;
-    if (r1 != li1) par[li1] = r1;
+    // every site now points straight at its root; a root's word becomes
+    // root | rep << 16 with rep = "none" until an edge slot claims it (the
+    // walks above read parents through a 16-bit mask)
+    par[li0] = r0 != li0 ? r0 : li0 | (NONE16 << 16);
+    par[li1] = r1 != li1 ? r1 : li1 | (NONE16 << 16);
   }
   __syncthreads();
   // rep slot of every edge cluster: par[root] = root | min slot << 16
-  for (uint32_t li = tid; li < T * T; li += NTH)
-    if (par[li] == li) par[li] = li | (NONE16 << 16);
-  __syncthreads();
   for (uint32_t k = tid; k < NS; k += NTH) {
     const uint32_t r = par[slot_li<T>(k)] & 0xFFFFu;
     atomicMin(&par[r], r | (k << 16));
