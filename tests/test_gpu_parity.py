@@ -250,10 +250,11 @@ def test_run_past_end_is_refused(hip):
             batch.run(1)
 
 
-@pytest.mark.parametrize("Ls", [(32, 64, 96), (128, 160, 256), (288, 384)])
+@pytest.mark.parametrize("Ls", [(32, 64, 96), (128, 160, 256), (320,), (288, 384)])
 def test_edge_slot_sweep_equals_hbm_forest_sweep(hip, coracle, Ls):
     """The edge-slot sweep (ENGINE_PLAIN when L is a multiple of 32: slots merged
-    in LDS, or in HBM for 288 = 9 x 9 tiles of 32 and 384 = 6 x 6 tiles of 64)
+    in LDS -- 320: 6400 slots, the most one workgroup holds -- or in HBM for
+    288 = 9 x 9 tiles of 32 and 384 = 6 x 6 tiles of 64)
     and the multi-kernel HBM-forest sweep (ENGINE_TILED) give identical
     trajectories, moment sums and per-sweep series, across split runs; and both
     equal the oracle."""

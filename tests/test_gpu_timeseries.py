@@ -122,3 +122,39 @@ def test_equilibration_check_and_script():
                          capture_output=True, text=True, timeout=300)
     assert res.returncode == 0, res.stderr
     assert "tau_int" in res.stdout and "2.60" in res.stdout
+
+
+def test_device_moments_match_reference_float_means():
+    """The reference forms 14 float64 series per lattice and takes np.mean over
+    the run and over each bin (simulation.py:206-241).  Rebuilt from the
+    device's per-sweep ring, those means agree with the device's bin sums
+    (exact integers and one FP64 sum) to 1e-12 relative, for every primitive."""
+    from rbim.engine import PRIMITIVES, Batch, moments_to_primitives
+
+    specs = _specs((16, 32, 64), temps=(1.6, 2.3, 3.0))
+    n_therm, spb, n_bins = 30, 40, 5
+    with Batch(specs, n_therm=n_therm, n_bins=n_bins, sweeps_per_bin=spb, series=spb * n_bins) as b:
+        b.run()
+        acc = b.moments()
+        ser = b.series()
+    for i in range(len(specs)):
+        st = ser[i]
+        E = st["energy"].astype(np.float64)
+        M = st["magnetization"].astype(np.float64)
+        S = st["largest_size"].astype(np.float64)
+        MC = st["largest_abs_mag"].astype(np.float64)
+        series = {
+            "energy": E, "energy_sq": E * E, "magnetization": M, "abs_magnetization": np.abs(M),
+            "magnetization_sq": M ** 2, "magnetization_4": M ** 4, "largest_size": S,
+            "largest_abs_mag": MC, "second_size": st["second_size"].astype(np.float64),
+            "mag_ratio": np.divide(MC, S, out=np.zeros_like(S), where=S > 0),
+            "n_clusters": st["n_clusters"].astype(np.float64),
+            "wrap_x": st["wrap_x"].astype(np.float64), "wrap_y": st["wrap_y"].astype(np.float64),
+            "wrap_any": ((st["wrap_x"] != 0) | (st["wrap_y"] != 0)).astype(np.float64),
+        }
+        prim, bins = moments_to_primitives(acc[i])
+        for name in PRIMITIVES:
+            ref = float(np.mean(series[name]))
+            ref_bins = series[name].reshape(n_bins, spb).mean(axis=1)
+            assert prim[name] == pytest.approx(ref, rel=1e-12, abs=1e-12), (i, name)
+            np.testing.assert_allclose(bins[name], ref_bins, rtol=1e-12, atol=1e-12, err_msg=name)
