@@ -1,14 +1,18 @@
-// Tiled Swendsen-Wang engine for any L >= 3 on gfx950: the plain path (one
-// lattice, sweep by sweep) and the large-lattice path (L > 128, many lattices
-// per launch).  A sweep is five stream-ordered kernels:
+// Multi-kernel tiled Swendsen-Wang sweep for any L >= 3 on gfx950: lattices
+// whose side is not a multiple of 32, the one-lattice parity entry points
+// (given bits, canonical labels) and RBIM_ENGINE_TILED.  A sweep is four
+// stream-ordered kernels (tiles of 64 x 64 when 64 divides L >= 128, else
+// 32 x 32; partial tiles at the edge when the tile does not divide L):
 //
-//   K1 tile      32x32 tile per workgroup: Philox activation of every site,
-//                occupation/flip bits to HBM, union-find of the bonds inside
-//                the tile in LDS (no winding: a tile never crosses the periodic
-//                boundary, so every path inside it has winding 0).  Every site
-//                gets its tile-local root (= min site of its tile-local
-//                cluster); each local root opens a node of the HBM forest and
-//                records its local size / n_plus.
+//   K1 tile      one workgroup per tile: measures the previous sweep on the
+//                tile (E, M from the spins; clusters from the root flag K3
+//                left and the root's counters), Philox activation of every
+//                site, occupation/flip bits to HBM, union-find of the bonds
+//                inside the tile in LDS (no winding: a tile never crosses the
+//                periodic boundary).  Every site gets its tile-local root (the
+//                min site of its tile-local cluster, as a 16-bit tile index);
+//                each local root opens a node of the HBM forest and records its
+//                local size / n_plus.
 //   K2 merge     one thread per tile-boundary bond (including the periodic
 //                ones): union of the two local roots in the HBM forest with a
 //                64-bit CAS on (parent | wx | wy), larger root hooked under
@@ -16,18 +20,17 @@
 //                non-tree bond and its loop winding decides wrap_x / wrap_y
 //                (_unionfind.py:101-124).  Bonds inside a tile close only
 //                windingless loops and need no test.
-//   K3 flip      per site: global root of its local root, s ^= flipbit(root);
-//                each local root adds its local cluster's (size, n_plus after
-//                the flip) to its global root -- one atomic per local cluster.
-//   K4 measure   per tile: E, M, n_clusters, two largest clusters (64-bit keys
-//                size << 32 | ~root) -> per-tile partials.
-//   K5 finalize  per lattice: fold the partials, fold the sweep into its bin's
-//                moment sums (simulation.py:212-236), advance the sweep counter.
+//   K3 flip      per tile: each local root resolves its global root and flip
+//                bit into an LDS table, every site flips from it, each local
+//                root adds its (size, n_plus after the flip) to its global root
+//                -- one atomic per local cluster -- and global roots are
+//                flagged for the next K1.
+//   K5 finalize  per lattice: fold the previous sweep's tile partials into its
+//                bin's moment sums (simulation.py:212-236), advance the counter.
 //
-// Per site and sweep only the occupation byte and the 4-byte local root are
-// written; the forest, the local counts and the cluster accumulators live at
-// local roots.  Every kernel reads the lattice's own sweep counter from HBM,
-// so a captured graph of sweeps replays without host arguments changing.
+// K4 + a flush kernel measure and fold the last sweep of a run.  Every kernel
+// reads the lattice's own sweep counter from HBM, so a captured graph of
+// sweeps replays without host arguments changing.
 #include "engine.h"
 #include "tile_common.h"
 
