@@ -9,6 +9,7 @@ in their thermal streams; every observable must agree within three combined
 jackknife errors.
 """
 
+import json
 import os
 import subprocess
 import sys
@@ -99,6 +100,19 @@ def test_run_sweep_cli(tmp_path):
     assert res["metadata"]["n_realizations"] == 3 and len(res["temperatures"]) == 4
     assert set(res["observables"]) == set(OBSERVABLES)
     assert np.all(np.isfinite(res["observables"]["energy"]["value"]))
+    # The file has the layout of a results file written by the reference
+    # (same arrays, dtypes and metadata keys), which is what the reference's
+    # analysis scripts open with load_results (analyze_crossings.py:72-88,
+    # analyze_theta.py:142, make_figures.py:93).
+    ref_path = os.path.join(GOLDEN, "survey_L16_p0.00.npz")
+    with np.load(str(out), allow_pickle=False) as ours, np.load(ref_path, allow_pickle=False) as ref:
+        assert set(ours.files) == set(ref.files)
+        for key in ref.files:
+            assert ours[key].dtype == ref[key].dtype, key
+            assert ours[key].shape[1:] == ref[key].shape[1:], key
+        ref_md = json.loads(str(ref["metadata_json"]))
+        our_md = json.loads(str(ours["metadata_json"]))
+        assert set(ref_md) <= set(our_md)
 
 
 def test_critical_temperature_from_crossings_at_scale():
