@@ -68,16 +68,15 @@ __device__ __forceinline__ uint32_t lds_find(uint32_t* uf, uint32_t x, int& wx, 
   }
 }
 
-__device__ __forceinline__ uint32_t lds_root(const uint32_t* uf, uint32_t x) {
-  uint32_t v = uf[x];
-  while ((v & PMASK) != x) {
-    const uint32_t p = v & PMASK;
-    const uint32_t vp = uf[p];
-    if ((vp & PMASK) == p) return p;
-    x = vp & PMASK;
-    v = uf[x];
+// Roots of x and y in the settled forest (phase C), walked together.
+__device__ __forceinline__ void lds_root2(const uint32_t* uf, uint32_t& x, uint32_t& y) {
+  uint32_t px = uf[x] & PMASK, py = uf[y] & PMASK;
+  while (px != x || py != y) {
+    x = px;
+    y = py;
+    px = uf[x] & PMASK;
+    py = uf[y] & PMASK;
   }
-  return x;
 }
 
 // Walk a and b to their current roots together (the two chains' loads are in
@@ -301,7 +300,7 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
   const bool given = s.given_occ != nullptr;
   uint64_t* gacc = s.acc + (size_t)lat * s.n_bins * RBIM_ACC_SLOTS;
 
-  int64_t cur_bin = -1;  // lane 0 of wave 0 only
+  int64_t cur_bin = -1, bin_end = 0;  // lane 0 of wave 0 only
   __syncthreads();
 
   // Measurement bookkeeping for one pass (counts are wave-uniform).
@@ -360,12 +359,13 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
     wrapflag[gm & 1] = 0u;
     st.wrap_x = (int32_t)(wf & 1u);
     st.wrap_y = (int32_t)((wf >> 1) & 1u);
-    const int64_t bin = (gm - s.n_therm) / s.spb;
-    if (bin != cur_bin) {
+    if (cur_bin < 0 || gm >= bin_end) {  // measured sweeps arrive in order
+      const int64_t bin = (gm - s.n_therm) / s.spb;
       if (cur_bin >= 0)
         for (int k = 0; k < RBIM_ACC_SLOTS; ++k) gacc[cur_bin * RBIM_ACC_SLOTS + k] = a[k];
       for (int k = 0; k < RBIM_ACC_SLOTS; ++k) a[k] = gacc[bin * RBIM_ACC_SLOTS + k];
       cur_bin = bin;
+      bin_end = s.n_therm + (bin + 1) * s.spb;
     }
     acc_sweep(a, st);
     if (s.last) s.last[lat] = st;
@@ -514,15 +514,16 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
       const Pair p = make_pair<LC, ALL>(q, N, L, magic);
       uint32_t r0 = 0u, r1 = 0u, ns0 = 0u, ns1 = 0u;
       if (p.v0) {
-        r0 = lds_root(uf, p.n0);
         const uint32_t b0 = sb[p.n0];
+        // n1 shares n0's run (and root) when the in-row bond n0 -> n1 is occupied
+        const bool linked = ((b0 >> 4) & 1u) && p.y0 != L - 1u;
+        r0 = p.n0;
+        r1 = p.v1 && !linked ? p.n1 : p.n0;
+        lds_root2(uf, r0, r1);
         ns0 = (b0 ^ (sb[r0] >> 5)) & 1u;
         sb[p.n0] = (uint8_t)((b0 & ~1u) | ns0);
         if (s.canon_out) s.canon_out[p.n0] = (int32_t)r0;
         if (p.v1) {
-          // n1 shares n0's run (and root) when the in-row bond n0 -> n1 is occupied
-          const bool linked = ((b0 >> 4) & 1u) && p.y0 != L - 1u;
-          r1 = linked ? r0 : lds_root(uf, p.n1);
           const uint32_t b1 = sb[p.n1];
           ns1 = (b1 ^ (sb[r1] >> 5)) & 1u;
           sb[p.n1] = (uint8_t)((b1 & ~1u) | ns1);

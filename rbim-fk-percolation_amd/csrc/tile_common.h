@@ -142,29 +142,24 @@ __device__ __forceinline__ uint32_t run_start(uint64_t thru, uint64_t ol, uint64
 // crosses the periodic boundary).  Two finds walked together with path
 // halving; the larger root hooks under the smaller, so every root is the
 // minimum index (= minimum site) of its tile-local cluster.
+// One step of a halving walk: x has the parent px and the grandparent g.  A
+// non-root x is pointed at g and the walk moves there; at a root (px == x)
+// the walk stays.  Only non-root words are written (hooks CAS roots only).
+__device__ __forceinline__ void halve_step(uint32_t* par, uint32_t& x, uint32_t px, uint32_t g) {
+  const bool halve = px != x && g != px;
+  if (halve) par[x] = g;
+  x = halve ? g : px;
+}
+
 __device__ __forceinline__ void tfind2(uint32_t* par, uint32_t& a, uint32_t& b) {
   uint32_t pa = par[a], pb = par[b];
   while (pa != a || pb != b) {
-    const uint32_t ga = pa != a ? par[pa] : pa;
-    const uint32_t gb = pb != b ? par[pb] : pb;
-    if (pa != a) {
-      if (ga == pa) {
-        a = pa;
-      } else {
-        par[a] = ga;
-        a = ga;
-        pa = par[ga];
-      }
-    }
-    if (pb != b) {
-      if (gb == pb) {
-        b = pb;
-      } else {
-        par[b] = gb;
-        b = gb;
-        pb = par[gb];
-      }
-    }
+    // a root re-reads its own word; a hook made meanwhile shows as a new parent
+    const uint32_t ga = par[pa], gb = par[pb];
+    halve_step(par, a, pa, ga);
+    halve_step(par, b, pb, gb);
+    pa = par[a];
+    pb = par[b];
   }
 }
 
