@@ -79,6 +79,19 @@ __device__ __forceinline__ void lds_root2(const uint32_t* uf, uint32_t& x, uint3
   }
 }
 
+// Roots of four sites at once (the two pairs of a lane when it owns exactly two).
+__device__ __forceinline__ void lds_root4(const uint32_t* uf, uint32_t (&x)[4]) {
+  uint32_t p[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) p[c] = uf[x[c]] & PMASK;
+  while (p[0] != x[0] || p[1] != x[1] || p[2] != x[2] || p[3] != x[3]) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) x[c] = p[c];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) p[c] = uf[x[c]] & PMASK;
+  }
+}
+
 // Walk a and b to their current roots together (the two chains' loads are in
 // flight at the same time), accumulating tree-path windings and halving both
 // paths.  a / b may be stale roots from a failed hook: the windings carried so
@@ -256,6 +269,8 @@ template <int NT, int LC>
 __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t sweeps) {
   constexpr int NW = NT / 64;
   constexpr bool ALL = LC > 0 && (LC % 2) == 0 && ((LC * LC / 2) % NT) == 0;
+  // exactly two pairs per lane: phase C walks the four roots together
+  constexpr bool MERGED = ALL && LC * LC / 2 == 2 * NT;
   using Key = ClusterKey<(LC > 0 && LC <= 32)>;
   using KT = typename Key::T;
   extern __shared__ uint32_t smem[];
@@ -311,7 +326,8 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
     m_up += popc_ballot(valid && (b & 1u));
     const bool root = valid && (uf[n] & PMASK) == n;
     m_root += popc_ballot(root);
-    if (root) top2_merge(k1, k2, Key::make(cnt[n], n), (KT)0u);
+    const KT key = Key::make(cnt[n], n);
+    top2_merge(k1, k2, root ? key : (KT)0u, (KT)0u);
   };
   auto publish = [&]() {
 #pragma unroll
@@ -458,16 +474,17 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
       if (p.v0) {
         b0 = sb[p.n0];
         uint32_t h0 = (b0 >> 3) & 1u & (p.x0 != L - 1u ? 1u : 0u);
-        if (h0 && p.y0 != 0u) {
-          const uint32_t left = sb[p.n0 - 1u], left_up = sb[p.nr0 - 1u];
-          h0 &= ~((left >> 3) & (left >> 4) & (left_up >> 4)) & 1u;
+        {
+          const bool edge = p.y0 == 0u;
+          const uint32_t left = sb[edge ? p.n0 : p.n0 - 1u], left_up = sb[edge ? p.nr0 : p.nr0 - 1u];
+          h0 &= ~((left >> 3) & (left >> 4) & (left_up >> 4) & (edge ? 0u : 1u)) & 1u;
         }
         req |= h0;
       }
       if (p.v1) {
         const uint32_t b1 = sb[p.n1];
         uint32_t h1 = (b1 >> 3) & 1u & (p.x1 != L - 1u ? 1u : 0u);
-        if (h1 && p.y1 != 0u) h1 &= ~((b0 >> 3) & (b0 >> 4) & (sb[p.nr0] >> 4)) & 1u;
+        h1 &= ~((b0 >> 3) & (b0 >> 4) & (sb[p.nr0] >> 4) & (p.y1 != 0u ? 1u : 0u)) & 1u;
         req |= h1 << 1;
         req |= ((b1 >> 4) & 1u & ((lane == 63 && p.y1 != L - 1u) ? 1u : 0u)) << 2;
       }
@@ -508,18 +525,37 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
     // Counts go to cnt[root] by LDS atomics; the sites of the first lane's
     // cluster (at low temperature usually most of the wave, one giant cluster)
     // are summed by ballots and added by a single atomic.
+    uint32_t rr[4] = {0u, 0u, 0u, 0u};  // MERGED: roots of n0, n1 of both pairs
+    uint32_t bb[2] = {0u, 0u};          // MERGED: state bytes of both n0
+    if constexpr (MERGED) {  // even L: n1 is linked to n0 by the in-row bond bit
+#pragma unroll
+      for (int it = 0; it < 2; ++it) {
+        const uint32_t n0 = 2u * (uint32_t)(it * NT + tid);
+        bb[it] = sb[n0];
+        rr[2 * it] = n0;
+        rr[2 * it + 1] = ((bb[it] >> 4) & 1u) ? n0 : n0 + 1u;
+      }
+      lds_root4(uf, rr);
+    }
 #pragma unroll 1
     for (int it = 0; it < iters; ++it) {
       const uint32_t q = (uint32_t)(it * NT + tid);
       const Pair p = make_pair<LC, ALL>(q, N, L, magic);
       uint32_t r0 = 0u, r1 = 0u, ns0 = 0u, ns1 = 0u;
       if (p.v0) {
-        const uint32_t b0 = sb[p.n0];
-        // n1 shares n0's run (and root) when the in-row bond n0 -> n1 is occupied
-        const bool linked = ((b0 >> 4) & 1u) && p.y0 != L - 1u;
-        r0 = p.n0;
-        r1 = p.v1 && !linked ? p.n1 : p.n0;
-        lds_root2(uf, r0, r1);
+        uint32_t b0;
+        if constexpr (MERGED) {
+          b0 = it == 0 ? bb[0] : bb[1];
+          r0 = it == 0 ? rr[0] : rr[2];
+          r1 = it == 0 ? rr[1] : rr[3];
+        } else {
+          b0 = sb[p.n0];
+          // n1 shares n0's run (and root) when the in-row bond n0 -> n1 is occupied
+          const bool linked = ((b0 >> 4) & 1u) && p.y0 != L - 1u;
+          r0 = p.n0;
+          r1 = p.v1 && !linked ? p.n1 : p.n0;
+          lds_root2(uf, r0, r1);
+        }
         ns0 = (b0 ^ (sb[r0] >> 5)) & 1u;
         sb[p.n0] = (uint8_t)((b0 & ~1u) | ns0);
         if (s.canon_out) s.canon_out[p.n0] = (int32_t)r0;
