@@ -42,32 +42,6 @@ __device__ __forceinline__ uint32_t uf_pack(uint32_t p, int wx, int wy) {
 __device__ __forceinline__ int uf_wx(uint32_t v) { return ((int)(v << 9)) >> 23; }
 __device__ __forceinline__ int uf_wy(uint32_t v) { return ((int)v) >> 23; }
 
-// Root of x with the winding of the tree path x -> root; halves the path.
-// Stores only touch non-root words, which hooks (CAS on roots) never target.
-__device__ __forceinline__ uint32_t lds_find(uint32_t* uf, uint32_t x, int& wx, int& wy) {
-  wx = 0;
-  wy = 0;
-  uint32_t v = uf[x];
-  while (true) {
-    const uint32_t p = v & PMASK;
-    if (p == x) return x;
-    const uint32_t vp = uf[p];
-    const uint32_t gp = vp & PMASK;
-    if (gp == p) {
-      wx += uf_wx(v);
-      wy += uf_wy(v);
-      return p;
-    }
-    const int hx = uf_wx(v) + uf_wx(vp);
-    const int hy = uf_wy(v) + uf_wy(vp);
-    uf[x] = uf_pack(gp, hx, hy);
-    wx += hx;
-    wy += hy;
-    x = gp;
-    v = uf[gp];
-  }
-}
-
 // Roots of x and y in the settled forest (phase C), walked together.
 __device__ __forceinline__ void lds_root2(const uint32_t* uf, uint32_t& x, uint32_t& y) {
   uint32_t px = uf[x] & PMASK, py = uf[y] & PMASK;
