@@ -142,17 +142,17 @@ __device__ __forceinline__ void load_halo(const LatticeSet& s, const FusedScratc
   hsp[tid] = (uint8_t)true_spin<T>(v.sin[hn], v.lin[hn], ntile, fb);
 }
 
-// Root in the settled tile forest, reading parents through a 16-bit mask (a
-// root's word may already carry its rep slot above bit 16).
-__device__ __forceinline__ uint32_t troot16(const uint32_t* par, uint32_t x) {
-  uint32_t p = par[x] & 0xFFFFu;
-  while (p != x) {
-    const uint32_t g = par[p] & 0xFFFFu;
-    if (g == p) return p;
-    x = g;
-    p = par[x] & 0xFFFFu;
+// Roots of x and y in the settled tile forest, walked together, reading
+// parents through a 16-bit mask (a root's word may already carry its rep slot
+// above bit 16).
+__device__ __forceinline__ void troot16_2(const uint32_t* par, uint32_t& x, uint32_t& y) {
+  uint32_t px = par[x] & 0xFFFFu, py = par[y] & 0xFFFFu;
+  while (px != x || py != y) {
+    x = px;
+    y = py;
+    px = par[x] & 0xFFFFu;
+    py = par[y] & 0xFFFFu;
   }
-  return x;
 }
 
 // ------------------------------------------------------------------- KA
@@ -268,13 +268,12 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
     const uint32_t b0 = bits[li0], b1 = bits[li1];
     uint32_t req = 0u;
     if (lx + 1u < T) {
-      uint32_t h0 = (b0 >> 1) & 1u;
-      if (h0 && ly0 != 0u) {
-        const uint32_t left = bits[li0 - 1u], left_up = bits[li0 - 1u + T];
-        h0 &= ~((left >> 1) & (left >> 2) & (left_up >> 2)) & 1u;
-      }
-      uint32_t h1 = (b1 >> 1) & 1u;
-      if (h1) h1 &= ~((b0 >> 1) & (b0 >> 2) & (bits[li0 + T] >> 2)) & 1u;
+      const bool first = ly0 == 0u;  // no plaquette to the left
+      const uint32_t lf = first ? li0 : li0 - 1u;
+      const uint32_t left = bits[lf], left_up = bits[lf + T];
+      const uint32_t h0 = (b0 >> 1) & 1u &
+                          ~((left >> 1) & (left >> 2) & (left_up >> 2) & (first ? 0u : 1u));
+      const uint32_t h1 = (b1 >> 1) & 1u & ~((b0 >> 1) & (b0 >> 2) & (bits[li0 + T] >> 2));
       req = h0 | (h1 << 1);
     }
     req |= ((b1 >> 2) & 1u & ((lane == 63 && ly0 + 1u != T - 1u) ? 1u : 0u)) << 2;
@@ -291,8 +290,8 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
   for (int it = 0; it < ITERS; ++it) {
     const uint32_t li0 = (uint32_t)(it * NTH + tid) << 1, li1 = li0 + 1u;
     const uint32_t b0 = bits[li0], b1 = bits[li1];
-    const uint32_t r0 = troot16(par, li0);
-    const uint32_t r1 = (b0 & 4u) ? r0 : troot16(par, li1);
+    uint32_t r0 = li0, r1 = (b0 & 4u) ? li0 : li1;  // li1 shares li0's run when linked
+    troot16_2(par, r0, r1);
     // the first lane's cluster (often most of the wave) is summed by ballots
     // and added by one atomic
     const uint32_t R = __builtin_amdgcn_readfirstlane(r0);

@@ -182,15 +182,15 @@ __device__ __forceinline__ void tunion(uint32_t* par, uint32_t a, uint32_t b) {
   }
 }
 
-__device__ __forceinline__ uint32_t troot(const uint32_t* par, uint32_t x) {
-  uint32_t p = par[x];
-  while (p != x) {
-    const uint32_t g = par[p];
-    if (g == p) return p;
-    x = g;
-    p = par[x];
+// Roots of x and y in the settled tile forest, walked together.
+__device__ __forceinline__ void troot2(const uint32_t* par, uint32_t& x, uint32_t& y) {
+  uint32_t px = par[x], py = par[y];
+  while (px != x || py != y) {
+    x = px;
+    y = py;
+    px = par[x];
+    py = par[y];
   }
-  return x;
 }
 
 __device__ __forceinline__ void wave_union_queue(uint32_t* par, uint32_t* q, uint32_t req,

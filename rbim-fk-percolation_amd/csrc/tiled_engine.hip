@@ -334,13 +334,12 @@ __global__ __launch_bounds__(NTH, 8) void k1_tile_full(LatticeSet s, TiledScratc
     const uint32_t b0 = bits[li0], b1 = bits[li1];
     uint32_t req = 0u;
     if (lx + 1u < T) {
-      uint32_t h0 = (b0 >> 1) & 1u;
-      if (h0 && ly0 != 0u) {
-        const uint32_t left = bits[li0 - 1u], left_up = bits[li0 - 1u + T];
-        h0 &= ~((left >> 1) & (left >> 2) & (left_up >> 2)) & 1u;
-      }
-      uint32_t h1 = (b1 >> 1) & 1u;
-      if (h1) h1 &= ~((b0 >> 1) & (b0 >> 2) & (bits[li0 + T] >> 2)) & 1u;
+      const bool first = ly0 == 0u;  // no plaquette to the left
+      const uint32_t lf = first ? li0 : li0 - 1u;
+      const uint32_t left = bits[lf], left_up = bits[lf + T];
+      const uint32_t h0 = (b0 >> 1) & 1u &
+                          ~((left >> 1) & (left >> 2) & (left_up >> 2) & (first ? 0u : 1u));
+      const uint32_t h1 = (b1 >> 1) & 1u & ~((b0 >> 1) & (b0 >> 2) & (bits[li0 + T] >> 2));
       req = h0 | (h1 << 1);
     }
     req |= ((b1 >> 2) & 1u & ((lane == 63 && ly0 + 1u != T - 1u) ? 1u : 0u)) << 2;
@@ -359,8 +358,8 @@ __global__ __launch_bounds__(NTH, 8) void k1_tile_full(LatticeSet s, TiledScratc
     const uint32_t lx = ql / (T / 2), ly0 = (ql % (T / 2)) * 2u;
     const uint32_t li0 = ql << 1, li1 = li0 + 1u;
     const uint32_t b0 = bits[li0], b1 = bits[li1];
-    const uint32_t r0 = troot(par, li0);
-    const uint32_t r1 = (b0 & 4u) ? r0 : troot(par, li1);
+    uint32_t r0 = li0, r1 = (b0 & 4u) ? li0 : li1;  // li1 shares li0's run when linked
+    troot2(par, r0, r1);
     if (r0 == r1) {
       atomicAdd(&lcnt[r0], 0x20000u | ((b0 & 1u) + (b1 & 1u)));
     } else {
