@@ -155,6 +155,19 @@ __device__ __forceinline__ void troot16_2(const uint32_t* par, uint32_t& x, uint
   }
 }
 
+// Roots of four sites at once (two site pairs of a lane), as troot16_2.
+__device__ __forceinline__ void troot16_4(const uint32_t* par, uint32_t (&x)[4]) {
+  uint32_t p[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) p[c] = par[x[c]] & 0xFFFFu;
+  while (p[0] != x[0] || p[1] != x[1] || p[2] != x[2] || p[3] != x[3]) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) x[c] = p[c];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) p[c] = par[x[c]] & 0xFFFFu;
+  }
+}
+
 // ------------------------------------------------------------------- KA
 template <int T, int NTH, bool BIG>
 __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
@@ -286,12 +299,8 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
   __syncthreads();
   // roots: counts (size << 16 | n_plus before the flip); every site is pointed
   // straight at its root (the forest is settled, so this is only compression)
-#pragma unroll 1
-  for (int it = 0; it < ITERS; ++it) {
-    const uint32_t li0 = (uint32_t)(it * NTH + tid) << 1, li1 = li0 + 1u;
-    const uint32_t b0 = bits[li0], b1 = bits[li1];
-    uint32_t r0 = li0, r1 = (b0 & 4u) ? li0 : li1;  // li1 shares li0's run when linked
-    troot16_2(par, r0, r1);
+  auto count_and_point = [&](uint32_t li0, uint32_t b0, uint32_t b1, uint32_t r0, uint32_t r1) {
+    const uint32_t li1 = li0 + 1u;
     // the first lane's cluster (often most of the wave) is summed by ballots
     // and added by one atomic
     const uint32_t R = __builtin_amdgcn_readfirstlane(r0);
@@ -309,9 +318,31 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
     }
     // every site now points straight at its root; a root's word becomes
     // root | rep << 16 with rep = "none" until an edge slot claims it (the
-    // walks above read parents through a 16-bit mask)
+    // walks read parents through a 16-bit mask)
     par[li0] = r0 != li0 ? r0 : li0 | (NONE16 << 16);
     par[li1] = r1 != li1 ? r1 : li1 | (NONE16 << 16);
+  };
+  if constexpr (ITERS % 2 == 0) {
+    // two pairs per step: four root walks in flight together
+#pragma unroll 1
+    for (int it = 0; it < ITERS; it += 2) {
+      const uint32_t la = (uint32_t)(it * NTH + tid) << 1, lb = la + 2u * NTH;
+      const uint32_t ba0 = bits[la], ba1 = bits[la + 1u], bb0 = bits[lb], bb1 = bits[lb + 1u];
+      // li1 shares li0's run when the in-row bond is occupied
+      uint32_t r[4] = {la, (ba0 & 4u) ? la : la + 1u, lb, (bb0 & 4u) ? lb : lb + 1u};
+      troot16_4(par, r);
+      count_and_point(la, ba0, ba1, r[0], r[1]);
+      count_and_point(lb, bb0, bb1, r[2], r[3]);
+    }
+  } else {
+#pragma unroll 1
+    for (int it = 0; it < ITERS; ++it) {
+      const uint32_t li0 = (uint32_t)(it * NTH + tid) << 1, li1 = li0 + 1u;
+      const uint32_t b0 = bits[li0], b1 = bits[li1];
+      uint32_t r0 = li0, r1 = (b0 & 4u) ? li0 : li1;  // li1 shares li0's run when linked
+      troot16_2(par, r0, r1);
+      count_and_point(li0, b0, b1, r0, r1);
+    }
   }
   __syncthreads();
   // rep slot of every edge cluster: par[root] = root | min slot << 16
