@@ -471,26 +471,50 @@ __device__ __forceinline__ uint32_t kpar(uint64_t v) { return (uint32_t)v; }
 __device__ __forceinline__ int kwx(uint64_t v) { return (int)(int16_t)(uint16_t)(v >> 32); }
 __device__ __forceinline__ int kwy(uint64_t v) { return (int)(int16_t)(uint16_t)(v >> 48); }
 
-// Root of x in the slot forest with the winding of the path x -> root; path
-// halving (stores only touch non-root words, which hooks never target).
-__device__ __forceinline__ uint32_t kfind(uint64_t* uf, uint32_t x, int& wx, int& wy) {
-  uint64_t vx = uf[x];
+// Roots of a and b in the slot forest with the windings of the paths to them,
+// walked together (both chains' loads in flight at once) with path halving;
+// stores only touch non-root words, which hooks never target.
+__device__ __forceinline__ void kfind2(uint64_t* uf, uint32_t& a, int& ax, int& ay, uint32_t& b,
+                                       int& bx, int& by) {
+  uint64_t va = uf[a], vb = uf[b];
   while (true) {
-    const uint32_t p = kpar(vx);
-    if (p == x) return x;
-    const uint64_t vp = uf[p];
-    const uint32_t gp = kpar(vp);
-    if (gp == p) {
-      wx += kwx(vx);
-      wy += kwy(vx);
-      return p;
+    const uint32_t pa = kpar(va), pb = kpar(vb);
+    const bool da = pa == a, db = pb == b;
+    if (da && db) return;
+    const uint64_t wa = da ? va : uf[pa];
+    const uint64_t wb = db ? vb : uf[pb];
+    if (!da) {
+      const uint32_t ga = kpar(wa);
+      if (ga == pa) {
+        ax += kwx(va);
+        ay += kwy(va);
+        a = pa;
+        va = wa;
+      } else {
+        const int hx = kwx(va) + kwx(wa), hy = kwy(va) + kwy(wa);
+        uf[a] = kpack(ga, hx, hy);
+        ax += hx;
+        ay += hy;
+        a = ga;
+        va = uf[ga];
+      }
     }
-    const int hx = kwx(vx) + kwx(vp), hy = kwy(vx) + kwy(vp);
-    uf[x] = kpack(gp, hx, hy);
-    wx += hx;
-    wy += hy;
-    x = gp;
-    vx = uf[gp];
+    if (!db) {
+      const uint32_t gb = kpar(wb);
+      if (gb == pb) {
+        bx += kwx(vb);
+        by += kwy(vb);
+        b = pb;
+        vb = wb;
+      } else {
+        const int hx = kwx(vb) + kwx(wb), hy = kwy(vb) + kwy(wb);
+        uf[b] = kpack(gb, hx, hy);
+        bx += hx;
+        by += hy;
+        b = gb;
+        vb = uf[gb];
+      }
+    }
   }
 }
 
@@ -508,8 +532,7 @@ __device__ __forceinline__ uint32_t kroot(const uint64_t* uf, uint32_t x) {
 __device__ uint32_t kunion(uint64_t* uf, uint32_t a, uint32_t b, int cx, int cy) {
   int ax = 0, ay = 0, bx = 0, by = 0;
   while (true) {
-    a = kfind(uf, a, ax, ay);
-    b = kfind(uf, b, bx, by);
+    kfind2(uf, a, ax, ay, b, bx, by);
     if (a == b) return ((cx + bx - ax) != 0 ? 1u : 0u) | ((cy + by - ay) != 0 ? 2u : 0u);
     if (a < b) {
       if (atomicCAS((unsigned long long*)&uf[b], (unsigned long long)kpack(b, 0, 0),
@@ -619,13 +642,15 @@ __global__ __launch_bounds__(KB_THREADS) void kb_merge(LatticeSet s, FusedScratc
   }
   if (wraps) atomicOr(&wrapbits, wraps);
   __syncthreads();
-  // reps fold into their roots
+  // reps fold into their roots and are pointed straight at them (the forest is
+  // settled and its windings are no longer read)
   for (uint32_t k = tid; k < ns; k += KB_THREADS) {
     if (mn[k] == 0xFFFFFFFFu) continue;
     const uint32_t r = kroot(uf, k);
     if (r != k) {
       atomicAdd((unsigned long long*)&cnt[r], (unsigned long long)cnt[k]);
       atomicMin(&mn[r], mn[k]);
+      uf[k] = kpack(r, 0, 0);
     }
   }
   __syncthreads();
@@ -714,13 +739,7 @@ __global__ __launch_bounds__(256) void kg_union(LatticeSet s, FusedScratch f) {
   uint64_t* uf = f.guf + (size_t)lat * ns;
   int ax = 0, ay = 0, bx = 0, by = 0;
   while (true) {
-    int dax = 0, day = 0, dbx = 0, dby = 0;
-    a = gfind(uf, a, dax, day);
-    c = gfind(uf, c, dbx, dby);
-    ax += dax;
-    ay += day;
-    bx += dbx;
-    by += dby;
+    gfind2(uf, a, ax, ay, c, bx, by);
     if (a == c) {
       const uint32_t w = ((cx + bx - ax) != 0 ? 1u : 0u) | ((cy + by - ay) != 0 ? 2u : 0u);
       if (w) atomicOr(&f.wrapw[lat], w);

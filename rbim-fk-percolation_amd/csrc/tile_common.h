@@ -25,28 +25,51 @@ __device__ __forceinline__ void gstore(uint64_t* p, uint64_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Root and tree-path winding in the HBM forest while hooks are in flight (K2):
-// agent-scope loads so a retry sees other workgroups' hooks; path halving.
-__device__ uint32_t gfind(uint64_t* uf, uint32_t x, int& wx, int& wy) {
-  wx = 0;
-  wy = 0;
-  uint64_t v = gload(&uf[x]);
+// Roots of a and b in the HBM forest while hooks are in flight, walked
+// together (both chains' loads in flight at once): agent-scope loads so a
+// retry sees other workgroups' hooks; path halving; the windings of the walked
+// paths are added to (ax, ay) and (bx, by).
+__device__ __forceinline__ void gfind2(uint64_t* uf, uint32_t& a, int& ax, int& ay, uint32_t& b,
+                                       int& bx, int& by) {
+  uint64_t va = gload(&uf[a]), vb = gload(&uf[b]);
   while (true) {
-    const uint32_t p = gpar(v);
-    if (p == x) return x;
-    const uint64_t vp = gload(&uf[p]);
-    const uint32_t gp = gpar(vp);
-    if (gp == p) {
-      wx += gwx(v);
-      wy += gwy(v);
-      return p;
+    const uint32_t pa = gpar(va), pb = gpar(vb);
+    const bool da = pa == a, db = pb == b;
+    if (da && db) return;
+    const uint64_t wa = da ? va : gload(&uf[pa]);
+    const uint64_t wb = db ? vb : gload(&uf[pb]);
+    if (!da) {
+      const uint32_t ga = gpar(wa);
+      if (ga == pa) {
+        ax += gwx(va);
+        ay += gwy(va);
+        a = pa;
+        va = wa;
+      } else {
+        const int hx = gwx(va) + gwx(wa), hy = gwy(va) + gwy(wa);
+        gstore(&uf[a], gpack(ga, hx, hy));
+        ax += hx;
+        ay += hy;
+        a = ga;
+        va = gload(&uf[ga]);
+      }
     }
-    const int hx = gwx(v) + gwx(vp), hy = gwy(v) + gwy(vp);
-    gstore(&uf[x], gpack(gp, hx, hy));
-    wx += hx;
-    wy += hy;
-    x = gp;
-    v = gload(&uf[gp]);
+    if (!db) {
+      const uint32_t gb = gpar(wb);
+      if (gb == pb) {
+        bx += gwx(vb);
+        by += gwy(vb);
+        b = pb;
+        vb = wb;
+      } else {
+        const int hx = gwx(vb) + gwx(wb), hy = gwy(vb) + gwy(wb);
+        gstore(&uf[b], gpack(gb, hx, hy));
+        bx += hx;
+        by += hy;
+        b = gb;
+        vb = gload(&uf[gb]);
+      }
+    }
   }
 }
 

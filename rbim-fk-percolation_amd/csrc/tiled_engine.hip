@@ -416,10 +416,12 @@ __global__ void k2_merge(LatticeSet s, TiledScratch t) {
   const uint32_t bxs = horiz ? (x == L - 1 ? 0u : (uint32_t)x + 1u) : (uint32_t)x;
   const uint32_t bys = horiz ? (uint32_t)y : (y == L - 1 ? 0u : (uint32_t)y + 1u);
   const uint32_t la = local_root(a, (uint32_t)x, (uint32_t)y), lb = local_root(b, bxs, bys);
+  // after a failed hook the walks go on from the stale roots: the windings
+  // carried so far stay valid (positions inside a tree never change)
+  uint32_t ra = la, rb = lb;
+  int ax = 0, ay = 0, bx = 0, by = 0;
   while (true) {
-    int ax, ay, bx, by;
-    const uint32_t ra = gfind(uf, la, ax, ay);
-    const uint32_t rb = gfind(uf, lb, bx, by);
+    gfind2(uf, ra, ax, ay, rb, bx, by);
     if (ra == rb) {
       const uint32_t wbits = ((cx + bx - ax) != 0 ? 1u : 0u) | ((cy + by - ay) != 0 ? 2u : 0u);
       if (wbits) atomicOr(&t.wrapf[2 * lat + (s.sweep_ctr[lat] & 1)], wbits);
