@@ -18,7 +18,8 @@ import numpy as np
 
 from rbim import _hip
 
-__all__ = ["Batch", "LatticeSpec", "moments_to_primitives", "engine_seed", "PRIMITIVE_SLOTS"]
+__all__ = ["Batch", "LatticeSpec", "moments_to_primitives", "moments_to_primitive_arrays",
+           "engine_seed", "PRIMITIVE_SLOTS"]
 
 PRIMITIVES = (
     "energy", "energy_sq", "magnetization", "abs_magnetization", "magnetization_sq",
@@ -63,28 +64,73 @@ def _slot_value(row: np.ndarray, slot: int, kind: str):
     return float(row[slot:slot + 1].view(np.float64)[0])
 
 
+_EXACT = float(2**53)  # integers below this convert to float64 exactly
+
+
+def moments_to_primitive_arrays(acc: np.ndarray, measure_percolation: bool = True
+                                ) -> Tuple[np.ndarray, np.ndarray]:
+    """Full-run and per-bin primitive means of many lattices at once.
+
+    ``acc`` is (n, n_bins, 20) uint64; returns ``(prims, bins)`` of shapes
+    (n, 14) and (n, n_bins, 14), columns in PRIMITIVES order.  Integer sums are
+    exact, so each mean is the correctly rounded quotient of exact integers:
+    float64 division of exactly converted operands while the sums stay below
+    2^53, Python integer division beyond.  The one FP64 slot is summed over the
+    bins in bin order.  (The reference's ``np.mean`` over a float series,
+    simulation.py:238-241, agrees to ~1e-16.)
+    """
+    acc = np.asarray(acc, dtype=np.uint64)
+    n, n_bins = acc.shape[0], acc.shape[1]
+    counts = acc[:, :, COUNT_SLOT].astype(np.int64)
+    if np.any(counts <= 0):
+        raise ZeroDivisionError("a bin holds no measured sweep")
+    total = counts.sum(axis=1)
+    prims = np.empty((n, len(PRIMITIVES)))
+    bins = np.empty((n, n_bins, len(PRIMITIVES)))
+    for j, name in enumerate(PRIMITIVES):
+        slot, kind = PRIMITIVE_SLOTS[name]
+        if not measure_percolation and name.startswith("wrap"):
+            prims[:, j] = np.nan
+            bins[:, :, j] = np.nan
+            continue
+        if kind == "d":
+            v = np.ascontiguousarray(acc[:, :, slot]).view(np.float64)
+            bins[:, :, j] = v / counts
+            s = np.zeros(n)
+            for b in range(n_bins):  # in bin order, as a Python sum of the bins
+                s = s + v[:, b]
+            prims[:, j] = s / total
+            continue
+        if kind == "i":
+            v = np.ascontiguousarray(acc[:, :, slot]).view(np.int64)
+            fast = np.abs(v).max(axis=1) < _EXACT
+            sums = v.sum(axis=1)  # |v| < 2^53 per bin: no int64 overflow
+        else:
+            v = acc[:, :, slot]
+            fast = v.max(axis=1) < _EXACT
+            if kind == "w":
+                fast &= acc[:, :, slot + 1].max(axis=1) == 0
+            sums = v.sum(axis=1)  # bins below 2^53: no uint64 overflow
+        fast &= np.abs(sums.astype(np.float64)) < _EXACT
+        bins[fast, :, j] = v[fast].astype(np.float64) / counts[fast]
+        prims[fast, j] = sums[fast].astype(np.float64) / total[fast]
+        for i in np.nonzero(~fast)[0]:
+            per_bin = [_slot_value(acc[i, b], slot, kind) for b in range(n_bins)]
+            bins[i, :, j] = [x / int(c) for x, c in zip(per_bin, counts[i])]
+            prims[i, j] = sum(per_bin) / int(total[i])
+    return prims, bins
+
+
 def moments_to_primitives(acc: np.ndarray, measure_percolation: bool = True
                           ) -> Tuple[Dict[str, float], Dict[str, np.ndarray]]:
     """Full-run and per-bin primitive means from one lattice's bin sums.
 
-    ``acc`` is (n_bins, 20) uint64.  Integer sums are exact, so each mean is the
-    correctly rounded quotient of exact integers (the reference's ``np.mean``
-    over a float series, simulation.py:238-241, agrees to ~1e-16).
+    ``acc`` is (n_bins, 20) uint64; the values are those of
+    :func:`moments_to_primitive_arrays`.
     """
-    n_bins = acc.shape[0]
-    counts = [int(acc[b, COUNT_SLOT]) for b in range(n_bins)]
-    total = sum(counts)
-    primitives: Dict[str, float] = {}
-    bins: Dict[str, np.ndarray] = {}
-    for name in PRIMITIVES:
-        slot, kind = PRIMITIVE_SLOTS[name]
-        if not measure_percolation and name.startswith("wrap"):
-            primitives[name] = float("nan")
-            bins[name] = np.full(n_bins, np.nan)
-            continue
-        per_bin = [_slot_value(acc[b], slot, kind) for b in range(n_bins)]
-        bins[name] = np.array([v / c for v, c in zip(per_bin, counts)], dtype=np.float64)
-        primitives[name] = float(sum(per_bin) / total)
+    p, b = moments_to_primitive_arrays(np.asarray(acc)[None], measure_percolation)
+    primitives = {name: float(p[0, j]) for j, name in enumerate(PRIMITIVES)}
+    bins = {name: b[0, :, j].copy() for j, name in enumerate(PRIMITIVES)}
     return primitives, bins
 
 

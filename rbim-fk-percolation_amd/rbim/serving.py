@@ -36,9 +36,9 @@ from rbim.disorder import generate_disorder
 from rbim.engine import Batch, engine_seed, footprint
 from rbim.simulation import (
     METADATA_CONVENTIONS,
+    _arrays_from_batch,
+    _assemble_arrays,
     _check_bins,
-    _results_from_batch,
-    assemble_results,
     scan_lattices,
 )
 
@@ -243,9 +243,9 @@ class ScanServer:
             act.remaining -= n
             return False
         req = act.ticket.request
-        raw = _results_from_batch(act.batch, req.measure_percolation)
+        prims, bins = _arrays_from_batch(act.batch, req.measure_percolation)
         act.batch.close()
-        act.ticket._result = assemble_results(req.L, req.p, act.temps, raw, act.R, act.metadata)
+        act.ticket._result = _assemble_arrays(req.L, act.temps, prims, bins, act.R, act.metadata)
         act.ticket.finished = time.perf_counter()
         act.ticket._event.set()
         return True

@@ -28,7 +28,7 @@ import numpy as np
 
 from rbim import _hip
 from rbim.disorder import generate_disorder
-from rbim.engine import Batch, LatticeSpec, engine_seed, moments_to_primitives
+from rbim.engine import Batch, LatticeSpec, engine_seed, moments_to_primitive_arrays
 from rbim.lattice import split_couplings
 
 __all__ = [
@@ -103,22 +103,22 @@ class RealizationResult:
     bin_primitives: Dict[str, np.ndarray] = field(default_factory=dict)
 
 
-def derive_observables(primitives: Dict[str, float], temperature: float, n_sites: int
-                       ) -> Dict[str, float]:
-    """Physical observables from (disorder-averaged) primitive moments
-    (simulation.py:108-145): C = var(E)/(T^2 N), chi = var(|M|)/(T N),
-    Binder = 1 - m4 / (3 m2^2) (0 when m2 = 0), P_inf = S_max / N, ..."""
+def _derive(P, temperature, n_sites: int):
+    """derive_observables on floats or on numpy arrays of any common shape
+    (elementwise; squares are written as products)."""
     N = n_sites
-    P = primitives
     m2 = P["magnetization_sq"] / N**2
     m4 = P["magnetization_4"] / N**4
+    with np.errstate(divide="ignore", invalid="ignore"):
+        binder = np.where(m2 > 0.0, 1.0 - m4 / (3.0 * (m2 * m2)), 0.0)
     return {
         "energy": P["energy"] / N,
-        "specific_heat": (P["energy_sq"] - P["energy"] ** 2) / (temperature**2 * N),
+        "specific_heat": (P["energy_sq"] - P["energy"] * P["energy"]) / (temperature * temperature * N),
         "magnetization": P["magnetization"] / N,
         "abs_magnetization": P["abs_magnetization"] / N,
-        "susceptibility": (P["magnetization_sq"] - P["abs_magnetization"] ** 2) / (temperature * N),
-        "binder": 1.0 - m4 / (3.0 * m2**2) if m2 > 0.0 else 0.0,
+        "susceptibility": (P["magnetization_sq"] - P["abs_magnetization"] * P["abs_magnetization"])
+        / (temperature * N),
+        "binder": binder,
         "p_infinity": P["largest_size"] / N,
         "largest_size": P["largest_size"],
         "largest_abs_magnetization": P["largest_abs_mag"],
@@ -130,6 +130,15 @@ def derive_observables(primitives: Dict[str, float], temperature: float, n_sites
         "wrap_y": P["wrap_y"],
         "wrap_any": P["wrap_any"],
     }
+
+
+def derive_observables(primitives: Dict[str, float], temperature: float, n_sites: int
+                       ) -> Dict[str, float]:
+    """Physical observables from (disorder-averaged) primitive moments
+    (simulation.py:108-145): C = var(E)/(T^2 N), chi = var(|M|)/(T N),
+    Binder = 1 - m4 / (3 m2^2) (0 when m2 = 0), P_inf = S_max / N, ..."""
+    P = {k: np.float64(primitives[k]) for k in PRIMITIVES}
+    return {k: float(v) for k, v in _derive(P, np.float64(temperature), n_sites).items()}
 
 
 def _check_bins(n_sweeps: int, n_bins: int) -> int:
@@ -144,13 +153,16 @@ def _check_bins(n_sweeps: int, n_bins: int) -> int:
     return per_bin
 
 
+def _arrays_from_batch(batch: Batch, measure_percolation: bool):
+    """(prims (n, 14), bins (n, n_bins, 14)) of every lattice of the batch."""
+    return moments_to_primitive_arrays(batch.moments(), measure_percolation)
+
+
 def _results_from_batch(batch: Batch, measure_percolation: bool) -> List[RealizationResult]:
-    acc = batch.moments()
-    out = []
-    for i in range(batch.n):
-        prims, bins = moments_to_primitives(acc[i], measure_percolation)
-        out.append(RealizationResult(primitives=prims, bin_primitives=bins))
-    return out
+    prims, bins = _arrays_from_batch(batch, measure_percolation)
+    return [RealizationResult(primitives={k: float(prims[i, j]) for j, k in enumerate(PRIMITIVES)},
+                              bin_primitives={k: bins[i, :, j].copy() for j, k in enumerate(PRIMITIVES)})
+            for i in range(batch.n)]
 
 
 def run_single_realization(
@@ -185,52 +197,64 @@ def run_single_realization(
         return _results_from_batch(batch, measure_percolation)[0]
 
 
+def _combine_arrays(prims: np.ndarray, bins: np.ndarray, temperatures: np.ndarray,
+                    n_sites: int) -> Dict[str, Dict[str, np.ndarray]]:
+    """Disorder average, jackknife over realizations and thermal error from
+    bins (simulation.py:261-333) for every temperature at once.
+
+    ``prims`` is (n_T, R, 14) and ``bins`` (n_T, R, n_bins, 14) of primitive
+    means in PRIMITIVES order; returns name -> {"value", "error",
+    "thermal_error"} arrays of length n_T.  Every reduction runs along a
+    contiguous last axis, i.e. as numpy reduces one temperature's vector.
+    """
+    nT, R = prims.shape[0], prims.shape[1]
+    n_bins = bins.shape[2]
+    T = np.asarray(temperatures, dtype=np.float64)
+    table = {k: np.ascontiguousarray(prims[:, :, j]) for j, k in enumerate(PRIMITIVES)}
+    central = _derive({k: v.mean(axis=-1) for k, v in table.items()}, T, n_sites)
+    out = {name: {"value": np.asarray(central[name], dtype=np.float64) * np.ones(nT)}
+           for name in OBSERVABLES}
+    if R > 1:
+        sums = {k: v.sum(axis=-1) for k, v in table.items()}
+        reduced = {k: (sums[k][:, None] - table[k]) / (R - 1) for k in PRIMITIVES}
+        loo = _derive(reduced, T[:, None], n_sites)
+        for name in OBSERVABLES:
+            x = np.asarray(loo[name], dtype=np.float64) * np.ones((nT, R))
+            spread = x - x.mean(axis=-1)[:, None]
+            out[name]["error"] = np.sqrt((R - 1) / R * np.sum(spread**2, axis=-1))
+    else:
+        for name in OBSERVABLES:
+            out[name]["error"] = np.full(nT, np.nan)
+    if n_bins > 1:
+        per_bin = _derive({k: np.ascontiguousarray(bins[:, :, :, j]) for j, k in enumerate(PRIMITIVES)},
+                          T[:, None, None], n_sites)
+        for name in OBSERVABLES:
+            x = np.asarray(per_bin[name], dtype=np.float64) * np.ones((nT, R, n_bins))
+            spread = np.std(x, axis=-1, ddof=1) / np.sqrt(n_bins)
+            out[name]["thermal_error"] = np.sqrt(np.mean(spread**2, axis=-1) / R)
+    else:
+        for name in OBSERVABLES:
+            out[name]["thermal_error"] = np.full(nT, np.nan)
+    if R == 1:
+        for name in OBSERVABLES:
+            out[name]["error"] = out[name]["thermal_error"].copy()
+    return out
+
+
+def _stack(results: Sequence[RealizationResult]):
+    prims = np.array([[r.primitives[k] for k in PRIMITIVES] for r in results], dtype=np.float64)
+    bins = np.stack([np.stack([np.asarray(r.bin_primitives[k], dtype=np.float64) for k in PRIMITIVES],
+                              axis=-1) for r in results])
+    return prims, bins
+
+
 def _combine_realizations(results: Sequence[RealizationResult], temperature: float,
                           n_sites: int) -> Dict[str, Dict[str, float]]:
     """Disorder average, jackknife over realizations, thermal error from bins
-    (simulation.py:261-333)."""
-    R = len(results)
-    table = {name: np.array([r.primitives[name] for r in results], dtype=float) for name in PRIMITIVES}
-    central = derive_observables({k: float(v.mean()) for k, v in table.items()}, temperature, n_sites)
-    combined: Dict[str, Dict[str, float]] = {}
-    if R > 1:
-        sums = {k: v.sum() for k, v in table.items()}
-        loo = {name: np.empty(R) for name in OBSERVABLES}
-        for i in range(R):
-            reduced = {k: (sums[k] - table[k][i]) / (R - 1) for k in PRIMITIVES}
-            derived = derive_observables(reduced, temperature, n_sites)
-            for name in OBSERVABLES:
-                loo[name][i] = derived[name]
-        for name in OBSERVABLES:
-            spread = loo[name] - loo[name].mean()
-            combined[name] = {"value": central[name],
-                              "error": float(np.sqrt((R - 1) / R * float(np.sum(spread**2))))}
-    else:
-        for name in OBSERVABLES:
-            combined[name] = {"value": central[name], "error": float("nan")}
-
-    n_bins = len(next(iter(results[0].bin_primitives.values())))
-    if n_bins > 1:
-        per_real = {name: [] for name in OBSERVABLES}
-        for r in results:
-            per_bin = {name: np.empty(n_bins) for name in OBSERVABLES}
-            for b in range(n_bins):
-                derived = derive_observables({k: float(r.bin_primitives[k][b]) for k in PRIMITIVES},
-                                             temperature, n_sites)
-                for name in OBSERVABLES:
-                    per_bin[name][b] = derived[name]
-            for name in OBSERVABLES:
-                per_real[name].append(np.std(per_bin[name], ddof=1) / np.sqrt(n_bins))
-        for name in OBSERVABLES:
-            spread = np.asarray(per_real[name], dtype=float)
-            combined[name]["thermal_error"] = float(np.sqrt(np.mean(spread**2) / R))
-    else:
-        for name in OBSERVABLES:
-            combined[name]["thermal_error"] = float("nan")
-    if R == 1:
-        for name in OBSERVABLES:
-            combined[name]["error"] = combined[name]["thermal_error"]
-    return combined
+    (simulation.py:261-333) at one temperature."""
+    prims, bins = _stack(results)
+    c = _combine_arrays(prims[None], bins[None], np.array([temperature]), n_sites)
+    return {name: {key: float(v[0]) for key, v in entry.items()} for name, entry in c.items()}
 
 
 def scan_lattices(L: int, realizations: Sequence[tuple], temperatures: Sequence[float],
@@ -246,18 +270,22 @@ def scan_lattices(L: int, realizations: Sequence[tuple], temperatures: Sequence[
     return specs
 
 
+def _assemble_arrays(L: int, temperatures: Sequence[float], prims: np.ndarray, bins: np.ndarray,
+                     R: int, metadata: Dict[str, object]) -> Dict[str, object]:
+    """The results dict from the primitive means of a scan's lattices in task
+    order (lattice k = t * R + r)."""
+    nT = len(temperatures)
+    c = _combine_arrays(prims.reshape(nT, R, -1), bins.reshape(nT, R, bins.shape[1], -1),
+                        np.asarray(temperatures, dtype=np.float64), L * L)
+    observables = {name: {key: np.array(c[name][key], dtype=np.float64)
+                          for key in ("value", "error", "thermal_error")} for name in OBSERVABLES}
+    return {"temperatures": np.asarray(temperatures), "observables": observables, "metadata": metadata}
+
+
 def assemble_results(L: int, p: float, temperatures: Sequence[float], raw: Sequence[RealizationResult],
                      R: int, metadata: Dict[str, object]) -> Dict[str, object]:
-    n_sites = L * L
-    nT = len(temperatures)
-    observables = {name: {"value": np.empty(nT), "error": np.empty(nT), "thermal_error": np.empty(nT)}
-                   for name in OBSERVABLES}
-    for t, temperature in enumerate(temperatures):
-        combined = _combine_realizations(raw[t * R:(t + 1) * R], temperature, n_sites)
-        for name in OBSERVABLES:
-            for key in ("value", "error", "thermal_error"):
-                observables[name][key][t] = combined[name][key]
-    return {"temperatures": np.asarray(temperatures), "observables": observables, "metadata": metadata}
+    prims, bins = _stack(raw)
+    return _assemble_arrays(L, temperatures, prims, bins, R, metadata)
 
 
 def run_temperature_scan(
@@ -301,7 +329,7 @@ def run_temperature_scan(
     with Batch(specs, n_therm=n_therm, n_bins=n_bins, sweeps_per_bin=per_bin, start=start,
                engine=engine) as batch:
         batch.run()
-        raw = _results_from_batch(batch, measure_percolation)
+        prims, bins = _arrays_from_batch(batch, measure_percolation)
     metadata = {
         "L": L,
         "p": p,
@@ -316,7 +344,7 @@ def run_temperature_scan(
         "engine": "rbim-hip gfx950",
         "rng": "Philox4x32-10 keyed by (seed, lattice, sweep, site pair)",
     }
-    return assemble_results(L, p, temperatures, raw, R, metadata)
+    return _assemble_arrays(L, temperatures, prims, bins, R, metadata)
 
 
 def save_results(path: str, results: Dict[str, object]) -> None:
