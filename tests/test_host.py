@@ -21,12 +21,18 @@ from rbim.disorder import (
     load_disorder,
     save_disorder,
 )
-from rbim.engine import PRIMITIVE_SLOTS, engine_seed, moments_to_primitives
+from rbim.engine import (
+    PRIMITIVE_SLOTS,
+    engine_seed,
+    moments_to_primitive_arrays,
+    moments_to_primitives,
+)
 from rbim.lattice import build_lattice, couplings_from_disorder, site_index, split_couplings
 from rbim.simulation import (
     OBSERVABLES,
     PRIMITIVES,
     RealizationResult,
+    _assemble_arrays,
     _combine_realizations,
     derive_observables,
     load_results,
@@ -223,6 +229,56 @@ def test_moments_to_primitives_exact_means():
     assert set(prims) == set(PRIMITIVES) == set(PRIMITIVE_SLOTS)
     nopc, _ = moments_to_primitives(acc, measure_percolation=False)
     assert np.isnan(nopc["wrap_x"]) and np.isnan(nopc["wrap_any"])
+
+
+def test_moment_arrays_exact_beyond_float_range():
+    """Batch decoding: sums below 2^53 take the float path, larger ones (here
+    M^4 sums past 2^64) the exact integer path; every mean is the correctly
+    rounded quotient, i.e. Python's int / int, lattice by lattice."""
+    rng = np.random.default_rng(5)
+    n, nb = 3, 4
+    acc = np.zeros((n, nb, 20), np.uint64)
+    big = []
+    for i in range(n):
+        for b in range(nb):
+            acc[i, b, 17] = 1 + b
+            acc[i, b, 0] = np.array([int(rng.integers(-2**40, 2**40))], np.int64).view(np.uint64)[0]
+            m4 = int(rng.integers(1, 2**62)) * (3**20 if i == 1 else 1)  # lattice 1 past 2^64
+            acc[i, b, 7], acc[i, b, 8] = m4 & (2**64 - 1), m4 >> 64
+            big.append(m4)
+    prims, bins = moments_to_primitive_arrays(acc)
+    j4 = PRIMITIVES.index("magnetization_4")
+    for i in range(n):
+        m4s = big[i * nb:(i + 1) * nb]
+        assert prims[i, j4] == sum(m4s) / sum(range(1, nb + 1))
+        assert list(bins[i, :, j4]) == [m / (b + 1) for b, m in enumerate(m4s)]
+        one_p, one_b = moments_to_primitives(acc[i])
+        for j, name in enumerate(PRIMITIVES):
+            assert one_p[name] == prims[i, j] and np.array_equal(one_b[name], bins[i, :, j])
+    assert prims.shape == (n, len(PRIMITIVES)) and bins.shape == (n, nb, len(PRIMITIVES))
+    with pytest.raises(ZeroDivisionError):
+        moments_to_primitive_arrays(np.zeros((1, 2, 20), np.uint64))
+
+
+def test_scan_estimators_equal_per_temperature_combination():
+    """The whole-scan estimator (all temperatures at once) gives, at every
+    temperature, what the one-temperature combination of the same
+    realizations gives, for R = 1 (error := thermal error) and R > 1."""
+    rng = np.random.default_rng(9)
+    for R, nb in ((1, 5), (6, 1), (5, 8)):
+        temps = [1.5, 2.0, 2.7]
+        prims = rng.normal(size=(len(temps) * R, len(PRIMITIVES))) + 3.0
+        bins = rng.normal(size=(len(temps) * R, nb, len(PRIMITIVES))) + 3.0
+        res = _assemble_arrays(8, temps, prims, bins, R, {})
+        for t, T in enumerate(temps):
+            raw = [RealizationResult(primitives={k: prims[t * R + r, j] for j, k in enumerate(PRIMITIVES)},
+                                     bin_primitives={k: bins[t * R + r, :, j] for j, k in enumerate(PRIMITIVES)})
+                   for r in range(R)]
+            one = _combine_realizations(raw, T, 64)
+            for name in OBSERVABLES:
+                for key in ("value", "error", "thermal_error"):
+                    a, b = res["observables"][name][key][t], one[name][key]
+                    assert a == b or (np.isnan(a) and np.isnan(b)), (R, nb, name, key, a, b)
 
 
 def test_engine_seed_is_deterministic():
