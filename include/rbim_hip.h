@@ -112,9 +112,11 @@ typedef struct {
   int32_t sweeps_per_launch;  /* LDS kernel sweeps per launch / tiled sweeps per
                                  graph; 0 -> default                              */
   int32_t threads;            /* LDS kernel workgroup size (64..1024); 0 -> default */
-  int32_t streams;            /* streams a tiled (L > 128) set of lattices is split
-                                 over so its latency-bound kernels overlap; every
-                                 distinct L also gets its own stream; 0 -> default */
+  int32_t streams;            /* streams each set of lattices of one L is split
+                                 over (every distinct L also gets its own stream);
+                                 0 -> default: 2 for tiled sets (L > 128), and for
+                                 LDS sets that fill the GPU more than once with a
+                                 partial last round; else 1                       */
   int32_t series;             /* > 0: keep the per-sweep stats of the last `series`
                                  measured sweeps of every lattice in a device ring
                                  (time series for autocorrelation / equilibration

@@ -703,11 +703,21 @@ int rbim_batch_create(rbim_ctx* ctx, const rbim_batch_desc* d, rbim_batch** out)
     }
     b->groups[gid[key]].idx.push_back(i);
   }
-  // split tiled groups into contiguous chunks, one stream each
+  // split groups into contiguous chunks, one stream each: tiled groups in two
+  // (ALU- and latency-bound kernels of the halves interleave); LDS groups in
+  // two when they fill the GPU more than once and the last round would run
+  // part-empty, so the chunks' launches refill the freed CUs
   {
+    int n_cu = 0;
+    (void)hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device);
     std::vector<Group> split;
     for (Group& g : b->groups) {
-      const int want = g.lds ? 1 : (d->streams > 0 ? d->streams : kTiledStreams);
+      int want = d->streams > 0 ? d->streams : kTiledStreams;
+      if (g.lds && d->streams <= 0) {
+        const long cap = (long)lds_blocks_per_cu(g.L, g.threads) * n_cu;
+        const long n = (long)g.idx.size();
+        want = cap > 0 && n > cap && n % cap != 0 ? 2 : 1;
+      }
       const int parts = std::max(1, std::min(want, (int)g.idx.size()));
       const size_t n = g.idx.size();
       for (int k = 0; k < parts; ++k) {

@@ -52,6 +52,8 @@ struct TiledScratch {
 hipError_t launch_init_spins(const LatticeSet& s, int start, hipStream_t st);
 hipError_t launch_lds_sweeps(const LatticeSet& s, int threads, int sweeps, hipStream_t st);
 size_t lds_bytes(int N);
+// Workgroups of the LDS kernel one CU holds at once (0 if unknown).
+int lds_blocks_per_cu(int L, int threads);
 // Two-kernel sweep for lattices of several full tiles whose tile-edge slots fit
 // one workgroup's LDS (fused_engine.hip); device pointers, [n_lat][N] unless noted.
 #define RBIM_FUSED_MAX_SLOTS 6400  // slots merged in one workgroup's LDS

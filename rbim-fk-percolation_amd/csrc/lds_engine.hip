@@ -632,6 +632,32 @@ hipError_t launch_lds_sweeps(const LatticeSet& s, int threads, int sweeps, hipSt
   }
 }
 
+template <int NT>
+int lds_blocks_per_cu_t(int L) {
+  const int N = L * L;
+  int n = 0;
+  hipError_t e;
+  switch (L) {
+    case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lds_sweep_kernel<NT, 16>, NT, lds_bytes(N)); break;
+    case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lds_sweep_kernel<NT, 32>, NT, lds_bytes(N)); break;
+    case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lds_sweep_kernel<NT, 64>, NT, lds_bytes(N)); break;
+    case 128: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lds_sweep_kernel<NT, 128>, NT, lds_bytes(N)); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lds_sweep_kernel<NT, 0>, NT, lds_bytes(N)); break;
+  }
+  return e == hipSuccess ? n : 0;
+}
+
+int lds_blocks_per_cu(int L, int threads) {
+  switch (threads) {
+    case 64: return lds_blocks_per_cu_t<64>(L);
+    case 128: return lds_blocks_per_cu_t<128>(L);
+    case 256: return lds_blocks_per_cu_t<256>(L);
+    case 512: return lds_blocks_per_cu_t<512>(L);
+    case 1024: return lds_blocks_per_cu_t<1024>(L);
+    default: return 0;
+  }
+}
+
 hipError_t launch_init_spins(const LatticeSet& s, int start, hipStream_t st) {
   dim3 grid((s.N + 255) / 256, s.n_lat);
   hipLaunchKernelGGL(init_spins_kernel, grid, dim3(256), 0, st, s, start);
