@@ -239,6 +239,32 @@ def test_batched_equals_plain_and_is_launch_shape_invariant(hip):
         assert np.array_equal(alone.moments()[0], out[0][0][2])
 
 
+def test_lds_sets_split_over_streams_are_invariant(hip):
+    """An LDS set split over streams -- explicitly, or by default when it fills
+    the GPU more than once with a part-empty last round (1100 lattices at L=64,
+    about 512 resident at a time) -- gives the moments and spins of the
+    unsplit set: every lattice is keyed by its own id."""
+    from rbim.engine import Batch, LatticeSpec
+
+    def run(specs, streams):
+        with Batch(specs, n_therm=3, n_bins=2, sweeps_per_bin=4, engine=BATCHED,
+                   streams=streams) as batch:
+            batch.run()
+            return batch.moments(), batch.spins()
+
+    small = _specs((16, 32), seed=3, temps=(1.8, 2.3, 2.9, 3.4))
+    ref_acc, ref_sp = run(small, 1)
+    acc, sp = run(small, 3)
+    assert np.array_equal(acc, ref_acc) and all(np.array_equal(a, b) for a, b in zip(sp, ref_sp))
+    rng = np.random.default_rng(4)
+    J_h, J_v = random_disorder(rng, 64, 0.1)
+    many = [LatticeSpec(L=64, J_h=J_h, J_v=J_v, beta=1.0 / (1.5 + 1.5 * k / 1100), seed=11,
+                        lattice_id=k) for k in range(1100)]
+    ref_acc, ref_sp = run(many, 1)
+    acc, sp = run(many, 0)
+    assert np.array_equal(acc, ref_acc) and all(np.array_equal(a, b) for a, b in zip(sp, ref_sp))
+
+
 def test_run_past_end_is_refused(hip):
     from rbim.engine import Batch
     from rbim._hip import EngineError
