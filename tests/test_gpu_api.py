@@ -171,6 +171,24 @@ def test_binning_does_not_change_central_values():
                 assert d[name] == pytest.approx(value, rel=1e-12, abs=1e-12)
 
 
+def test_percolation_off_leaves_the_rest_unchanged():
+    """measure_percolation=False (simulation.py:229-236): the wrap observables
+    are NaN; every other observable is exactly what the same seed gives with
+    the measurement on (the flag changes no draw)."""
+    temps = [1.6, 2.3, 3.0]
+    on = run_temperature_scan(12, 0.1, temps, n_realizations=3, n_sweeps=200, n_therm=50,
+                              n_bins=4, seed=21)
+    off = run_temperature_scan(12, 0.1, temps, n_realizations=3, n_sweeps=200, n_therm=50,
+                               n_bins=4, seed=21, measure_percolation=False)
+    assert off["metadata"]["measure_percolation"] is False
+    for name, entry in off["observables"].items():
+        for key in ("value", "error", "thermal_error"):
+            if name.startswith("wrap"):
+                assert np.all(np.isnan(entry[key])), (name, key)
+            else:
+                assert np.array_equal(entry[key], on["observables"][name][key]), (name, key)
+
+
 @pytest.mark.slow
 def test_energy_stays_within_bounds():
     for T in (1.0, TC_EXACT, 5.0):
