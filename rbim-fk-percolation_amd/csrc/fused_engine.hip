@@ -549,6 +549,37 @@ __device__ uint32_t kunion(uint64_t* uf, uint32_t a, uint32_t b, int cx, int cy)
 // Fold measured sweep gm of lattice lat into its bin: E, M from part_em,
 // interior clusters from part_int[gm & 1], edge clusters and wraps from
 // bst[gm & 1].  Single thread.
+// The sweep record of measured sweep gm from its totals (E, M, cluster count,
+// the two largest clusters, the wrap word), folded into its bin.  Single
+// thread.
+__device__ void fused_fold(const LatticeSet& s, int lat, int64_t gm, int64_t e, int64_t m,
+                           int64_t ncl, const TileMeas& a, int64_t wrapw) {
+  rbim_sweep_stats st;
+  st.energy = e;
+  st.magnetization = m;
+  st.n_clusters = ncl;
+  const int64_t s1 = (int64_t)(a.k1 >> 32), mc1 = 2 * (int64_t)a.p1 - s1;
+  st.largest_size = s1;
+  st.largest_abs_mag = mc1 < 0 ? -mc1 : mc1;
+  if (a.k2 != 0ull) {
+    const int64_t s2 = (int64_t)(a.k2 >> 32), mc2 = 2 * (int64_t)a.p2 - s2;
+    st.second_size = s2;
+    st.second_abs_mag = mc2 < 0 ? -mc2 : mc2;
+  } else {
+    st.second_size = 0;
+    st.second_abs_mag = 0;
+  }
+  st.wrap_x = (int32_t)(wrapw & 1);
+  st.wrap_y = (int32_t)((wrapw >> 1) & 1);
+  const int64_t bin = (gm - s.n_therm) / s.spb;
+  acc_sweep(s.acc + ((size_t)lat * s.n_bins + bin) * RBIM_ACC_SLOTS, st);
+  if (s.last) s.last[lat] = st;
+  series_put(s, lat, gm, st);
+}
+
+// Fold measured sweep gm of lattice lat into its bin: E, M from part_em,
+// interior clusters from part_int[gm & 1], edge clusters and wraps from
+// bst[gm & 1].  Single thread.
 __device__ void fused_finalize(const LatticeSet& s, const FusedScratch& f, int lat, int64_t gm) {
   const int tiles = f.tiles_x * f.tiles_x;
   const int64_t* em = f.part_em + (size_t)lat * tiles * 2;
@@ -565,27 +596,45 @@ __device__ void fused_finalize(const LatticeSet& s, const FusedScratch& f, int l
     top2_insert(a, (uint64_t)pi[5 * i + 1], (uint32_t)pi[5 * i + 3]);
     top2_insert(a, (uint64_t)pi[5 * i + 2], (uint32_t)pi[5 * i + 4]);
   }
-  rbim_sweep_stats st;
-  st.energy = e;
-  st.magnetization = m;
-  st.n_clusters = ncl;
-  const int64_t s1 = (int64_t)(a.k1 >> 32), mc1 = 2 * (int64_t)a.p1 - s1;
-  st.largest_size = s1;
-  st.largest_abs_mag = mc1 < 0 ? -mc1 : mc1;
-  if (a.k2 != 0ull) {
-    const int64_t s2 = (int64_t)(a.k2 >> 32), mc2 = 2 * (int64_t)a.p2 - s2;
-    st.second_size = s2;
-    st.second_abs_mag = mc2 < 0 ? -mc2 : mc2;
-  } else {
-    st.second_size = 0;
-    st.second_abs_mag = 0;
+  fused_fold(s, lat, gm, e, m, ncl, a, eb[5]);
+}
+
+// Sum of a workgroup's (e, m, n, top-2) partials: wave shuffles, then thread
+// 0 combines the waves' rows of red (NW <= 16).
+template <int NTHR>
+__device__ void block_meas(int64_t& e, int64_t& m, int64_t& n, TileMeas& a, int64_t (*red)[PARTS]) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    e += __shfl_xor(e, off);
+    m += __shfl_xor(m, off);
+    n += __shfl_xor(n, off);
+    const uint64_t o1 = __shfl_xor(a.k1, off), o2 = __shfl_xor(a.k2, off);
+    const uint32_t q1 = __shfl_xor(a.p1, off), q2 = __shfl_xor(a.p2, off);
+    top2_insert(a, o1, q1);
+    top2_insert(a, o2, q2);
   }
-  st.wrap_x = (int32_t)(eb[5] & 1);
-  st.wrap_y = (int32_t)((eb[5] >> 1) & 1);
-  const int64_t bin = (gm - s.n_therm) / s.spb;
-  acc_sweep(s.acc + ((size_t)lat * s.n_bins + bin) * RBIM_ACC_SLOTS, st);
-  if (s.last) s.last[lat] = st;
-  series_put(s, lat, gm, st);
+  if ((threadIdx.x & 63) == 0) {
+    int64_t* r = red[threadIdx.x >> 6];
+    r[0] = e;
+    r[1] = m;
+    r[2] = n;
+    r[3] = (int64_t)a.k1;
+    r[4] = (int64_t)a.k2;
+    r[5] = a.p1;
+    r[6] = a.p2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    e = m = n = 0;
+    a = {0, 0, 0, 0ull, 0ull, 0u, 0u};
+    for (int w = 0; w < NTHR / 64; ++w) {
+      e += red[w][0];
+      m += red[w][1];
+      n += red[w][2];
+      top2_insert(a, (uint64_t)red[w][3], (uint32_t)red[w][5]);
+      top2_insert(a, (uint64_t)red[w][4], (uint32_t)red[w][6]);
+    }
+  }
 }
 
 constexpr int KB_THREADS = 1024;
@@ -824,36 +873,69 @@ __global__ __launch_bounds__(256) void kg_flip(LatticeSet s, FusedScratch f) {
   }
 }
 
-// Per lattice: the merged clusters' partials and the wrap bits of sweep g,
-// then the same bookkeeping as the end of kb_merge.
-__global__ void kg_sweep_end(LatticeSet s, FusedScratch f, int nblocks) {
-  const int lat = blockIdx.x * blockDim.x + threadIdx.x;
-  if (lat >= s.n_lat || !lattice_active(s, lat)) return;
+// Per lattice (one workgroup): the merged clusters' partials and the wrap
+// bits of sweep g, then the same bookkeeping as the end of kb_merge.  The
+// per-block and per-tile partials are summed by all threads.
+constexpr int KG_END_THREADS = 256;
+
+__global__ __launch_bounds__(KG_END_THREADS) void kg_sweep_end(LatticeSet s, FusedScratch f,
+                                                               int nblocks) {
+  __shared__ int64_t red[KG_END_THREADS / 64][PARTS];
+  const int lat = blockIdx.x;
+  if (!lattice_active(s, lat)) return;
+  const int tid = threadIdx.x;
   const int64_t g = s.sweep_ctr[lat];
   if (g >= s.n_therm) {
+    int64_t e = 0, m = 0, ncl = 0;
     TileMeas a = {0, 0, 0, 0ull, 0ull, 0u, 0u};
-    int64_t ncl = 0;
     const int64_t* p = f.gpart + (size_t)lat * nblocks * 5;
-    for (int i = 0; i < nblocks; ++i) {
+    for (int i = tid; i < nblocks; i += KG_END_THREADS) {
       ncl += p[5 * i];
       top2_insert(a, (uint64_t)p[5 * i + 1], (uint32_t)p[5 * i + 3]);
       top2_insert(a, (uint64_t)p[5 * i + 2], (uint32_t)p[5 * i + 4]);
     }
-    int64_t* o = f.bst + ((size_t)(g & 1) * s.n_lat + lat) * 6;
-    o[0] = ncl;
-    o[1] = (int64_t)a.k1;
-    o[2] = (int64_t)a.k2;
-    o[3] = a.p1;
-    o[4] = a.p2;
-    o[5] = f.wrapw[lat];
+    block_meas<KG_END_THREADS>(e, m, ncl, a, red);
+    if (tid == 0) {
+      int64_t* o = f.bst + ((size_t)(g & 1) * s.n_lat + lat) * 6;
+      o[0] = ncl;
+      o[1] = (int64_t)a.k1;
+      o[2] = (int64_t)a.k2;
+      o[3] = a.p1;
+      o[4] = a.p2;
+      o[5] = f.wrapw[lat];
+    }
+    __syncthreads();  // red is reused below
   }
-  f.wrapw[lat] = 0u;
-  if (g >= 1 && f.fin[lat] == g - 1) {
-    if (g - 1 >= s.n_therm) fused_finalize(s, f, lat, g - 1);
-    f.fin[lat] = g;
+  const bool fold = g >= 1 && f.fin[lat] == g - 1;
+  if (fold && g - 1 >= s.n_therm) {
+    const int64_t gm = g - 1;
+    const int tiles = f.tiles_x * f.tiles_x;
+    const int64_t* em = f.part_em + (size_t)lat * tiles * 2;
+    const int64_t* pi = f.part_int + ((size_t)(gm & 1) * s.n_lat + lat) * tiles * 5;
+    int64_t e = 0, m = 0, ncl = 0;
+    TileMeas a = {0, 0, 0, 0ull, 0ull, 0u, 0u};
+    for (int i = tid; i < tiles; i += KG_END_THREADS) {
+      e += em[2 * i];
+      m += em[2 * i + 1];
+      ncl += pi[5 * i];
+      top2_insert(a, (uint64_t)pi[5 * i + 1], (uint32_t)pi[5 * i + 3]);
+      top2_insert(a, (uint64_t)pi[5 * i + 2], (uint32_t)pi[5 * i + 4]);
+    }
+    block_meas<KG_END_THREADS>(e, m, ncl, a, red);
+    if (tid == 0) {
+      const int64_t* eb = f.bst + ((size_t)(gm & 1) * s.n_lat + lat) * 6;
+      top2_insert(a, (uint64_t)eb[1], (uint32_t)eb[3]);
+      top2_insert(a, (uint64_t)eb[2], (uint32_t)eb[4]);
+      fused_fold(s, lat, gm, e, m, ncl + eb[0], a, eb[5]);
+    }
   }
-  f.cur[lat] ^= 1;
-  s.sweep_ctr[lat] = g + 1;
+  __syncthreads();  // every thread has read the counters it updates
+  if (tid == 0) {
+    f.wrapw[lat] = 0u;
+    if (fold) f.fin[lat] = g;
+    f.cur[lat] ^= 1;
+    s.sweep_ctr[lat] = g + 1;
+  }
 }
 
 // ------------------------------------------------------------ run closing
@@ -951,7 +1033,7 @@ hipError_t launch_fused_t(const LatticeSet& s, const FusedScratch& f, hipStream_
     hipLaunchKernelGGL((kg_union<T>), dim3((f.nslots / 2 + 255) / 256, s.n_lat), dim3(256), 0, st, s, f);
     hipLaunchKernelGGL(kg_fold, dim3(nb, s.n_lat), dim3(256), 0, st, s, f);
     hipLaunchKernelGGL(kg_flip, dim3(nb, s.n_lat), dim3(256), 0, st, s, f);
-    hipLaunchKernelGGL(kg_sweep_end, dim3((s.n_lat + 63) / 64), dim3(64), 0, st, s, f, nb);
+    hipLaunchKernelGGL(kg_sweep_end, dim3(s.n_lat), dim3(KG_END_THREADS), 0, st, s, f, nb);
     return hipGetLastError();
   }
   static std::atomic<uint64_t> configured{0};
