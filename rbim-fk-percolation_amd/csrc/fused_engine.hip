@@ -202,23 +202,29 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
   const bool meas_em = g >= 1 && f.fin[lat] == g - 1 && g - 1 >= s.n_therm;
   const bool meas_cl = g >= s.n_therm;
 
+  // the tile's own spins, labels and couplings are fetched first, so their
+  // loads are in flight while the flip bits are staged
+  uint32_t sp2[ITERS], lb2[ITERS], j2[ITERS];
+#pragma unroll
+  for (int it = 0; it < ITERS; ++it) {
+    const uint32_t ql = (uint32_t)(it * NTH + tid);
+    const uint32_t lx = ql / (T / 2), ly0 = (ql % (T / 2)) * 2u;
+    const uint32_t n0 = (x0 + lx) * L + y0 + ly0;  // even: the pair loads are aligned
+    sp2[it] = *reinterpret_cast<const uint16_t*>(v.sin + n0);
+    lb2[it] = *reinterpret_cast<const uint32_t*>(v.lin + n0);
+    j2[it] = *reinterpret_cast<const uint16_t*>(jb + n0);
+  }
   const uint32_t* fb = flipbits_of<BIG>(f, lat, fb_lds, NTH);
   __syncthreads();
   load_halo<T>(s, f, v, fb, tx, ty, hsp);
   // the tile's own spins (pending flips applied) and couplings into LDS
 #pragma unroll
   for (int it = 0; it < ITERS; ++it) {
-    const uint32_t ql = (uint32_t)(it * NTH + tid);
-    const uint32_t lx = ql / (T / 2), ly0 = (ql % (T / 2)) * 2u;
-    const uint32_t li0 = ql << 1;
-    const uint32_t n0 = (x0 + lx) * L + y0 + ly0;  // even: the pair loads are aligned
-    const uint32_t sp2 = *reinterpret_cast<const uint16_t*>(v.sin + n0);
-    const uint32_t lb2 = *reinterpret_cast<const uint32_t*>(v.lin + n0);
-    const uint32_t j2 = *reinterpret_cast<const uint16_t*>(jb + n0);
-    const uint32_t t0 = true_spin<T>((int8_t)(sp2 & 0xFFu), lb2 & 0xFFFFu, tile, fb);
-    const uint32_t t1 = true_spin<T>((int8_t)(sp2 >> 8), lb2 >> 16, tile, fb);
-    bits[li0] = (uint8_t)(t0 | (j2 & 3u) << 4);
-    bits[li0 + 1u] = (uint8_t)(t1 | ((j2 >> 8) & 3u) << 4);
+    const uint32_t li0 = (uint32_t)(it * NTH + tid) << 1;
+    const uint32_t t0 = true_spin<T>((int8_t)(sp2[it] & 0xFFu), lb2[it] & 0xFFFFu, tile, fb);
+    const uint32_t t1 = true_spin<T>((int8_t)(sp2[it] >> 8), lb2[it] >> 16, tile, fb);
+    bits[li0] = (uint8_t)(t0 | (j2[it] & 3u) << 4);
+    bits[li0 + 1u] = (uint8_t)(t1 | ((j2[it] >> 8) & 3u) << 4);
   }
   __syncthreads();
 
