@@ -643,6 +643,32 @@ __device__ void block_meas(int64_t& e, int64_t& m, int64_t& n, TileMeas& a, int6
   }
 }
 
+// fused_finalize with its per-tile sums spread over a workgroup of NTHR
+// threads (every thread calls it; red is overwritten).
+template <int NTHR>
+__device__ void fused_finalize_block(const LatticeSet& s, const FusedScratch& f, int lat,
+                                     int64_t gm, int64_t (*red)[PARTS]) {
+  const int tiles = f.tiles_x * f.tiles_x;
+  const int64_t* em = f.part_em + (size_t)lat * tiles * 2;
+  const int64_t* pi = f.part_int + ((size_t)(gm & 1) * s.n_lat + lat) * tiles * 5;
+  int64_t e = 0, m = 0, ncl = 0;
+  TileMeas a = {0, 0, 0, 0ull, 0ull, 0u, 0u};
+  for (int i = (int)threadIdx.x; i < tiles; i += NTHR) {
+    e += em[2 * i];
+    m += em[2 * i + 1];
+    ncl += pi[5 * i];
+    top2_insert(a, (uint64_t)pi[5 * i + 1], (uint32_t)pi[5 * i + 3]);
+    top2_insert(a, (uint64_t)pi[5 * i + 2], (uint32_t)pi[5 * i + 4]);
+  }
+  block_meas<NTHR>(e, m, ncl, a, red);
+  if (threadIdx.x == 0) {
+    const int64_t* eb = f.bst + ((size_t)(gm & 1) * s.n_lat + lat) * 6;
+    top2_insert(a, (uint64_t)eb[1], (uint32_t)eb[3]);
+    top2_insert(a, (uint64_t)eb[2], (uint32_t)eb[4]);
+    fused_fold(s, lat, gm, e, m, ncl + eb[0], a, eb[5]);
+  }
+}
+
 constexpr int KB_THREADS = 1024;
 
 template <int T>
@@ -753,11 +779,16 @@ __global__ __launch_bounds__(KB_THREADS) void kb_merge(LatticeSet s, FusedScratc
       o[4] = a.p2;
       o[5] = wrapbits;
     }
-    // sweep g-1 is complete now (its E, M came from this sweep's KA)
-    if (g >= 1 && f.fin[lat] == g - 1) {
-      if (g - 1 >= s.n_therm) fused_finalize(s, f, lat, g - 1);
-      f.fin[lat] = g;
-    }
+  }
+  // sweep g-1 is complete now (its E, M came from this sweep's KA)
+  const bool fold = g >= 1 && f.fin[lat] == g - 1;
+  if (fold && g - 1 >= s.n_therm) {
+    __syncthreads();  // thread 0 has read red
+    fused_finalize_block<KB_THREADS>(s, f, lat, g - 1, red);
+  }
+  __syncthreads();  // every thread has read the counters thread 0 updates
+  if (tid == 0) {
+    if (fold) f.fin[lat] = g;
     f.cur[lat] ^= 1;
     s.sweep_ctr[lat] = g + 1;
   }
@@ -913,28 +944,7 @@ __global__ __launch_bounds__(KG_END_THREADS) void kg_sweep_end(LatticeSet s, Fus
     __syncthreads();  // red is reused below
   }
   const bool fold = g >= 1 && f.fin[lat] == g - 1;
-  if (fold && g - 1 >= s.n_therm) {
-    const int64_t gm = g - 1;
-    const int tiles = f.tiles_x * f.tiles_x;
-    const int64_t* em = f.part_em + (size_t)lat * tiles * 2;
-    const int64_t* pi = f.part_int + ((size_t)(gm & 1) * s.n_lat + lat) * tiles * 5;
-    int64_t e = 0, m = 0, ncl = 0;
-    TileMeas a = {0, 0, 0, 0ull, 0ull, 0u, 0u};
-    for (int i = tid; i < tiles; i += KG_END_THREADS) {
-      e += em[2 * i];
-      m += em[2 * i + 1];
-      ncl += pi[5 * i];
-      top2_insert(a, (uint64_t)pi[5 * i + 1], (uint32_t)pi[5 * i + 3]);
-      top2_insert(a, (uint64_t)pi[5 * i + 2], (uint32_t)pi[5 * i + 4]);
-    }
-    block_meas<KG_END_THREADS>(e, m, ncl, a, red);
-    if (tid == 0) {
-      const int64_t* eb = f.bst + ((size_t)(gm & 1) * s.n_lat + lat) * 6;
-      top2_insert(a, (uint64_t)eb[1], (uint32_t)eb[3]);
-      top2_insert(a, (uint64_t)eb[2], (uint32_t)eb[4]);
-      fused_fold(s, lat, gm, e, m, ncl + eb[0], a, eb[5]);
-    }
-  }
+  if (fold && g - 1 >= s.n_therm) fused_finalize_block<KG_END_THREADS>(s, f, lat, g - 1, red);
   __syncthreads();  // every thread has read the counters it updates
   if (tid == 0) {
     f.wrapw[lat] = 0u;
