@@ -126,6 +126,31 @@ int default_threads(int L) {
   return 1024;
 }
 
+// Workgroup sizes of a batch's LDS groups (threads not given): the per-L
+// default, doubled -- fewer pairs per lane, more wavefronts in flight, up to
+// one pair per lane -- while the whole batch would give the SIMDs fewer than
+// four wavefronts each.  Lattices up to 32 x 32 only (beyond, the larger
+// workgroups measured no faster).
+void default_lds_threads(std::vector<Group>& groups, int n_cu) {
+  const long target = 16l * (n_cu > 0 ? n_cu : 256);  // wavefronts
+  long waves = 0;
+  for (Group& g : groups)
+    if (g.lds) {
+      g.threads = default_threads(g.L);
+      waves += (long)g.idx.size() * (g.threads / 64);
+    }
+  bool grew = true;
+  while (waves < target && grew) {
+    grew = false;
+    for (Group& g : groups)
+      if (g.lds && g.N <= 1024 && 2 * g.threads <= g.N / 2) {
+        waves += (long)g.idx.size() * (g.threads / 64);
+        g.threads *= 2;
+        grew = true;
+      }
+  }
+}
+
 int default_lds_sweeps(int L) {
   const int N = L * L;
   return N <= 1024 ? 2000 : N <= 4096 ? 500 : 200;
@@ -710,13 +735,14 @@ int rbim_batch_create(rbim_ctx* ctx, const rbim_batch_desc* d, rbim_batch** out)
     }
     b->groups[gid[key]].idx.push_back(i);
   }
+  int n_cu = 0;
+  (void)hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device);
+  if (d->threads == 0) default_lds_threads(b->groups, n_cu);
   // split groups into contiguous chunks, one stream each: tiled groups in two
   // (ALU- and latency-bound kernels of the halves interleave); LDS groups in
   // two when they fill the GPU more than once and the last round would run
   // part-empty, so the chunks' launches refill the freed CUs
   {
-    int n_cu = 0;
-    (void)hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device);
     std::vector<Group> split;
     for (Group& g : b->groups) {
       int want = d->streams > 0 ? d->streams : kTiledStreams;
