@@ -111,7 +111,9 @@ typedef struct {
   int32_t engine;             /* rbim_engine                                      */
   int32_t sweeps_per_launch;  /* LDS kernel sweeps per launch / tiled sweeps per
                                  graph; 0 -> default                              */
-  int32_t threads;            /* LDS kernel workgroup size (64..1024); 0 -> default */
+  int32_t threads;            /* LDS kernel workgroup size (64..1024); 0 -> default:
+                                 per L, doubled for L <= 32 while the batch would give
+                                 the GPU's SIMDs fewer than four wavefronts each    */
   int32_t streams;            /* streams each set of lattices of one L is split
                                  over (every distinct L also gets its own stream);
                                  0 -> default: 2 for tiled sets (L > 128), and for
