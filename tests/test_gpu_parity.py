@@ -239,6 +239,37 @@ def test_batched_equals_plain_and_is_launch_shape_invariant(hip):
         assert np.array_equal(alone.moments()[0], out[0][0][2])
 
 
+@pytest.mark.parametrize("engine", [PLAIN, BATCHED, 3])
+def test_random_shapes_bit_exact(hip, coracle, engine):
+    """A seeded draw of shapes the fixed lists above do not hold: 12 lattices
+    with L from 3 to 140 (odd, even, multiples of 32 mixed in one batch, L <= 128
+    for the LDS engine), p up to 0.2, T in [1, 4], cold and hot starts via the
+    batch; trajectories (spins, all 20 moment slots) equal the oracle's."""
+    from rbim.engine import Batch, LatticeSpec
+
+    rng = np.random.default_rng(20261203 + engine)
+    top = 128 if engine == BATCHED else 140
+    Ls = sorted({int(x) for x in rng.integers(3, top + 1, size=9)} | {32, 96 if top > 96 else 64})
+    specs = []
+    for k, L in enumerate(Ls):
+        J_h, J_v = random_disorder(rng, L, float(rng.uniform(0.0, 0.2)))
+        specs.append(LatticeSpec(L=L, J_h=J_h, J_v=J_v, beta=1.0 / float(rng.uniform(1.0, 4.0)),
+                                 seed=int(rng.integers(0, 2**63)), lattice_id=int(rng.integers(0, 2**31))))
+    n_therm, spb, n_bins = 4, 3, 3
+    total = n_therm + spb * n_bins
+    for start in ("hot", "cold"):
+        with Batch(specs, n_therm=n_therm, n_bins=n_bins, sweeps_per_bin=spb, engine=engine,
+                   start=start) as batch:
+            batch.run(5)
+            batch.run(total - 5)
+            acc = batch.moments()
+            spins = batch.spins()
+        for i, spec in enumerate(specs):
+            s_ref, acc_ref = _oracle_trajectory(coracle, spec, total, n_therm, spb, n_bins, start=start)
+            assert np.array_equal(spins[i], s_ref), (engine, start, spec.L)
+            assert np.array_equal(acc[i], acc_ref), (engine, start, spec.L)
+
+
 def test_lds_sets_split_over_streams_are_invariant(hip):
     """An LDS set split over streams -- explicitly, or by default when it fills
     the GPU more than once with a part-empty last round (1100 lattices at L=64,
