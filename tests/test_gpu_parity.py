@@ -241,10 +241,11 @@ def test_batched_equals_plain_and_is_launch_shape_invariant(hip):
 
 @pytest.mark.parametrize("engine", [PLAIN, BATCHED, 3])
 def test_random_shapes_bit_exact(hip, coracle, engine):
-    """A seeded draw of shapes the fixed lists above do not hold: 12 lattices
-    with L from 3 to 140 (odd, even, multiples of 32 mixed in one batch, L <= 128
-    for the LDS engine), p up to 0.2, T in [1, 4], cold and hot starts via the
-    batch; trajectories (spins, all 20 moment slots) equal the oracle's."""
+    """A seeded draw of shapes the fixed lists above do not hold: ten or so
+    lattices with L from 3 to 140 (odd, even and multiples of 32 mixed in one
+    batch; L <= 128 for the LDS engine), p up to 0.2, T in [1, 4], random seeds
+    and lattice ids, hot and cold starts; trajectories (spins, all 20 moment
+    slots) equal the oracle's."""
     from rbim.engine import Batch, LatticeSpec
 
     rng = np.random.default_rng(20261203 + engine)
