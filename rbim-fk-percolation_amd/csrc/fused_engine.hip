@@ -179,7 +179,6 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
   __shared__ uint32_t lcnt[T * T];
   __shared__ uint8_t bits[T * T];  // bit0 spin, bit1 occ_h, bit2 occ_v, bit3 flip draw, bit4/5 J<0
   __shared__ uint8_t hsp[2 * T];
-  __shared__ uint32_t fb_lds[fb_words<BIG>()];
   __shared__ int64_t red[NTH / 64][PARTS];
   const int lat = blockIdx.y;
   if (!lattice_active(s, lat)) return;
@@ -202,8 +201,21 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
   const bool meas_em = g >= 1 && f.fin[lat] == g - 1 && g - 1 >= s.n_therm;
   const bool meas_cl = g >= s.n_therm;
 
-  // the tile's own spins, labels and couplings are fetched first, so their
-  // loads are in flight while the flip bits are staged
+  // The state this sweep starts from: spins, labels and couplings of the
+  // tile, the halo sites (the column beyond x0+T-1 and the row beyond
+  // y0+T-1), and the flip words of the previous sweep's edge clusters that
+  // they can need -- WPT words each of this tile, the right and the upper
+  // neighbour tile, held by lanes 0 .. 3 WPT - 1 of every wavefront and read
+  // by shuffles.  All loads are issued together, with no barrier before them.
+  constexpr uint32_t WPT = 4u * T / 32u;
+  const uint32_t nt_r = ty * nt + (tx + 1u == nt ? 0u : tx + 1u);
+  const uint32_t nt_u = (ty + 1u == nt ? 0u : ty + 1u) * nt + tx;
+  uint32_t fw = 0u;
+  if ((uint32_t)lane < 3u * WPT) {
+    const uint32_t w = (uint32_t)lane % WPT;
+    const uint32_t src = lane < (int)WPT ? tile : lane < (int)(2u * WPT) ? nt_r : nt_u;
+    fw = f.flipbits[(size_t)lat * ((uint32_t)f.nslots / 32u) + src * WPT + w];
+  }
   uint32_t sp2[ITERS], lb2[ITERS], j2[ITERS];
 #pragma unroll
   for (int it = 0; it < ITERS; ++it) {
@@ -214,17 +226,36 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
     lb2[it] = *reinterpret_cast<const uint32_t*>(v.lin + n0);
     j2[it] = *reinterpret_cast<const uint16_t*>(jb + n0);
   }
-  const uint32_t* fb = flipbits_of<BIG>(f, lat, fb_lds, NTH);
-  __syncthreads();
-  load_halo<T>(s, f, v, fb, tx, ty, hsp);
-  // the tile's own spins (pending flips applied) and couplings into LDS
+  const bool halo = tid < 2 * T;  // tid < T: right column, else upper row
+  int8_t hs = 1;
+  uint32_t hl = 0u;
+  if (halo) {
+    const bool right = tid < T;
+    const uint32_t i = right ? (uint32_t)tid : (uint32_t)tid - T;
+    const uint32_t hx = right ? (x0 + T == L ? 0u : x0 + T) : x0 + i;
+    const uint32_t hy = right ? y0 + i : (y0 + T == L ? 0u : y0 + T);
+    hs = v.sin[hx * L + hy];
+    hl = v.lin[hx * L + hy];
+  }
+  // spin bit of a site, its edge cluster's pending flip applied (wbase: the
+  // first flip word of the site's tile); the shuffle runs on every lane
+  auto spin_bit = [&](int8_t sp, uint32_t lb, uint32_t wbase) -> uint32_t {
+    const uint32_t slot = lb & 0x7FFFu;
+    const uint32_t word = (uint32_t)__shfl((int)fw, (int)(wbase + (slot >> 5)));
+    const uint32_t flip = (lb & LAB_EDGE) ? (word >> (slot & 31u)) & 1u : 0u;
+    return (sp > 0 ? 1u : 0u) ^ flip;
+  };
 #pragma unroll
   for (int it = 0; it < ITERS; ++it) {
     const uint32_t li0 = (uint32_t)(it * NTH + tid) << 1;
-    const uint32_t t0 = true_spin<T>((int8_t)(sp2[it] & 0xFFu), lb2[it] & 0xFFFFu, tile, fb);
-    const uint32_t t1 = true_spin<T>((int8_t)(sp2[it] >> 8), lb2[it] >> 16, tile, fb);
+    const uint32_t t0 = spin_bit((int8_t)(sp2[it] & 0xFFu), lb2[it] & 0xFFFFu, 0u);
+    const uint32_t t1 = spin_bit((int8_t)(sp2[it] >> 8), lb2[it] >> 16, 0u);
     bits[li0] = (uint8_t)(t0 | (j2[it] & 3u) << 4);
     bits[li0 + 1u] = (uint8_t)(t1 | ((j2[it] >> 8) & 3u) << 4);
+  }
+  {
+    const uint32_t hb = spin_bit(hs, hl, tid < T ? WPT : 2u * WPT);
+    if (halo) hsp[tid] = (uint8_t)hb;
   }
   __syncthreads();
 
