@@ -675,7 +675,8 @@ __device__ void block_meas(int64_t& e, int64_t& m, int64_t& n, TileMeas& a, int6
 }
 
 // fused_finalize with its per-tile sums spread over a workgroup of NTHR
-// threads (every thread calls it; red is overwritten).
+// threads, for lattices of many tiles (every thread calls it; red is
+// overwritten).
 template <int NTHR>
 __device__ void fused_finalize_block(const LatticeSet& s, const FusedScratch& f, int lat,
                                      int64_t gm, int64_t (*red)[PARTS]) {
@@ -810,16 +811,12 @@ __global__ __launch_bounds__(KB_THREADS) void kb_merge(LatticeSet s, FusedScratc
       o[4] = a.p2;
       o[5] = wrapbits;
     }
-  }
-  // sweep g-1 is complete now (its E, M came from this sweep's KA)
-  const bool fold = g >= 1 && f.fin[lat] == g - 1;
-  if (fold && g - 1 >= s.n_therm) {
-    __syncthreads();  // thread 0 has read red
-    fused_finalize_block<KB_THREADS>(s, f, lat, g - 1, red);
-  }
-  __syncthreads();  // every thread has read the counters thread 0 updates
-  if (tid == 0) {
-    if (fold) f.fin[lat] = g;
+    // sweep g-1 is complete now (its E, M came from this sweep's KA); a few
+    // tiles per lattice here, so one thread folds them
+    if (g >= 1 && f.fin[lat] == g - 1) {
+      if (g - 1 >= s.n_therm) fused_finalize(s, f, lat, g - 1);
+      f.fin[lat] = g;
+    }
     f.cur[lat] ^= 1;
     s.sweep_ctr[lat] = g + 1;
   }
