@@ -298,9 +298,9 @@ def main():
         print(json.dumps(measure_workload(args.workload, args.steps)), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: every box in this build has one MI355X "
-              f"(asked for --gpus {args.gpus}); independent lattices shard across GPUs "
-              f"with no communication, so the N-GPU figure is N x the 1-GPU one.", file=sys.stderr)
+        print(f"bench.py measures one GPU (asked for --gpus {args.gpus}): the lattices are "
+              f"independent and shard across GPUs with no communication, so run one bench "
+              f"per GPU; an N-GPU figure is N x the 1-GPU one.", file=sys.stderr)
         sys.exit(2)
 
     main_pt = measure_point(args.L, args, args.steps, args.warmup, args.plain_steps, not args.no_cpu)
