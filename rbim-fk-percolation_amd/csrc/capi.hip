@@ -261,15 +261,18 @@ int group_alloc(rbim_ctx* ctx, Group& g, const std::vector<uint8_t>& jbits,
   return RBIM_OK;
 }
 
-// Queue `n` sweeps of one group on `stream`; counts kernel launches.
-int group_run(rbim_ctx* ctx, Group& g, int64_t n, hipStream_t stream, int64_t* launches) {
+// Queue the next launch of one group on `stream` -- one LDS dispatch or one
+// graph of sweeps (single sweeps below a graph's length) -- of the `n` sweeps
+// still to run; returns how many it queued and counts kernel launches.
+int group_chunk(rbim_ctx* ctx, Group& g, int64_t n, hipStream_t stream, int64_t* launches,
+                int64_t* queued) {
+  *queued = 0;
+  if (n <= 0) return RBIM_OK;
   if (g.lds) {
-    while (n > 0) {
-      const int k = (int)std::min<int64_t>(n, g.per_launch);
-      HIP_TRY(ctx, launch_lds_sweeps(g.set, g.threads, k, stream));
-      ++*launches;
-      n -= k;
-    }
+    const int k = (int)std::min<int64_t>(n, g.per_launch);
+    HIP_TRY(ctx, launch_lds_sweeps(g.set, g.threads, k, stream));
+    ++*launches;
+    *queued = k;
     return RBIM_OK;
   }
   // one sweep of a tiled group (both layouts)
@@ -294,18 +297,25 @@ int group_run(rbim_ctx* ctx, Group& g, int64_t n, hipStream_t stream, int64_t* l
       HIP_TRY(ctx, hipGraphInstantiate(&g.graph, graph, nullptr, nullptr, 0));
       HIP_TRY(ctx, hipGraphDestroy(graph));
     }
-    while (n >= S) {
-      HIP_TRY(ctx, hipGraphLaunch(g.graph, stream));
-      *launches += (int64_t)S * per_sweep;
-      n -= S;
-    }
+    HIP_TRY(ctx, hipGraphLaunch(g.graph, stream));
+    *launches += (int64_t)S * per_sweep;
+    *queued = S;
+    return RBIM_OK;
   }
-  for (int64_t i = 0; i < n; ++i) {
+  const int64_t k = S > 1 ? n : 1;
+  for (int64_t i = 0; i < k; ++i) {
     HIP_TRY(ctx, sweep(stream));
     *launches += per_sweep;
   }
-  // each sweep's measurement rides in the next sweep's kernels; the last one
-  // of the run is measured here so the moment sums (and spins) are complete
+  *queued = k;
+  return RBIM_OK;
+}
+
+// After the last sweep of a run of a tiled group: each sweep's measurement
+// rides in the next sweep's kernels, so the last one of the run is measured
+// here and the moment sums (and spins) are complete.
+int group_close(rbim_ctx* ctx, Group& g, hipStream_t stream, int64_t* launches) {
+  if (g.lds) return RBIM_OK;
   if (g.fused) {
     HIP_TRY(ctx, launch_fused_close(g.set, g.fz, stream));
     *launches += 3;
@@ -314,6 +324,17 @@ int group_run(rbim_ctx* ctx, Group& g, int64_t n, hipStream_t stream, int64_t* l
     *launches += 2;
   }
   return RBIM_OK;
+}
+
+// Queue `n` sweeps of one group on `stream`.
+int group_run(rbim_ctx* ctx, Group& g, int64_t n, hipStream_t stream, int64_t* launches) {
+  while (n > 0) {
+    int64_t q = 0;
+    const int rc = group_chunk(ctx, g, n, stream, launches, &q);
+    if (rc != RBIM_OK) return rc;
+    n -= q;
+  }
+  return group_close(ctx, g, stream, launches);
 }
 
 void group_free(Group& g) {
@@ -815,11 +836,27 @@ int rbim_batch_run(rbim_batch* b, int64_t n_sweeps) {
     if (rc != RBIM_OK) return rc;
   } else {
     // fork onto the group streams and join back, so the batch stream (what
-    // query, sync and the timing events see) completes after every group
+    // query, sync and the timing events see) completes after every group.
+    // The groups' launches are queued in turn, one launch each per round: a
+    // host that queued one group's whole run first would block on that
+    // stream's full queue before the next group had any work on the device.
     HIP_TRY(ctx, hipEventRecord(b->fork, b->stream));
+    for (Group& g : b->groups) HIP_TRY(ctx, hipStreamWaitEvent(g.stream, b->fork, 0));
+    std::vector<int64_t> left(b->groups.size(), n_sweeps);
+    for (bool more = n_sweeps > 0; more;) {
+      more = false;
+      for (size_t i = 0; i < b->groups.size(); ++i) {
+        if (left[i] <= 0) continue;
+        Group& g = b->groups[i];
+        int64_t q = 0;
+        const int rc = group_chunk(ctx, g, left[i], g.stream, &b->launches, &q);
+        if (rc != RBIM_OK) return rc;
+        left[i] -= q;
+        more = more || left[i] > 0;
+      }
+    }
     for (Group& g : b->groups) {
-      HIP_TRY(ctx, hipStreamWaitEvent(g.stream, b->fork, 0));
-      int rc = group_run(ctx, g, n_sweeps, g.stream, &b->launches);
+      const int rc = group_close(ctx, g, g.stream, &b->launches);
       if (rc != RBIM_OK) return rc;
       HIP_TRY(ctx, hipEventRecord(g.done, g.stream));
       HIP_TRY(ctx, hipStreamWaitEvent(b->stream, g.done, 0));
