@@ -145,7 +145,8 @@ __device__ __forceinline__ void load_halo(const LatticeSet& s, const FusedScratc
 // Roots of x and y in the settled tile forest, walked together, reading
 // parents through a 16-bit mask (a root's word may already carry its rep slot
 // above bit 16).
-__device__ __forceinline__ void troot16_2(const uint32_t* par, uint32_t& x, uint32_t& y) {
+[[maybe_unused]] __device__ __forceinline__ void troot16_2(const uint32_t* par, uint32_t& x,
+                                                          uint32_t& y) {
   uint32_t px = par[x] & 0xFFFFu, py = par[y] & 0xFFFFu;
   while (px != x || py != y) {
     x = px;

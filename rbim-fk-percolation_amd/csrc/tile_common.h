@@ -118,7 +118,7 @@ __device__ __forceinline__ void top2_insert(TileMeas& a, uint64_t k, uint32_t p)
 // part (read before the sweep overwrites the roots) and the spin part (summed
 // while the sweep reads the spins) need not be live at the same time: each
 // wave reduces into red[wave][.] and, after a barrier, thread 0 combines.
-__device__ void wave_clusters(TileMeas a, int64_t (*red)[PARTS]) {
+[[maybe_unused]] __device__ void wave_clusters(TileMeas a, int64_t (*red)[PARTS]) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     a.ncl += __shfl_xor(a.ncl, off);
@@ -137,7 +137,7 @@ __device__ void wave_clusters(TileMeas a, int64_t (*red)[PARTS]) {
   }
 }
 
-__device__ void wave_spins(int32_t e, int32_t m, int64_t (*red)[PARTS]) {
+[[maybe_unused]] __device__ void wave_spins(int32_t e, int32_t m, int64_t (*red)[PARTS]) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     e += __shfl_xor(e, off);
