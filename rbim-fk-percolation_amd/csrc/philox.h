@@ -41,6 +41,15 @@ RBIM_HD void rbim_mulhilo32(uint32_t a, uint32_t b, uint32_t* hi, uint32_t* lo) 
   *lo = (uint32_t)p;
 }
 
+// a ^ b ^ c; one v_bitop3_b32 (truth table 0x96) in device code
+RBIM_HD uint32_t rbim_xor3(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+  return a ^ b ^ c;
+#endif
+}
+
 RBIM_HD rbim_u32x4 rbim_philox4x32_10(rbim_u32x4 c, uint32_t k0, uint32_t k1) {
   const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
   const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
@@ -52,9 +61,14 @@ RBIM_HD rbim_u32x4 rbim_philox4x32_10(rbim_u32x4 c, uint32_t k0, uint32_t k1) {
     rbim_mulhilo32(M0, c.x, &hi0, &lo0);
     rbim_mulhilo32(M1, c.z, &hi1, &lo1);
     rbim_u32x4 n;
-    n.x = hi1 ^ c.y ^ k0;
+    // With a counter whose words 1-3 are uniform over a wavefront (sweep,
+    // lattice id, tag) two of the three operands are uniform up to round 1
+    // (n.x) and round 2 (n.z): there plain xors pair them on the scalar unit,
+    // leaving one vector xor, and keep round 0's n.x (and the product it
+    // feeds) scalar.
+    n.x = r < 2 ? hi1 ^ c.y ^ k0 : rbim_xor3(hi1, c.y, k0);
     n.y = lo1;
-    n.z = hi0 ^ c.w ^ k1;
+    n.z = r < 3 ? hi0 ^ c.w ^ k1 : rbim_xor3(hi0, c.w, k1);
     n.w = lo0;
     c = n;
     k0 += W0;
