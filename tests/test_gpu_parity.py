@@ -297,6 +297,30 @@ def test_lds_sets_split_over_streams_are_invariant(hip):
     assert np.array_equal(acc, ref_acc) and all(np.array_equal(a, b) for a, b in zip(sp, ref_sp))
 
 
+def test_tiled_sets_split_over_streams_are_invariant(hip):
+    """Tiled sets of two sizes (edge-slot sweep, and the multi-kernel sweep)
+    split over one, two (the default) or three streams: a run queues one
+    launch per group in turn -- graphs of three sweeps and single sweeps mixed,
+    runs ending inside a graph's length -- and every split gives the moments,
+    spins and per-sweep series of the unsplit set."""
+    from rbim._hip import ENGINE_TILED
+    from rbim.engine import Batch
+
+    specs = _specs((96, 160), seed=21, temps=(1.6, 2.3, 3.0))
+    for engine in (PLAIN, ENGINE_TILED):
+        out = []
+        for streams in (1, 0, 3):
+            with Batch(specs, n_therm=4, n_bins=2, sweeps_per_bin=5, engine=engine, streams=streams,
+                       sweeps_per_launch=3, series=10) as batch:
+                batch.run(4)
+                batch.run()
+                out.append((batch.moments(), batch.spins(), batch.series()))
+        for acc, spins, series in out[1:]:
+            assert np.array_equal(acc, out[0][0]), engine
+            assert np.array_equal(series, out[0][2]), engine
+            assert all(np.array_equal(a, b) for a, b in zip(spins, out[0][1])), engine
+
+
 def test_run_past_end_is_refused(hip):
     from rbim.engine import Batch
     from rbim._hip import EngineError
