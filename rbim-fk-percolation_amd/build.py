@@ -27,6 +27,11 @@ FLAGS = [
     "-Wall",
     # packed FP32 instructions stay off in all device code of this library
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # The kernels' atomics on a wavefront-uniform address are made by one lane
+    # (a cluster's first-lane count) or by the few lanes with a wrap bit; the
+    # compiler's wavefront-reduction rewrite of such atomics only adds
+    # instructions to them.
+    "-mllvm", "-amdgpu-atomic-optimizer-strategy=None",
 ]
 
 
