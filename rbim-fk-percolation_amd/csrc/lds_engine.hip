@@ -289,7 +289,11 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
   const bool given = s.given_occ != nullptr;
   uint64_t* gacc = s.acc + (size_t)lat * s.n_bins * RBIM_ACC_SLOTS;
 
-  int64_t cur_bin = -1, bin_end = 0;  // lane 0 of wave 0 only
+  // The fold of each measured sweep runs while the first 2L threads unite
+  // the boundary bonds (B2): on the first wavefront past them, else on the
+  // last one.  Its bin state lives in registers of that lane.
+  const int fold_tid = min((int)((2u * L + 63u) / 64u) * 64, NT - 64);
+  int64_t cur_bin = -1, bin_end = 0;  // lane fold_tid only
   __syncthreads();
 
   // Measurement bookkeeping for one pass (counts are wave-uniform).
@@ -320,7 +324,7 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
     m_sat = m_up = m_root = 0u;
     k1 = k2 = 0u;
   };
-  // Lane 0 of wave 0: fold the published partials of sweep gm into its bin.
+  // One lane: fold the published partials of sweep gm into its bin.
   auto finalize = [&](int64_t gm) {
     uint32_t sat = 0u, up = 0u, roots = 0u;
     KT f1 = 0u, f2 = 0u;
@@ -427,7 +431,6 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
     }
     if (pending) publish();
     __syncthreads();
-    if (pending && tid == 0) finalize(g - 1);
     // ----------------------------------------------------------- B1
     // Bonds that do not cross the periodic boundary: x bonds below row L-1 and
     // the in-row y bond that leaves the wave's chunk (lane 63).  The forest
@@ -494,6 +497,9 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
         wraps |= lds_union(uf, a, horiz ? t - L : t * L, horiz ? 1 : 0, horiz ? 0 : 1);
     }
     if (wraps) atomicOr(&wrapflag[g & 1], wraps);
+    // sweep g-1's partials were published before the barrier after A, and its
+    // wrap slot is not the one B2 of sweep g sets
+    if (pending && tid == fold_tid) finalize(g - 1);
     __syncthreads();
     // ------------------------------------------------------------ C
     // Counts go to cnt[root] by LDS atomics; the sites of the first lane's
@@ -574,14 +580,12 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
     }
     publish();
     __syncthreads();
-    if (tid == 0) finalize(gend - 1);
+    if (tid == fold_tid) finalize(gend - 1);
   }
 
-  if (tid == 0) {
-    if (cur_bin >= 0)
-      for (int k = 0; k < RBIM_ACC_SLOTS; ++k) gacc[cur_bin * RBIM_ACC_SLOTS + k] = a[k];
-    s.sweep_ctr[lat] = gend > g0 ? gend : g0;
-  }
+  if (tid == fold_tid && cur_bin >= 0)
+    for (int k = 0; k < RBIM_ACC_SLOTS; ++k) gacc[cur_bin * RBIM_ACC_SLOTS + k] = a[k];
+  if (tid == 0) s.sweep_ctr[lat] = gend > g0 ? gend : g0;
   for (uint32_t n = tid; n < N; n += NT) gsp[n] = (sb[n] & 1u) ? (int8_t)1 : (int8_t)-1;
 }
 
