@@ -703,6 +703,9 @@ __device__ void fused_finalize_block(const LatticeSet& s, const FusedScratch& f,
 }
 
 constexpr int KB_THREADS = 1024;
+// The last wavefront of a merge workgroup folds the previous sweep into its
+// bin (one thread, a few tiles per lattice) while the others merge.
+constexpr int KB_WORK = KB_THREADS - 64;
 
 template <int T>
 __global__ __launch_bounds__(KB_THREADS) void kb_merge(LatticeSet s, FusedScratch f) {
@@ -719,8 +722,15 @@ __global__ __launch_bounds__(KB_THREADS) void kb_merge(LatticeSet s, FusedScratc
   const int tid = threadIdx.x;
   const int64_t g = s.sweep_ctr[lat];
   const uint64_t* ent = f.slots + (size_t)lat * ns;
+  const bool worker = tid < KB_WORK;
   if (tid == 0) wrapbits = 0u;
-  for (uint32_t k = tid; k < ns; k += KB_THREADS) {
+  // sweep g-1 is complete (its E, M came from this sweep's KA; its interior
+  // clusters, edge clusters and wraps sit in the other parity's partials)
+  if (tid == KB_WORK && g >= 1 && f.fin[lat] == g - 1) {
+    if (g - 1 >= s.n_therm) fused_finalize(s, f, lat, g - 1);
+    f.fin[lat] = g;
+  }
+  for (uint32_t k = tid; worker && k < ns; k += KB_WORK) {
     const uint64_t e = ent[k];
     const uint32_t t = k / NS, loc = k % NS, rep = (uint32_t)(e & 0xFFFu);
     uf[k] = kpack(t * NS + rep, 0, 0);
@@ -740,7 +750,7 @@ __global__ __launch_bounds__(KB_THREADS) void kb_merge(LatticeSet s, FusedScratc
   // (side 3, i-T) into the upper neighbour's (side 2, i-T)
   const uint32_t nb = (ns / NS) * 2u * T;
   uint32_t wraps = 0u;
-  for (uint32_t b = tid; b < nb; b += KB_THREADS) {
+  for (uint32_t b = tid; worker && b < nb; b += KB_WORK) {
     const uint32_t t = b / (2u * T), i = b % (2u * T);
     const uint64_t word = f.xbits[((size_t)lat * (ns / NS) + t) * (2 * T / 64) + i / 64];
     if (!((word >> (i % 64)) & 1ull)) continue;
@@ -758,7 +768,7 @@ __global__ __launch_bounds__(KB_THREADS) void kb_merge(LatticeSet s, FusedScratc
   __syncthreads();
   // reps fold into their roots and are pointed straight at them (the forest is
   // settled and its windings are no longer read)
-  for (uint32_t k = tid; k < ns; k += KB_THREADS) {
+  for (uint32_t k = tid; worker && k < ns; k += KB_WORK) {
     if (mn[k] == 0xFFFFFFFFu) continue;
     const uint32_t r = kroot(uf, k);
     if (r != k) {
@@ -772,7 +782,7 @@ __global__ __launch_bounds__(KB_THREADS) void kb_merge(LatticeSet s, FusedScratc
   const bool meas_cl = g >= s.n_therm;
   TileMeas ms = {0, 0, 0, 0ull, 0ull, 0u, 0u};
   uint32_t* fbo = f.flipbits + (size_t)lat * (ns / 32u);
-  for (uint32_t k0 = 0; k0 < ns; k0 += KB_THREADS) {
+  for (uint32_t k0 = 0; worker && k0 < ns; k0 += KB_WORK) {
     const uint32_t k = k0 + tid;
     uint32_t fl = 0u;
     if (k < ns) {
@@ -811,12 +821,6 @@ __global__ __launch_bounds__(KB_THREADS) void kb_merge(LatticeSet s, FusedScratc
       o[3] = a.p1;
       o[4] = a.p2;
       o[5] = wrapbits;
-    }
-    // sweep g-1 is complete now (its E, M came from this sweep's KA); a few
-    // tiles per lattice here, so one thread folds them
-    if (g >= 1 && f.fin[lat] == g - 1) {
-      if (g - 1 >= s.n_therm) fused_finalize(s, f, lat, g - 1);
-      f.fin[lat] = g;
     }
     f.cur[lat] ^= 1;
     s.sweep_ctr[lat] = g + 1;
