@@ -764,7 +764,7 @@ __global__ __launch_bounds__(KB_THREADS) void kb_merge(LatticeSet s, FusedScratc
       wraps |= kunion(uf, t * NS + 3u * T + j, tu * NS + 2u * T + j, 0, ty + 1u == nt ? 1 : 0);
     }
   }
-  if (wraps) atomicOr(&wrapbits, wraps);
+  wave_or_wraps(&wrapbits, wraps);
   __syncthreads();
   // reps fold into their roots and are pointed straight at them (the forest is
   // settled and its windings are no longer read)
@@ -857,23 +857,24 @@ __global__ __launch_bounds__(256) void kg_union(LatticeSet s, FusedScratch f) {
   }
   uint64_t* uf = f.guf + (size_t)lat * ns;
   int ax = 0, ay = 0, bx = 0, by = 0;
+  uint32_t w = 0u;
   while (true) {
     gfind2(uf, a, ax, ay, c, bx, by);
     if (a == c) {
-      const uint32_t w = ((cx + bx - ax) != 0 ? 1u : 0u) | ((cy + by - ay) != 0 ? 2u : 0u);
-      if (w) atomicOr(&f.wrapw[lat], w);
-      return;
+      w = ((cx + bx - ax) != 0 ? 1u : 0u) | ((cy + by - ay) != 0 ? 2u : 0u);
+      break;
     }
     if (a < c) {
       if (atomicCAS((unsigned long long*)&uf[c], (unsigned long long)gpack(c, 0, 0),
                     (unsigned long long)gpack(a, ax - bx - cx, ay - by - cy)) == gpack(c, 0, 0))
-        return;
+        break;
     } else {
       if (atomicCAS((unsigned long long*)&uf[a], (unsigned long long)gpack(a, 0, 0),
                     (unsigned long long)gpack(c, bx - ax + cx, by - ay + cy)) == gpack(a, 0, 0))
-        return;
+        break;
     }
   }
+  wave_or_wraps(&f.wrapw[lat], w);
 }
 
 // Reps fold their counts and (min site << 1 | flip) into their roots.

@@ -496,7 +496,7 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
       if ((sb[a] >> (horiz ? 3 : 4)) & 1u)
         wraps |= lds_union(uf, a, horiz ? t - L : t * L, horiz ? 1 : 0, horiz ? 0 : 1);
     }
-    if (wraps) atomicOr(&wrapflag[g & 1], wraps);
+    wave_or_wraps(&wrapflag[g & 1], wraps);
     // sweep g-1's partials were published before the barrier after A, and its
     // wrap slot is not the one B2 of sweep g sets
     if (pending && tid == fold_tid) finalize(g - 1);

@@ -88,6 +88,15 @@ __device__ __forceinline__ uint32_t groot(uint64_t* gf, uint32_t x) {
 }
 
 
+// OR of the lanes' wrap bits (1: x, 2: y) into *flag, one atomic per
+// wavefront; every lane that calls it works on the lattice of that flag.
+template <typename P>
+__device__ __forceinline__ void wave_or_wraps(P* flag, uint32_t w) {
+  const uint64_t mx = __ballot((w & 1u) != 0u), my = __ballot((w & 2u) != 0u);
+  if ((mx | my) != 0ull && __lane_id() == (unsigned)(__ffsll((long long)__ballot(true)) - 1))
+    atomicOr(flag, (mx ? 1u : 0u) | (my ? 2u : 0u));
+}
+
 __device__ __forceinline__ bool lattice_active(const LatticeSet& s, int lat) {
   return s.sweep_ctr[lat] < s.total;
 }

@@ -420,23 +420,24 @@ __global__ void k2_merge(LatticeSet s, TiledScratch t) {
   // carried so far stay valid (positions inside a tree never change)
   uint32_t ra = la, rb = lb;
   int ax = 0, ay = 0, bx = 0, by = 0;
+  uint32_t wbits = 0u;
   while (true) {
     gfind2(uf, ra, ax, ay, rb, bx, by);
     if (ra == rb) {
-      const uint32_t wbits = ((cx + bx - ax) != 0 ? 1u : 0u) | ((cy + by - ay) != 0 ? 2u : 0u);
-      if (wbits) atomicOr(&t.wrapf[2 * lat + (s.sweep_ctr[lat] & 1)], wbits);
-      return;
+      wbits = ((cx + bx - ax) != 0 ? 1u : 0u) | ((cy + by - ay) != 0 ? 2u : 0u);
+      break;
     }
     if (ra < rb) {
       if (atomicCAS((unsigned long long*)&uf[rb], (unsigned long long)rb,
                     (unsigned long long)gpack(ra, ax - bx - cx, ay - by - cy)) == rb)
-        return;
+        break;
     } else {
       if (atomicCAS((unsigned long long*)&uf[ra], (unsigned long long)ra,
                     (unsigned long long)gpack(rb, bx - ax + cx, by - ay + cy)) == ra)
-        return;
+        break;
     }
   }
+  wave_or_wraps(&t.wrapf[2 * lat + (s.sweep_ctr[lat] & 1)], wbits);
 }
 
 // ----------------------------------------------------------------- K3
