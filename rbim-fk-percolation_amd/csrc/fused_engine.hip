@@ -882,15 +882,39 @@ __global__ __launch_bounds__(256) void kg_fold(LatticeSet s, FusedScratch f) {
   const int lat = blockIdx.y;
   if (!lattice_active(s, lat)) return;
   const uint32_t ns = (uint32_t)f.nslots;
+  // ns is a multiple of 128: a wavefront's 64 slots are all in or all out
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= ns) return;
   const size_t base = (size_t)lat * ns;
   const uint32_t m = f.gmn[base + k];
-  if (m == 0xFFFFFFFFu) return;
-  const uint32_t r = groot(f.guf + base, k);
-  if (r == k) return;
-  atomicAdd((unsigned long long*)&f.gcnt[base + r], (unsigned long long)f.gcnt[base + k]);
-  atomicMin(&f.gmn[base + r], m);
+  bool fold = m != 0xFFFFFFFFu;
+  const uint32_t r = fold ? groot(f.guf + base, k) : k;
+  fold = fold && r != k;
+  const uint64_t c = fold ? f.gcnt[base + k] : 0ull;
+  // The folding reps of a wavefront under the first one's root (below the
+  // transition nearly all: one giant cluster) add their counts and take the
+  // minimum of their (site << 1 | flip) words by shuffles, and one lane makes
+  // the two atomics -- per-lane atomics on one address serialise in the L2.
+  const uint64_t fm = __ballot(fold);
+  if (fm == 0ull) return;
+  const int first = __ffsll((long long)fm) - 1;
+  const uint32_t R = (uint32_t)__shfl((int)r, first);
+  const bool in = fold && r == R;
+  unsigned long long cs = in ? c : 0ull;
+  uint32_t mm = in ? m : 0xFFFFFFFFu;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    cs += __shfl_xor(cs, off);
+    mm = min(mm, (uint32_t)__shfl_xor((int)mm, off));
+  }
+  if ((int)__lane_id() == first) {
+    atomicAdd((unsigned long long*)&f.gcnt[base + R], cs);
+    atomicMin(&f.gmn[base + R], mm);
+  }
+  if (fold && !in) {
+    atomicAdd((unsigned long long*)&f.gcnt[base + r], (unsigned long long)c);
+    atomicMin(&f.gmn[base + r], m);
+  }
 }
 
 // Flip bit of every slot's cluster; merged clusters measured at their roots
