@@ -293,6 +293,9 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
   // the boundary bonds (B2): on the first wavefront past them, else on the
   // last one.  Its bin state lives in registers of that lane.
   const int fold_tid = min((int)((2u * L + 63u) / 64u) * 64, NT - 64);
+  // The B1 union queues (in cnt) are cleared during B2 as well, by the
+  // wavefronts past the fold lane's (all of them when there are none).
+  const int clr0 = fold_tid + 64 < NT ? fold_tid + 64 : 0;
   int64_t cur_bin = -1, bin_end = 0;  // lane fold_tid only
   __syncthreads();
 
@@ -466,7 +469,7 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
         req |= ((b1 >> 4) & 1u & ((lane == 63 && p.y1 != L - 1u) ? 1u : 0u)) << 2;
       }
       if (compact) {
-        // the wave's requests go through a dense queue (in cnt, cleared below)
+        // the wave's requests go through a dense queue (in cnt, cleared in B2)
         const uint32_t qa[3] = {p.n0, p.n1, p.n1};
         const uint32_t qb[3] = {p.nr0, p.nr1, p.nu1};
         wave_union_queue(uf, cnt + wave * qw, req, qa, qb, below);
@@ -480,8 +483,6 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
         }
       }
     }
-    if (compact)
-      for (uint32_t i = (uint32_t)lane; i < qw; i += 64u) cnt[wave * qw + i] = 0u;
     __syncthreads();
     // ----------------------------------------------------------- B2
     // The at most 2L bonds across the periodic boundary, with windings: x
@@ -500,6 +501,8 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
     // sweep g-1's partials were published before the barrier after A, and its
     // wrap slot is not the one B2 of sweep g sets
     if (pending && tid == fold_tid) finalize(g - 1);
+    if (compact && tid >= clr0)
+      for (uint32_t i = (uint32_t)(tid - clr0); i < N; i += (uint32_t)(NT - clr0)) cnt[i] = 0u;
     __syncthreads();
     // ------------------------------------------------------------ C
     // Counts go to cnt[root] by LDS atomics; the sites of the first lane's
