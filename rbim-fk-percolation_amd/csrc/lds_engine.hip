@@ -424,12 +424,12 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
       if (p.v0) {
         sb[p.n0] = (uint8_t)((b0 & 7u) | (oh0 << 3) | (ov0 << 4) | (fl0 << 5));
         uf[p.n0] = start0;
-        cnt[p.n0] = 0u;
+        if (!compact) cnt[p.n0] = 0u;  // else B1's queue holds cnt until B2 clears it
       }
       if (p.v1) {
         sb[p.n1] = (uint8_t)((b1 & 7u) | (oh1 << 3) | (ov1 << 4) | (fl1 << 5));
         uf[p.n1] = internal ? start0 : p.n1;
-        cnt[p.n1] = 0u;
+        if (!compact) cnt[p.n1] = 0u;
       }
     }
     if (pending) publish();
