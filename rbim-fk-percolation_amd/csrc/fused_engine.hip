@@ -176,8 +176,9 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
   constexpr uint32_t NS = 4u * T;  // edge slots per tile
   constexpr uint32_t QW = T * T / (NTH / 64);  // union queue entries per wave (>= 192)
   static_assert(QW >= 192, "a wave queues up to 3 unions per lane");
+  static_assert(QW % 4u == 0u, "queues are cleared with 16-byte stores");
   __shared__ uint32_t par[T * T];
-  __shared__ uint32_t lcnt[T * T];
+  __shared__ __align__(16) uint32_t lcnt[T * T];
   __shared__ uint8_t bits[T * T];  // bit0 spin, bit1 occ_h, bit2 occ_v, bit3 flip draw, bit4/5 J<0
   __shared__ uint8_t hsp[2 * T];
   __shared__ int64_t red[NTH / 64][PARTS];
@@ -333,7 +334,10 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
     const uint32_t qb[3] = {li0 + T, li1 + T, li1 + 1u};
     wave_union_queue(par, lcnt + wave * QW, req, qa, qb, below);
   }
-  for (uint32_t i = lane; i < QW; i += 64) lcnt[wave * QW + i] = 0u;  // this wave's queue
+  {  // this wave's queue, cleared with 16-byte stores (QW is a multiple of 4)
+    uint4* q4 = reinterpret_cast<uint4*>(lcnt + wave * QW);
+    for (uint32_t i = lane; i < QW / 4u; i += 64u) q4[i] = make_uint4(0u, 0u, 0u, 0u);
+  }
   __syncthreads();
   // roots: counts (size << 16 | n_plus before the flip); every site is pointed
   // straight at its root (the forest is settled, so this is only compression)
