@@ -771,7 +771,8 @@ __global__ __launch_bounds__(KB_THREADS) void kb_merge(LatticeSet s, FusedScratc
   wave_or_wraps(&wrapbits, wraps);
   __syncthreads();
   // reps fold into their roots and are pointed straight at them (the forest is
-  // settled and its windings are no longer read)
+  // settled and its windings are no longer read); per-lane LDS atomics, as
+  // the wavefront sums of kg_fold cost more here than they save
   for (uint32_t k = tid; worker && k < ns; k += KB_WORK) {
     if (mn[k] == 0xFFFFFFFFu) continue;
     const uint32_t r = kroot(uf, k);
