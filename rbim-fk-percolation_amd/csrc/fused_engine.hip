@@ -263,7 +263,7 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
 
   // activation (+ E, M of the starting state) and run pre-labels
   int32_t me = 0, mm = 0;
-#pragma unroll 2
+#pragma unroll
   for (int it = 0; it < ITERS; ++it) {
     const uint32_t ql = (uint32_t)(it * NTH + tid);
     const uint32_t lx = ql / (T / 2), ly0 = (ql % (T / 2)) * 2u;
@@ -399,7 +399,7 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
   // one tile the local index orders like the site; the winners' n_plus is
   // looked up afterwards.
   uint32_t ik1 = 0u, ik2 = 0u, incl = 0u;
-#pragma unroll 2
+#pragma unroll
   for (int it = 0; it < ITERS; ++it) {
     const uint32_t ql = (uint32_t)(it * NTH + tid);
     const uint32_t lx = ql / (T / 2), ly0 = (ql % (T / 2)) * 2u;
