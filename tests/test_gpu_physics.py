@@ -42,13 +42,15 @@ def _zscores(ours, ref):
 
 
 @pytest.mark.parametrize("fixture", ["survey_L16_p0.00", "survey_L16_p0.10", "survey_L32_p0.00",
-                                     "survey_L32_p0.05", "prod_L16_p0.10", "prod_L32_p0.10"])
+                                     "survey_L32_p0.05", "prod_L16_p0.10", "prod_L32_p0.10",
+                                     "prod_L48_p0.10", "prod_L64_p0.05"])
 def test_scan_agrees_with_reference_results(fixture):
     """Each fixture has 16 observables x 24 temperatures = 384 comparisons, so
     a strict 3-sigma rule on every point would fail by chance.  The criterion:
     at most 1% of the points beyond 3 combined sigma and none beyond 5.  The
-    R = 10000 scans (200 000 lattices each, about 10 s on the GPU) measured
-    max 2.1 sigma over all 640 points.  (The
+    R = 10000 scans (200 000 lattices each, 4-22 s on the GPU; L = 48 runs
+    the LDS kernel with a run-time L) at L = 16 and 32 measured max 2.1 sigma
+    over all 640 points.  (The
     reference's own wrap_x at L=16, p=0.10, T=2.6 is 3.8 sigma below the value
     a 100x longer run of the same disorder gives, and 2.5 sigma below its own
     wrap_y: such points occur.)"""
