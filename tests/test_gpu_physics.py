@@ -3,9 +3,11 @@ result files, and the p = 0 energy against Kaufman's exact finite-L formula.
 
 The result files are the reference's own scans: tests/golden/survey_*.npz
 (results/survey/, R = 64, 2000 measured + 500 thermalisation sweeps, 20 bins,
-seed 1, 24 temperatures) and tests/golden/prod_*.npz (results/prod/, the same
+seed 1, 24 temperatures), tests/golden/prod_*.npz (results/prod/, the same
 schedule with R = 10000 realizations at 20 temperatures in [1.58, 2.25], so
-errors 12x smaller).  Re-run with the same seed, the disorder is identical
+errors 12x smaller), and, with seed 2 and R = 512, fine_* (results/fine/, T in
+[1.6, 2.1]), low_* (results/low/, T in [1.2, 2.0]) and the hot- and cold-start
+pair eq_*_T1.4_{hot,cold} (results/eq/, 5000 thermalisation sweeps).  Re-run with the same seed, the disorder is identical
 (numpy draws, same SeedSequence path), so the two scans differ only in their
 thermal streams; every observable must agree within three combined jackknife
 errors.
@@ -43,7 +45,9 @@ def _zscores(ours, ref):
 
 @pytest.mark.parametrize("fixture", ["survey_L16_p0.00", "survey_L16_p0.10", "survey_L32_p0.00",
                                      "survey_L32_p0.05", "prod_L16_p0.10", "prod_L32_p0.10",
-                                     "prod_L48_p0.10", "prod_L64_p0.05"])
+                                     "prod_L48_p0.10", "prod_L64_p0.05", "fine_L32_p0.10",
+                                     "low_L32_p0.08", "eq_L32_p0.08_T1.4_cold",
+                                     "eq_L32_p0.08_T1.4_hot"])
 def test_scan_agrees_with_reference_results(fixture):
     """Each fixture has 16 observables x 24 temperatures = 384 comparisons, so
     a strict 3-sigma rule on every point would fail by chance.  The criterion:
@@ -58,7 +62,8 @@ def test_scan_agrees_with_reference_results(fixture):
     md = ref["metadata"]
     ours = run_temperature_scan(md["L"], md["p"], ref["temperatures"],
                                 n_realizations=md["n_realizations"], n_sweeps=md["n_sweeps"],
-                                n_therm=md["n_therm"], n_bins=md["n_bins"], seed=md["seed"])
+                                n_therm=md["n_therm"], n_bins=md["n_bins"], seed=md["seed"],
+                                start=md.get("start") or "hot")
     z = _zscores(ours, ref)
     beyond3 = [row for row in z if row[4] > 3.0]
     assert len(beyond3) <= 0.01 * len(z), beyond3
