@@ -12,7 +12,10 @@ generates it), spins start hot.
 Printed as one JSON line, with
   roofline      achieved algorithmic HBM bytes of the dominant kernel
                 (4 B per site-update: spin read + write, J_h + J_v read) per
-                average launch duration, vs MI355X's 8 TB/s;
+                average launch duration, vs MI355X's 8 TB/s; with the measured
+                HBM traffic and, as valu_issue, the share of the VALU issue
+                cycles the kernel's profiled instruction count fills (its
+                actual bound);
   plain         the one-lattice-at-a-time tiled path at the same L;
   cpu_baseline  the oracle's C restatement of the reference sweep on all host
                 cores (one lattice per thread), on a bounded sample (~10 s);
@@ -37,6 +40,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "rbim-fk-percolation_amd"))
 
 PEAK_HBM_GBS = 8000.0
+# VALU issue capacity: 256 CUs x 4 SIMDs at the 2.4 GHz peak engine clock; a
+# wave64 VALU instruction holds its SIMD for 4 cycles
+VALU_SIMD_CYCLES_PER_S = 1024 * 2.4e9
 ALGO_BYTES_PER_SITE_UPDATE = 4
 
 
@@ -101,24 +107,26 @@ def gpu_plain(spec, K):
     return spec.L * spec.L * K / (ms / 1e3)
 
 
-def profiled_traffic(L, n_lat, sweeps_per_launch, threads):
-    """HBM bytes per launch (LDS kernel) or per sweep (tiled kernels) of the
-    dominant kernel, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
-    this benchmark's batches (calibrated against a plain device copy); None when
-    the configuration differs from the profiled one."""
+def profiled_fields(L, n_lat, sweeps_per_launch, threads):
+    """The `key: integer` lines of the committed rocprofv3 summary of this
+    benchmark's batch (HBM traffic from FETCH_SIZE / WRITE_SIZE calibrated
+    against a plain device copy; SQ VALU instructions per wavefront-sweep of
+    the dominant kernel), or {} when the configuration is not the profiled one."""
     if (L, n_lat, sweeps_per_launch, threads) == (32, 1280, 2000, 256):
-        name, key = "lds_sweep_kernel_L32_batched.md", "traffic_bytes_per_launch:"
+        name = "lds_sweep_kernel_L32_batched.md"
     elif (L, n_lat) == (256, 1280):
-        name, key = "edge_slot_sweep_L256_batched.md", "traffic_bytes_per_sweep:"
+        name = "edge_slot_sweep_L256_batched.md"
     else:
-        return None
+        return {}
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
-        return None
+        return {}
+    out = {}
     for line in open(path):
-        if line.startswith(key):
-            return int(line.split(":")[1])
-    return None
+        key, sep, val = line.partition(":")
+        if sep and key.isidentifier() and val.strip().isdigit():
+            out[key] = int(val)
+    return out
 
 
 def cpu_baseline(specs, site_updates_per_thread, per_thread):
@@ -188,6 +196,17 @@ def measure_point(L, args, steps, warmup, plain_steps, with_cpu):
         kernel = ("edge-slot sweep: ka_tile + kb_merge" if fused_tile_size(L) and args.engine != 3
                   else "tiled sweep: k1_tile_full + k2_merge + k3_flip + k5_finalize")
     achieved = ALGO_BYTES_PER_SITE_UPDATE * units / (avg_launch_ms / 1e3) / 1e9
+    prof = profiled_fields(L, n_lat, res["sweeps_per_launch"], res["threads"])
+    traffic = prof.get("traffic_bytes_per_launch" if lds else "traffic_bytes_per_sweep")
+    valu = None
+    if "valu_per_wavefront_sweep" in prof and "wavefronts_per_sweep" in prof:
+        # what the kernel is bound by: VALU issue, from the profiled instruction
+        # count over this run's time per sweep
+        cycles = 4.0 * prof["valu_per_wavefront_sweep"] * prof["wavefronts_per_sweep"]
+        valu = {"kernel": "lds_sweep_kernel" if lds else "ka_tile",
+                "instructions_per_wavefront_sweep": prof["valu_per_wavefront_sweep"],
+                "wavefronts_per_sweep": prof["wavefronts_per_sweep"],
+                "frac": cycles / (VALU_SIMD_CYCLES_PER_S * res["ms"] / steps / 1e3)}
     plain = gpu_plain(specs[len(specs) // 2], plain_steps)
     point = {
         "L": L,
@@ -207,10 +226,11 @@ def measure_point(L, args, steps, warmup, plain_steps, with_cpu):
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBS,
-            "traffic": profiled_traffic(L, n_lat, res["sweeps_per_launch"], res["threads"]),
+            "traffic": traffic,
             "kernel": kernel,
             "avg_launch_ms": avg_launch_ms,
             "launches": launches if lds else steps,
+            "valu_issue": valu,
         },
         "plain": {"value": plain, "unit": "site-updates/s", "L": L,
                   "sample": f"one lattice, {plain_steps} sweeps, tiled kernels in hipGraphs"},
