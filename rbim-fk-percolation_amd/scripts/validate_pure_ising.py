@@ -5,10 +5,12 @@ At p = 0 the model is the ferromagnetic square-lattice Ising model with
 T_c = 2 / ln(1 + sqrt(2)) = 2.269185...  Two independent finite-size crossings
 must land on it: the Binder cumulant U = 1 - <m^4> / (3 <m^2>^2) (magnetic) and
 the FK wrapping probability wrap_y (geometric); their coincidence at p = 0 is
-the Coniglio-Klein identity.  Same flags and report as the reference's
-scripts/validate_pure_ising.py; every (L, T, replica) lattice here is one
-device batch, and --replicas independent random streams per temperature add
-statistics (at p = 0 every "realization" has the same couplings).
+the Coniglio-Klein identity.  The flags are those of the reference's
+scripts/validate_pure_ising.py and the estimates the same (mean over all size
+pairs of the first crossing on the grid), reported as tables.  Every
+(L, T, replica) lattice is one device batch; --replicas independent random
+streams per temperature add statistics (at p = 0 every "realization" has the
+same couplings).
 
     python rbim-fk-percolation_amd/scripts/validate_pure_ising.py --sizes 32 64 128 --sweeps 20000
 """
@@ -16,7 +18,6 @@ statistics (at p = 0 every "realization" has the same couplings).
 from __future__ import annotations
 
 import argparse
-import itertools
 import os
 import sys
 from typing import Dict, List, Optional, Sequence, Tuple
@@ -25,46 +26,31 @@ import numpy as np
 
 sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 
+from rbim.crossings import mean_crossing, size_pair_crossings  # noqa: E402
 from rbim.simulation import run_temperature_scan  # noqa: E402
 
 TC_EXACT = 2.0 / np.log(1.0 + np.sqrt(2.0))
 
 
-def crossing(temperatures: np.ndarray, a: np.ndarray, b: np.ndarray) -> Optional[float]:
-    """First temperature where curves a and b meet, linearly interpolated
-    between the grid points that bracket a sign change of a - b (a grid point
-    where they are equal counts); None when a - b keeps its sign."""
-    d = np.asarray(a, dtype=float) - np.asarray(b, dtype=float)
-    for i in range(len(d) - 1):
-        if d[i] == 0.0:
-            return float(temperatures[i])
-        if d[i] * d[i + 1] < 0.0:
-            t0, t1 = float(temperatures[i]), float(temperatures[i + 1])
-            return t0 + (t1 - t0) * d[i] / (d[i] - d[i + 1])
-    return None
-
-
-def pairwise_crossings(temperatures, curves: Dict[int, np.ndarray]) -> List[Tuple[int, int, float]]:
-    out = []
-    for La, Lb in itertools.combinations(sorted(curves), 2):
-        t = crossing(temperatures, curves[La], curves[Lb])
-        if t is not None:
-            out.append((La, Lb, t))
-    return out
-
-
-def report(name: str, temperatures, curves) -> Optional[float]:
-    found = pairwise_crossings(temperatures, curves)
-    print(f"\n{name} crossings")
-    if not found:
-        print("  none found on this temperature grid")
-        return None
-    for La, Lb, t in found:
-        print(f"  L={La:3d} vs L={Lb:3d}:  T = {t:.4f}   ({t - TC_EXACT:+.4f} from exact)")
-    estimate = float(np.mean([c[2] for c in found]))
-    print(f"  mean: T = {estimate:.4f}   ({estimate - TC_EXACT:+.4f} from exact, "
-          f"{100 * abs(estimate - TC_EXACT) / TC_EXACT:.2f}%)")
-    return estimate
+def crossing_summary(name: str, temperatures, curves: Dict[int, np.ndarray]
+                     ) -> Tuple[Optional[float], List[str]]:
+    """Mean crossing temperature of one observable over all size pairs, and
+    the lines of a table of the pairs (offset and relative offset from the
+    exact T_c)."""
+    pairs = size_pair_crossings(temperatures, curves)
+    estimate = mean_crossing(pairs)
+    lines = [f"{name}: {pairs.size} crossing(s)"]
+    if estimate is None:
+        lines.append("    (the curves do not cross on this grid)")
+        return None, lines
+    lines.append(f"    {'pair':>12}  {'T_cross':>8}  {'T - T_c':>8}")
+    for row in pairs:
+        tag = f"{row['L_small']}/{row['L_large']}"
+        lines.append(f"    {tag:>12}  {row['T']:8.4f}  {row['T'] - TC_EXACT:+8.4f}")
+    rel = abs(estimate - TC_EXACT) / TC_EXACT
+    lines.append(f"    {'mean':>12}  {estimate:8.4f}  {estimate - TC_EXACT:+8.4f}  "
+                 f"({100 * rel:.3f} % of T_c)")
+    return estimate, lines
 
 
 def measure(sizes: Sequence[int], temperatures, sweeps: int, bins: int, seed: int,
@@ -92,16 +78,15 @@ def main(argv: Sequence[str] | None = None) -> Tuple[Optional[float], Optional[f
     ap.add_argument("--replicas", type=int, default=1, help="independent streams per temperature")
     args = ap.parse_args(argv)
     temperatures = np.linspace(args.t_min, args.t_max, args.n_temps)
-    print(f"exact T_c = {TC_EXACT:.6f}")
-    print(f"sizes {args.sizes}, {args.n_temps} temperatures in [{args.t_min}, {args.t_max}], "
-          f"{args.sweeps} sweeps, {args.replicas} replica(s)")
+    print(f"T_c (Onsager) {TC_EXACT:.6f}; L in {args.sizes}; T grid of {args.n_temps} points "
+          f"on [{args.t_min}, {args.t_max}]; {args.sweeps} sweeps x {args.replicas} replica(s)")
     binder, wrapping = measure(args.sizes, temperatures, args.sweeps, args.bins, args.seed,
                                args.replicas)
-    binder_tc = report("Binder cumulant", temperatures, binder)
-    wrapping_tc = report("FK wrapping probability", temperatures, wrapping)
+    binder_tc, table = crossing_summary("Binder cumulant", temperatures, binder)
+    wrapping_tc, table2 = crossing_summary("FK wrapping probability wrap_y", temperatures, wrapping)
+    print("\n".join(["", *table, "", *table2]))
     if binder_tc is not None and wrapping_tc is not None:
-        print(f"\nConiglio-Klein check: magnetic and geometric estimates differ by "
-              f"{abs(binder_tc - wrapping_tc):.4f}")
+        print(f"\nConiglio-Klein: |T_Binder - T_wrap| = {abs(binder_tc - wrapping_tc):.4f}")
     return binder_tc, wrapping_tc
 
 

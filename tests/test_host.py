@@ -329,19 +329,30 @@ def test_integrated_autocorrelation_ar1_closed_form():
 
 def test_crossing_interpolation():
     """Linear crossing of two lines and of sampled curves with a known
-    intersection; no crossing -> None; a grid point where they meet."""
-    import os
-    import sys
-
-    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
-                                    "rbim-fk-percolation_amd", "scripts"))
-    import validate_pure_ising as v
+    intersection; no crossing -> NaN; a grid point where they meet; the first
+    of two crossings wins; a meeting at the last grid point is not an event;
+    all size pairs at once equal pair by pair."""
+    from rbim.crossings import first_crossing, mean_crossing, size_pair_crossings
 
     t = np.linspace(2.0, 2.5, 11)
-    assert abs(v.crossing(t, 2.0 * (t - 2.2691), -0.5 * (t - 2.2691)) - 2.2691) < 1e-12
-    assert v.crossing(t, t, t + 1.0) is None
-    assert v.crossing(t, np.where(t == t[3], 0.0, t - t[3]), np.zeros_like(t)) == t[3]
-    curves = {8: 1.0 - (t - 2.27), 16: 1.0 - 2.0 * (t - 2.27), 32: 1.0 - 4.0 * (t - 2.27)}
-    found = v.pairwise_crossings(t, curves)
-    assert [(a, b) for a, b, _ in found] == [(8, 16), (8, 32), (16, 32)]
-    assert all(abs(x - 2.27) < 1e-12 for _, _, x in found)
+    assert abs(first_crossing(t, 2.0 * (t - 2.2691), -0.5 * (t - 2.2691)) - 2.2691) < 1e-12
+    assert np.isnan(first_crossing(t, t, t + 1.0))
+    assert first_crossing(t, np.where(t == t[3], 0.0, t - t[3]), np.zeros_like(t)) == t[3]
+    # (t - 2.12)(t - 2.38): zeros at 2.12 and 2.38 -> the first; the secant through
+    # the bracketing grid points 2.10 (+0.0056) and 2.15 (-0.0069) crosses at
+    # 2.10 + 0.05 * 0.0056 / 0.0125
+    quad = (t - 2.12) * (t - 2.38)
+    assert abs(first_crossing(t, quad, 0.0 * t) - (2.10 + 0.05 * 0.0056 / 0.0125)) < 1e-12
+    assert np.isnan(first_crossing(t, t - t[-1], 0.0 * t))
+    curves = {16: 1.0 - 2.0 * (t - 2.27), 8: 1.0 - (t - 2.27), 32: 1.0 - 4.0 * (t - 2.27),
+              64: 5.0 + 0.0 * t}
+    found = size_pair_crossings(t, curves)
+    assert [(int(a), int(b)) for a, b, _ in found] == [(8, 16), (8, 32), (16, 32)]
+    assert all(abs(x - 2.27) < 1e-12 for x in found["T"])
+    assert abs(mean_crossing(found) - 2.27) < 1e-12
+    rng = np.random.default_rng(5)
+    noisy = {L: rng.normal(size=t.size) for L in (4, 8, 12, 20)}
+    pairs = size_pair_crossings(t, noisy)
+    for a, b, x in pairs:
+        assert first_crossing(t, noisy[int(a)], noisy[int(b)]) == x
+    assert mean_crossing(size_pair_crossings(t, {8: t})) is None
