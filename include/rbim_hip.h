@@ -181,6 +181,21 @@ int rbim_sweep(rbim_ctx* ctx, int32_t L, int8_t* spins, const int8_t* Jh, const 
                uint32_t thr31, uint64_t seed, uint32_t lattice_id, uint32_t sweep,
                int32_t measure, int32_t engine, rbim_sweep_stats* out);
 
+/* rbim_sweep that allocates nothing.  It has no counterpart in the reference:
+ * it is for callers that cannot allocate device memory (tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, checks and results as
+ * rbim_sweep (bit-identical spins and stats for the same engine); every device
+ * buffer of the sweep is carved out of a RBIM_NOALLOC_SCRATCH_BYTES array
+ * that each GPU holds from library load until the process ends, which limits
+ * L to RBIM_NOALLOC_MAX_L (larger L -> RBIM_E_INVAL).  Calls on one GPU share
+ * that array: they are serialised by a lock inside this process (calls that
+ * overlapped on it would race), and each call clears what it uses first.   */
+#define RBIM_NOALLOC_MAX_L 256
+#define RBIM_NOALLOC_SCRATCH_BYTES (2u << 20)
+int rbim_sweep_noalloc(rbim_ctx* ctx, int32_t L, int8_t* spins, const int8_t* Jh,
+                       const int8_t* Jv, uint32_t thr31, uint64_t seed, uint32_t lattice_id,
+                       uint32_t sweep, int32_t measure, int32_t engine, rbim_sweep_stats* out);
+
 /* cluster_statistics (src/rbim/clusters.py:98-118): sizes and signed cluster
  * magnetizations from labels 0..n_clusters-1. */
 int rbim_cluster_statistics(rbim_ctx* ctx, int64_t n_sites, const int32_t* labels,

@@ -20,7 +20,35 @@ struct rbim_ctx {
   std::string err;
 };
 
+// Device memory of rbim_sweep_noalloc: every buffer of its one-lattice sweep
+// is carved out of this array, so that entry point never allocates.  Each GPU
+// holds its own copy from module load until the process ends.  Host code
+// never uses the array's name as an address: rbim_open resolves the device
+// address once per device (hipGetSymbolAddress) into g_scratch.
+static __device__ __align__(256) uint8_t scratch[RBIM_NOALLOC_SCRATCH_BYTES];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+
+// Per device: the resolved address of `scratch` and the lock that serialises
+// the calls sharing it (they would race on the array otherwise).
+struct ScratchSlot {
+  std::mutex mu;
+  uint8_t* ptr = nullptr;
+};
+ScratchSlot g_scratch[kMaxDevices];
+
+// With `device` current: resolve the device address of `scratch` once.
+hipError_t resolve_scratch(int device) {
+  ScratchSlot& sl = g_scratch[device];
+  std::lock_guard<std::mutex> lk(sl.mu);
+  if (sl.ptr) return hipSuccess;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e == hipSuccess) sl.ptr = static_cast<uint8_t*>(p);
+  return e;
+}
 
 std::mutex g_err_mu;
 std::string g_err;
@@ -419,6 +447,123 @@ int run_single(rbim_ctx* ctx, int L, int engine, int8_t* spins, const int8_t* Jh
   return RBIM_OK;
 }
 
+// ------------------------------------------ allocation-free one-lattice sweep
+// Carves the device buffers of a one-lattice group -- the LDS kernel's
+// (lds) or the multi-kernel tiled sweep's -- plus staging for the raw
+// couplings out of [base, base + cap), 256-byte aligned, and points g's
+// descriptors at them.  Returns the bytes spanned, or 0 when they exceed cap.
+// base == nullptr only sizes the layout.
+size_t carve_single(uint8_t* base, size_t cap, int L, bool lds, Group& g, int8_t** jh_raw,
+                    int8_t** jv_raw) {
+  const size_t N = (size_t)L * L;
+  size_t used = 0;
+  bool fits = true;
+  auto take = [&](size_t n) -> uint8_t* {
+    const size_t off = (used + 255) & ~(size_t)255;
+    if (off + n > cap) fits = false;
+    used = off + n;
+    return base ? base + off : nullptr;
+  };
+  LatticeSet& s = g.set;
+  s.spins = (int8_t*)take(N);
+  s.jbits = take(N);
+  s.thr31 = (const uint32_t*)take(4);
+  s.seed = (const uint64_t*)take(8);
+  s.lat_id = (const uint32_t*)take(4);
+  s.sweep_ctr = (int64_t*)take(8);
+  s.acc = (uint64_t*)take(RBIM_ACC_SLOTS * 8);
+  s.last = (rbim_sweep_stats*)take(sizeof(rbim_sweep_stats));
+  *jh_raw = (int8_t*)take(N);
+  *jv_raw = (int8_t*)take(N);
+  if (!lds) {
+    TiledScratch& t = g.tile;
+    t.tile = tiled_tile_size(L, 0);
+    t.tiles_x = t.tiles_y = (L + t.tile - 1) / t.tile;
+    t.occ = take(N);
+    t.lroot = (uint16_t*)take(2 * N);
+    t.gf = (uint64_t*)take(8 * N);
+    t.lst = (uint32_t*)take(4 * N);
+    t.st = (uint64_t*)take(8 * N);
+    t.part = (int64_t*)take((size_t)t.tiles_x * t.tiles_y * 7 * 8);
+    t.wrapf = (uint32_t*)take(8);
+    t.pending = (int32_t*)take(4);
+  }
+  used = (used + 255) & ~(size_t)255;
+  return fits ? used : 0;
+}
+
+__global__ void scratch_zero_kernel(uint4* p, size_t n16) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n16;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// Packs the staged couplings into coupling bits and writes the lattice's
+// scalars (the buffers are the caller's scratch, hence the casts).
+__global__ void noalloc_setup_kernel(LatticeSet s, const int8_t* jh, const int8_t* jv,
+                                     uint32_t thr, uint64_t seed, uint32_t lid, int64_t g0) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < s.N)
+    const_cast<uint8_t*>(s.jbits)[n] = (uint8_t)((jh[n] < 0 ? 1 : 0) | (jv[n] < 0 ? 2 : 0));
+  if (n == 0) {
+    const_cast<uint32_t*>(s.thr31)[0] = thr;
+    const_cast<uint64_t*>(s.seed)[0] = seed;
+    const_cast<uint32_t*>(s.lat_id)[0] = lid;
+    s.sweep_ctr[0] = g0;
+  }
+}
+
+// One sweep of one lattice, as run_single without given bits, on buffers
+// carved from the device's scratch array: no allocation of any kind.
+int run_single_noalloc(rbim_ctx* ctx, uint8_t* base, int L, bool lds, int8_t* spins,
+                       const int8_t* Jh, const int8_t* Jv, uint32_t thr, uint64_t seed,
+                       uint32_t lid, uint32_t sweep, int measure, rbim_sweep_stats* stats) {
+  const int N = L * L;
+  Group g;  // descriptors only: its arena stays empty
+  g.L = L;
+  g.N = N;
+  g.n_lat = 1;
+  g.lds = lds;
+  g.threads = default_threads(L);
+  g.per_launch = 1;
+  int8_t *jh_raw = nullptr, *jv_raw = nullptr;
+  const size_t used = carve_single(base, RBIM_NOALLOC_SCRATCH_BYTES, L, lds, g, &jh_raw, &jv_raw);
+  if (used == 0)
+    return set_err(ctx, RBIM_E_INVAL, "L=%d needs more than the %u-byte scratch", L,
+                   (unsigned)RBIM_NOALLOC_SCRATCH_BYTES);
+  LatticeSet& s = g.set;
+  s.L = L;
+  s.N = N;
+  s.n_lat = 1;
+  s.magic = magic_for(L);
+  s.n_therm = measure ? (int64_t)sweep : (int64_t)sweep + 1;
+  s.spb = 1;
+  s.n_bins = 1;
+  s.total = s.n_therm + 1;
+  s.series = nullptr;
+  s.series_cap = 0;
+  s.given_occ = nullptr;
+  s.given_flip = nullptr;
+  s.canon_out = nullptr;
+  hipStream_t st = ctx->stream;
+  // the previous call's contents stay in the array: clear everything this one uses
+  hipLaunchKernelGGL(scratch_zero_kernel, dim3(512), dim3(256), 0, st, (uint4*)base, used / 16);
+  HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipMemcpyAsync(s.spins, spins, N, hipMemcpyHostToDevice, st));
+  HIP_TRY(ctx, hipMemcpyAsync(jh_raw, Jh, N, hipMemcpyHostToDevice, st));
+  HIP_TRY(ctx, hipMemcpyAsync(jv_raw, Jv, N, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(noalloc_setup_kernel, dim3((N + 255) / 256), dim3(256), 0, st, s, jh_raw,
+                     jv_raw, thr, seed, lid, (int64_t)sweep);
+  HIP_TRY(ctx, hipGetLastError());
+  int64_t launches = 0;
+  const int rc = group_run(ctx, g, 1, st, &launches);
+  if (rc != RBIM_OK) return rc;
+  HIP_TRY(ctx, hipMemcpyAsync(spins, s.spins, N, hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipMemcpyAsync(stats, s.last, sizeof(rbim_sweep_stats), hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  return RBIM_OK;
+}
+
 // active[2N] in bond order -> per-site bits (bit0 horizontal, bit1 vertical)
 std::vector<uint8_t> pack_active(const uint8_t* active, int N) {
   std::vector<uint8_t> o(N);
@@ -484,11 +629,16 @@ int rbim_open(int device, rbim_ctx** out) {
                    e != hipSuccess ? hipGetErrorString(e) : "device count is 0");
   if (device < 0 || device >= n)
     return set_err(nullptr, RBIM_E_INVAL, "device %d out of range (have %d)", device, n);
+  if (device >= kMaxDevices)
+    return set_err(nullptr, RBIM_E_INVAL, "device %d beyond the %d this library tracks", device,
+                   kMaxDevices);
   rbim_ctx* c = new rbim_ctx();
   c->device = device;
   e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = resolve_scratch(device);
   if (e != hipSuccess) {
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return set_err(nullptr, RBIM_E_HIP, "context creation failed: %s", hipGetErrorString(e));
   }
@@ -589,6 +739,32 @@ int rbim_sweep(rbim_ctx* ctx, int32_t L, int8_t* spins, const int8_t* Jh, const 
   rbim_sweep_stats st{};
   int rc = run_single(ctx, L, engine, spins, Jh, Jv, nullptr, nullptr, thr31, seed, lattice_id,
                       sweep, measure, nullptr, &st, nullptr);
+  if (rc == RBIM_OK && out && measure) *out = st;
+  return rc;
+}
+
+int rbim_sweep_noalloc(rbim_ctx* ctx, int32_t L, int8_t* spins, const int8_t* Jh,
+                       const int8_t* Jv, uint32_t thr31, uint64_t seed, uint32_t lattice_id,
+                       uint32_t sweep, int32_t measure, int32_t engine, rbim_sweep_stats* out) {
+  if (!ctx || !spins || !Jh || !Jv) return set_err(ctx, RBIM_E_INVAL, "NULL argument");
+  if (!valid_L(L)) return set_err(ctx, RBIM_E_INVAL, "L must be at least 3 (got %d)", L);
+  if (L > RBIM_NOALLOC_MAX_L)
+    return set_err(ctx, RBIM_E_INVAL,
+                   "rbim_sweep_noalloc holds L <= %d in its %u-byte scratch (got L=%d)",
+                   RBIM_NOALLOC_MAX_L, (unsigned)RBIM_NOALLOC_SCRATCH_BYTES, L);
+  if (engine != RBIM_ENGINE_PLAIN && engine != RBIM_ENGINE_BATCHED && engine != RBIM_ENGINE_AUTO)
+    return set_err(ctx, RBIM_E_INVAL, "unknown engine %d", engine);
+  const bool lds = engine == RBIM_ENGINE_BATCHED;
+  if (lds && L > RBIM_LDS_MAX_L)
+    return set_err(ctx, RBIM_E_INVAL, "the LDS engine holds L <= %d (got %d)", RBIM_LDS_MAX_L, L);
+  if (thr31 > (1u << 31)) return set_err(ctx, RBIM_E_INVAL, "thr31 exceeds 2^31");
+  ScratchSlot& sl = g_scratch[ctx->device];
+  std::lock_guard<std::mutex> lk(sl.mu);  // calls on one device share its array
+  if (!sl.ptr) return set_err(ctx, RBIM_E_STATE, "scratch of device %d not resolved", ctx->device);
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  rbim_sweep_stats st{};
+  const int rc = run_single_noalloc(ctx, sl.ptr, L, lds, spins, Jh, Jv, thr31, seed, lattice_id,
+                                    sweep, measure, &st);
   if (rc == RBIM_OK && out && measure) *out = st;
   return rc;
 }

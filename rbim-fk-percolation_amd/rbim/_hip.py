@@ -125,6 +125,8 @@ SIGNATURES = {
     "rbim_activate": ([_P, _i32, _P, _P, _P, _u32, _u64, _u32, _u32, _P, _P], ctypes.c_int),
     "rbim_sweep_given_bits": ([_P, _i32, _P, _P, _P, _P, _P, _i32, _P, _P], ctypes.c_int),
     "rbim_sweep": ([_P, _i32, _P, _P, _P, _u32, _u64, _u32, _u32, _i32, _i32, _P], ctypes.c_int),
+    "rbim_sweep_noalloc": ([_P, _i32, _P, _P, _P, _u32, _u64, _u32, _u32, _i32, _i32, _P],
+                           ctypes.c_int),
     "rbim_cluster_statistics": ([_P, _i64, _P, _P, _i32, _P, _P], ctypes.c_int),
     "rbim_energy_magnetization": ([_P, _i32, _P, _P, _P, _P, _P], ctypes.c_int),
     "rbim_batch_footprint": ([_P, _P], ctypes.c_int),
