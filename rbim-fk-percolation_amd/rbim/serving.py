@@ -42,7 +42,8 @@ from rbim.simulation import (
     scan_lattices,
 )
 
-__all__ = ["ScanRequest", "Ticket", "ScanServer", "poisson_requests"]
+__all__ = ["ScanRequest", "Ticket", "ScanServer", "poisson_requests", "replay",
+           "mean_request_site_updates"]
 
 
 @dataclass
@@ -276,6 +277,38 @@ class ScanServer:
             if not progressed:
                 self._wake.wait(self.poll_s)
                 self._wake.clear()
+
+
+def replay(server: "ScanServer", arrivals, sweep_scale: float = 1.0):
+    """Submit (arrival time, ScanRequest) pairs at their times (each request's
+    sweeps multiplied by ``sweep_scale``), wait for all of them, and return
+    (server stats, offered site-updates, wall seconds)."""
+    offered = 0
+    for _, req in arrivals:
+        req.n_sweeps = max(req.n_bins, int(req.n_sweeps * sweep_scale))
+        therm = req.n_therm if req.n_therm is not None else max(1, req.n_sweeps // 10)
+        offered += req.L * req.L * req.n_realizations * len(req.temperatures) * (req.n_sweeps + therm)
+    t0 = time.perf_counter()
+    tickets = []
+    for at, req in arrivals:
+        delay = at - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        tickets.append(server.submit(req))
+    for t in tickets:
+        t.result()
+    return server.stats(), offered, time.perf_counter() - t0
+
+
+def mean_request_site_updates(Ls=(32, 64, 128, 256), n_temps=(8, 40), realizations=(4, 32),
+                              sweeps=(1000, 20000)) -> float:
+    """Expected site-updates of one request of :func:`poisson_requests`'
+    stream (independent uniform draws; n_therm = n_sweeps // 10)."""
+    mean_sites = float(np.mean(np.asarray(Ls, dtype=float) ** 2))
+    mean_t = (n_temps[0] + n_temps[1]) / 2.0
+    mean_r = (realizations[0] + realizations[1]) / 2.0
+    mean_sw = float(np.mean([k + k // 10 for k in range(sweeps[0], sweeps[1] + 1)]))
+    return mean_sites * mean_t * mean_r * mean_sw
 
 
 def poisson_requests(rate_hz: float, duration_s: float, seed: int = 0,

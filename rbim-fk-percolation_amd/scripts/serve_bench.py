@@ -17,11 +17,10 @@ import argparse
 import json
 import os
 import sys
-import time
 
 sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 
-from rbim.serving import ScanServer, poisson_requests  # noqa: E402
+from rbim.serving import ScanServer, poisson_requests, replay  # noqa: E402
 
 
 def main(argv=None):
@@ -34,22 +33,8 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args(argv)
     arrivals = poisson_requests(args.rate, args.duration, seed=args.seed)
-    for _, req in arrivals:
-        req.n_sweeps = max(req.n_bins, int(req.n_sweeps * args.sweep_scale))
-    offered = sum(req.L * req.L * req.n_realizations * len(req.temperatures) *
-                  (req.n_sweeps + max(1, req.n_sweeps // 10)) for _, req in arrivals)
     server = ScanServer(max_active=args.max_active, chunk_sweeps=args.chunk)
-    t0 = time.perf_counter()
-    tickets = []
-    for at, req in arrivals:
-        delay = at - (time.perf_counter() - t0)
-        if delay > 0:
-            time.sleep(delay)
-        tickets.append(server.submit(req))
-    for t in tickets:
-        t.result()
-    wall = time.perf_counter() - t0
-    stats = server.stats()
+    stats, offered, wall = replay(server, arrivals, args.sweep_scale)
     server.close()
     stats.update({"requests": len(arrivals), "rate_hz": args.rate, "duration_s": args.duration,
                   "sweep_scale": args.sweep_scale, "offered_site_updates": float(offered),
