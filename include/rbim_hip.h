@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define RBIM_ABI_VERSION 2
+#define RBIM_ABI_VERSION 3
 
 /* Moment slots per (lattice, bin); sums over the measured sweeps of a bin.
  * 128-bit sums are stored as (lo, hi) uint64 pairs.                       */
@@ -230,6 +230,61 @@ int rbim_batch_info_get(rbim_batch* b, rbim_batch_info* out);
  * scripts/measure_autocorrelation.py:69-90 reads these series. */
 int rbim_batch_series(rbim_batch* b, rbim_sweep_stats* out, int64_t* recorded);
 int rbim_batch_destroy(rbim_batch* b);
+
+/* --------------------------------------------------------- slot pools */
+/* A pool keeps n_slots lattice slots of one side L resident on the device,
+ * each with its own schedule, and advances every loaded, unfinished slot in
+ * shared launches: the serving loop's continuous batching, which replaces the
+ * reference's process pool (src/rbim/simulation.py:395-399) for a stream of
+ * scans.  Requests are loaded into free slots between runs; a slot stops at
+ * its own n_therm + n_bins * sweeps_per_bin.  A lattice's moments depend only
+ * on its seed, lattice id, couplings, start and schedule -- never on the slot,
+ * the other slots or the run boundaries -- so they equal a batch's, bit for
+ * bit.  A pool handle is used by one host thread at a time.                 */
+typedef struct rbim_pool rbim_pool;
+typedef struct {
+  int32_t L;                  /* side of every slot's lattice                      */
+  int32_t n_slots;            /* 1 .. 2^20                                         */
+  int32_t max_bins;           /* bins of each slot's moment sums (>= every load's) */
+  int32_t engine;             /* rbim_engine, as for batches                       */
+  int32_t threads;            /* LDS kernel workgroup size; 0 -> default           */
+  int32_t sweeps_per_launch;  /* LDS kernel sweeps per launch; 0 -> default        */
+  int32_t tile;               /* tiled engines' tile side; 0 -> by L               */
+} rbim_pool_desc;
+
+/* Device bytes a pool would hold (admission by HBM footprint). */
+int rbim_pool_footprint(const rbim_pool_desc* desc, uint64_t* bytes);
+int rbim_pool_create(rbim_ctx* ctx, const rbim_pool_desc* desc, rbim_pool** out);
+/* Loads n lattices into the free slots `slots` (queued after the pool's
+ * earlier work): couplings Jh / Jv (n x L^2 concatenated, +-1), thresholds,
+ * seeds, lattice ids, initial spins (n x L^2) or NULL for `start`, and one
+ * schedule for all n.  RBIM_E_STATE if a slot is in use.                    */
+int rbim_pool_load(rbim_pool* p, int32_t n, const int32_t* slots, const int8_t* Jh,
+                   const int8_t* Jv, const uint32_t* thr31, const uint64_t* seed,
+                   const uint32_t* lattice_id, const int8_t* spins0, int32_t start,
+                   int64_t n_therm, int32_t n_bins, int64_t sweeps_per_bin);
+/* Queues up to n_sweeps sweeps of every loaded, unfinished slot (asynchronous). */
+int rbim_pool_run(rbim_pool* p, int64_t n_sweeps);
+int rbim_pool_query(rbim_pool* p, int32_t* done);
+int rbim_pool_sync(rbim_pool* p);
+/* Sweeps queued so far and total sweeps of each slot (-1: slot not loaded);
+ * host bookkeeping, no device access.                                       */
+int rbim_pool_progress(rbim_pool* p, int32_t n, const int32_t* slots, int64_t* done,
+                       int64_t* total);
+/* Waits for the pool, then copies each slot's moment sums, [n][max_bins][20]
+ * (the first n_bins bins of its load are meaningful), and its spins
+ * ([n][L^2], or NULL).                                                      */
+int rbim_pool_read(rbim_pool* p, int32_t n, const int32_t* slots, uint64_t* moments,
+                   int8_t* spins);
+/* Frees slots for later loads (each load clears its slots first). */
+int rbim_pool_release(rbim_pool* p, int32_t n, const int32_t* slots);
+int rbim_pool_destroy(rbim_pool* p);
+
+/* Test hook with no counterpart in the reference: after `after` more device
+ * allocations succeed, the next one fails as out of memory (RBIM_E_OOM);
+ * after < 0 disarms it.  Lets tests show that callers survive a failed
+ * allocation.                                                               */
+int rbim_inject_alloc_failure(int64_t after);
 
 #ifdef __cplusplus
 }

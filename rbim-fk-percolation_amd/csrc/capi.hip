@@ -88,12 +88,24 @@ uint32_t magic_for(int L) {
   return (uint32_t)(((1ull << 32) + (uint64_t)L - 1) / (uint64_t)L);
 }
 
+// Failure injection (rbim_inject_alloc_failure): while >= 0, the number of
+// device allocations that still succeed before one fails as out of memory.
+std::atomic<int64_t> g_alloc_countdown{-1};
+
+bool injected_alloc_failure() {
+  int64_t c = g_alloc_countdown.load();
+  while (c >= 0)
+    if (g_alloc_countdown.compare_exchange_weak(c, c == 0 ? -1 : c - 1)) return c == 0;
+  return false;
+}
+
 // Device allocations of one group, released together.
 struct DevArena {
   std::vector<void*> ptrs;
   size_t bytes = 0;
   hipError_t alloc(void** p, size_t n) {
     if (n == 0) n = 16;
+    if (injected_alloc_failure()) return hipErrorOutOfMemory;
     hipError_t e = hipMalloc(p, n);
     if (e == hipSuccess) {
       ptrs.push_back(*p);
@@ -185,37 +197,46 @@ int default_lds_sweeps(int L) {
 }
 
 // Allocates and fills the device side of a group.  jh/jv/spins0 are the
-// group's host slices (concatenated); thr/seed/lid per lattice.
+// group's host slices (concatenated); thr/seed/lid per lattice.  With
+// `empty` (slot pools) the host vectors are not read: every array is
+// allocated and zeroed, and the slots are filled later on the device.
 int group_alloc(rbim_ctx* ctx, Group& g, const std::vector<uint8_t>& jbits,
                 const std::vector<uint32_t>& thr, const std::vector<uint64_t>& seed,
                 const std::vector<uint32_t>& lid, const int8_t* spins0, int start,
-                int64_t n_therm, int64_t spb, int64_t n_bins, int64_t g0, int series = 0) {
+                int64_t n_therm, int64_t spb, int64_t n_bins, int64_t g0, int series = 0,
+                bool empty = false) {
   const size_t NN = (size_t)g.N * g.n_lat;
   LatticeSet& s = g.set;
   s.L = g.L;
   s.N = g.N;
   s.n_lat = g.n_lat;
+  s.n_run = g.n_lat;
+  s.sched = nullptr;
+  s.lat_map = nullptr;
   s.magic = magic_for(g.L);
   s.n_therm = n_therm;
   s.spb = spb;
   s.n_bins = n_bins;
   s.total = n_therm + n_bins * spb;
   void* p;
+  // host data into a fresh allocation, or zeros when the group starts empty
+  auto upload = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
+    return empty ? hipMemsetAsync(dst, 0, bytes, ctx->stream)
+                 : hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+  };
+  std::vector<int64_t> ctr0(empty ? 0 : g.n_lat, g0);
   HIP_TRY(ctx, g.arena.alloc(&p, NN)); s.spins = (int8_t*)p;
   HIP_TRY(ctx, g.arena.alloc(&p, NN)); s.jbits = (const uint8_t*)p;
-  HIP_TRY(ctx, hipMemcpyAsync(p, jbits.data(), NN, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, upload(p, jbits.data(), NN));
   HIP_TRY(ctx, g.arena.alloc(&p, 4 * g.n_lat)); s.thr31 = (const uint32_t*)p;
-  HIP_TRY(ctx, hipMemcpyAsync(p, thr.data(), 4 * g.n_lat, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, upload(p, thr.data(), 4 * g.n_lat));
   HIP_TRY(ctx, g.arena.alloc(&p, 8 * g.n_lat)); s.seed = (const uint64_t*)p;
-  HIP_TRY(ctx, hipMemcpyAsync(p, seed.data(), 8 * g.n_lat, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, upload(p, seed.data(), 8 * g.n_lat));
   HIP_TRY(ctx, g.arena.alloc(&p, 4 * g.n_lat)); s.lat_id = (const uint32_t*)p;
-  HIP_TRY(ctx, hipMemcpyAsync(p, lid.data(), 4 * g.n_lat, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, upload(p, lid.data(), 4 * g.n_lat));
   HIP_TRY(ctx, g.arena.alloc(&p, 8 * g.n_lat)); s.sweep_ctr = (int64_t*)p;
-  {
-    std::vector<int64_t> c(g.n_lat, g0);
-    HIP_TRY(ctx, hipMemcpyAsync(p, c.data(), 8 * g.n_lat, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  }
+  HIP_TRY(ctx, upload(p, ctr0.data(), 8 * g.n_lat));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   const size_t accb = (size_t)g.n_lat * n_bins * RBIM_ACC_SLOTS * 8;
   HIP_TRY(ctx, g.arena.alloc(&p, accb)); s.acc = (uint64_t*)p;
   HIP_TRY(ctx, hipMemsetAsync(p, 0, accb, ctx->stream));
@@ -257,11 +278,8 @@ int group_alloc(rbim_ctx* ctx, Group& g, const std::vector<uint8_t>& jbits,
     HIP_TRY(ctx, zalloc(&p, nl * tiles * 2 * 8)); f.part_em = (int64_t*)p;
     HIP_TRY(ctx, zalloc(&p, 2 * nl * 6 * 8)); f.bst = (int64_t*)p;
     HIP_TRY(ctx, g.arena.alloc(&p, nl * 8)); f.fin = (int64_t*)p;
-    {
-      std::vector<int64_t> c(g.n_lat, g0);
-      HIP_TRY(ctx, hipMemcpyAsync(p, c.data(), nl * 8, hipMemcpyHostToDevice, ctx->stream));
-      HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    }
+    HIP_TRY(ctx, upload(p, ctr0.data(), nl * 8));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     HIP_TRY(ctx, zalloc(&p, nl * 4)); f.cur = (int32_t*)p;
   } else if (!g.lds) {
     TiledScratch& t = g.tile;
@@ -280,7 +298,9 @@ int group_alloc(rbim_ctx* ctx, Group& g, const std::vector<uint8_t>& jbits,
     HIP_TRY(ctx, g.arena.alloc(&p, 4 * g.n_lat)); t.pending = (int32_t*)p;
     HIP_TRY(ctx, hipMemsetAsync(p, 0, 4 * g.n_lat, ctx->stream));
   }
-  if (spins0) {
+  if (empty) {
+    HIP_TRY(ctx, hipMemsetAsync(s.spins, 0, NN, ctx->stream));
+  } else if (spins0) {
     HIP_TRY(ctx, hipMemcpyAsync(s.spins, spins0, NN, hipMemcpyHostToDevice, ctx->stream));
   } else {
     HIP_TRY(ctx, launch_init_spins(s, start, ctx->stream));
@@ -535,6 +555,9 @@ int run_single_noalloc(rbim_ctx* ctx, uint8_t* base, int L, bool lds, int8_t* sp
   s.L = L;
   s.N = N;
   s.n_lat = 1;
+  s.n_run = 1;
+  s.sched = nullptr;
+  s.lat_map = nullptr;
   s.magic = magic_for(L);
   s.n_therm = measure ? (int64_t)sweep : (int64_t)sweep + 1;
   s.spb = 1;
@@ -588,6 +611,83 @@ int ranks_from_canon(rbim_ctx* ctx, const int32_t* canon, int N, int32_t* labels
   a.release();
   if (e != hipSuccess) return set_err(ctx, RBIM_E_HIP, "label ranking failed: %s", hipGetErrorString(e));
   return RBIM_OK;
+}
+
+
+// ------------------------------------------------------------- slot pools
+// Per-slot device state a load clears: `planes` copies, plane_stride bytes
+// apart, of per_lat bytes for each slot (multiples of 4 bytes).
+struct Region {
+  uint8_t* base;
+  uint64_t per_lat, plane_stride;
+  int32_t planes;
+};
+struct RegionTable {
+  Region r[16];
+  int32_t n;
+};
+
+__global__ void pool_reset_kernel(RegionTable t, const int32_t* slots) {
+  const uint64_t slot = (uint64_t)slots[blockIdx.y];
+  const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+  for (int i = 0; i < t.n; ++i) {
+    const Region rg = t.r[i];
+    for (int pl = 0; pl < rg.planes; ++pl) {
+      uint32_t* w = reinterpret_cast<uint32_t*>(rg.base + pl * rg.plane_stride + slot * rg.per_lat);
+      for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < rg.per_lat / 4; k += step)
+        w[k] = 0u;
+    }
+  }
+}
+
+// One staged chunk of lattices k = 0 .. n-1 on the device.
+struct PoolStage {
+  const uint8_t* jbits;  // [n][N]
+  const int8_t* spins;   // [n][N], or NULL: the start's spins
+  const uint32_t* thr;   // [n]
+  const uint64_t* seed;  // [n]
+  const uint32_t* lid;   // [n]
+  const int32_t* slots;  // [n]
+};
+
+// Staged lattices into their slots: couplings, spins, scalars and schedule;
+// the sweep counter and the edge-slot sweep's bookkeeping start at zero.
+__global__ void pool_load_kernel(LatticeSet s, int64_t* sched, int64_t* fin, int32_t* cur,
+                                 PoolStage st, int start, int64_t n_therm, int64_t spb,
+                                 int64_t total) {
+  const int k = blockIdx.y;
+  const int slot = st.slots[k];
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t base = (size_t)slot * s.N, src = (size_t)k * s.N;
+  if (n < s.N) {
+    const_cast<uint8_t*>(s.jbits)[base + n] = st.jbits[src + n];
+    s.spins[base + n] = st.spins ? st.spins[src + n]
+                        : start == RBIM_START_COLD
+                            ? (int8_t)1
+                            : (int8_t)rbim_init_spin(st.seed[k], st.lid[k], (uint32_t)n);
+  }
+  if (n == 0) {
+    const_cast<uint32_t*>(s.thr31)[slot] = st.thr[k];
+    const_cast<uint64_t*>(s.seed)[slot] = st.seed[k];
+    const_cast<uint32_t*>(s.lat_id)[slot] = st.lid[k];
+    s.sweep_ctr[slot] = 0;
+    sched[3 * (size_t)slot] = n_therm;
+    sched[3 * (size_t)slot + 1] = spb;
+    sched[3 * (size_t)slot + 2] = total;
+    if (fin) fin[slot] = 0;
+    if (cur) cur[slot] = 0;
+  }
+}
+
+// Lattices per staged load chunk: the whole pool while its staging stays
+// under 32 MiB.
+int pool_chunk(int N, int n_slots) {
+  const size_t per = 2 * (size_t)N + 32;
+  return (int)std::max<size_t>(1, std::min<size_t>((size_t)n_slots, (32ull << 20) / per));
+}
+
+size_t pool_stage_bytes(int N, int chunk) {
+  return (size_t)chunk * (2 * (size_t)N + 32) + 6 * 256;
 }
 
 }  // namespace
@@ -1142,6 +1242,347 @@ int rbim_batch_destroy(rbim_batch* b) {
   if (b->fork) (void)hipEventDestroy(b->fork);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
+  return RBIM_OK;
+}
+
+}  // extern "C"
+
+// ============================================================ slot pools
+struct rbim_pool {
+  rbim_ctx* ctx = nullptr;
+  Group g;                      // every slot: n_lat = n_slots
+  hipStream_t stream = nullptr;
+  hipEvent_t staged = nullptr;  // the last load's host-to-device copy has run
+  hipEvent_t mapped = nullptr;  // the last run's slot list copy has run
+  int max_bins = 1, chunk = 1;
+  int64_t* d_sched = nullptr;   // [n_slots][3]
+  int32_t* d_map = nullptr;     // [n_slots] slots of the current run
+  uint8_t* d_stage = nullptr;   // one load chunk
+  uint8_t* h_stage = nullptr;   // pinned, one load chunk
+  int32_t* h_map = nullptr;     // pinned [n_slots]
+  size_t stage_bytes = 0;
+  RegionTable regions{};
+  std::vector<int64_t> done, total;  // host mirror of every slot's progress
+  std::vector<uint8_t> loaded;
+  int64_t launches = 0;
+};
+
+extern "C" {
+
+static int check_pool_desc(rbim_ctx* ctx, const rbim_pool_desc* d) {
+  if (!d) return set_err(ctx, RBIM_E_INVAL, "desc is NULL");
+  if (!valid_L(d->L)) return set_err(ctx, RBIM_E_INVAL, "L must be at least 3 (got %d)", d->L);
+  if (d->n_slots < 1 || d->n_slots > (1 << 20))
+    return set_err(ctx, RBIM_E_INVAL, "n_slots must be in 1 .. 2^20 (got %d)", d->n_slots);
+  if (d->max_bins < 1) return set_err(ctx, RBIM_E_INVAL, "max_bins must be positive");
+  if (d->engine < RBIM_ENGINE_AUTO || d->engine > RBIM_ENGINE_TILED)
+    return set_err(ctx, RBIM_E_INVAL, "unknown engine %d", d->engine);
+  if (d->engine == RBIM_ENGINE_BATCHED && d->L > RBIM_LDS_MAX_L)
+    return set_err(ctx, RBIM_E_INVAL, "the LDS engine holds L <= %d (got %d)", RBIM_LDS_MAX_L, d->L);
+  if (d->tile != 0 && d->tile != 32 && d->tile != 64)
+    return set_err(ctx, RBIM_E_INVAL, "tile must be 0 (by L), 32 or 64");
+  if (d->threads != 0 && d->threads != 64 && d->threads != 128 && d->threads != 256 &&
+      d->threads != 512 && d->threads != 1024)
+    return set_err(ctx, RBIM_E_INVAL, "threads must be 64, 128, 256, 512 or 1024");
+  if (d->sweeps_per_launch < 0) return set_err(ctx, RBIM_E_INVAL, "sweeps_per_launch < 0");
+  return RBIM_OK;
+}
+
+static void pool_engine(const rbim_pool_desc* d, bool* lds, bool* fused) {
+  *lds = d->engine == RBIM_ENGINE_BATCHED || (d->engine == RBIM_ENGINE_AUTO && d->L <= RBIM_LDS_MAX_L);
+  *fused = !*lds && d->engine != RBIM_ENGINE_TILED && fused_tile_size(d->L, d->tile) > 0;
+}
+
+int rbim_pool_footprint(const rbim_pool_desc* d, uint64_t* bytes) {
+  int rc = check_pool_desc(nullptr, d);
+  if (rc != RBIM_OK) return rc;
+  bool lds, fused;
+  pool_engine(d, &lds, &fused);
+  const int N = d->L * d->L;
+  uint64_t b = group_bytes(d->L, d->n_slots, lds, fused, d->max_bins, 0, d->tile);
+  b += (uint64_t)d->n_slots * (24 + 4) + pool_stage_bytes(N, pool_chunk(N, d->n_slots));
+  if (bytes) *bytes = b;
+  return RBIM_OK;
+}
+
+int rbim_pool_create(rbim_ctx* ctx, const rbim_pool_desc* d, rbim_pool** out) {
+  if (!ctx || !out) return set_err(ctx, RBIM_E_INVAL, "NULL argument");
+  int rc = check_pool_desc(ctx, d);
+  if (rc != RBIM_OK) return rc;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  uint64_t need = 0;
+  rbim_pool_footprint(d, &need);
+  size_t fr = 0, tot = 0;
+  HIP_TRY(ctx, hipMemGetInfo(&fr, &tot));
+  if (need + (64ull << 20) > fr)
+    return set_err(ctx, RBIM_E_ADMIT, "pool needs %llu bytes, %llu free", (unsigned long long)need,
+                   (unsigned long long)fr);
+  rbim_pool* p = new rbim_pool();
+  p->ctx = ctx;
+  p->max_bins = d->max_bins;
+  Group& g = p->g;
+  g.L = d->L;
+  g.N = d->L * d->L;
+  g.n_lat = d->n_slots;
+  pool_engine(d, &g.lds, &g.fused);
+  g.tile_pref = d->tile;
+  g.threads = d->threads ? d->threads : default_threads(g.L);
+  // tiled pools launch sweep by sweep: their grids follow the loaded slots
+  g.per_launch = g.lds ? (d->sweeps_per_launch > 0 ? d->sweeps_per_launch : default_lds_sweeps(g.L)) : 1;
+  p->done.assign(d->n_slots, 0);
+  p->total.assign(d->n_slots, 0);
+  p->loaded.assign(d->n_slots, 0);
+  p->chunk = pool_chunk(g.N, d->n_slots);
+  p->stage_bytes = pool_stage_bytes(g.N, p->chunk);
+  auto fail = [&](int code) { rbim_pool_destroy(p); return code; };
+  hipError_t e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&p->staged, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&p->mapped, hipEventDisableTiming);
+  if (e != hipSuccess)
+    return fail(set_err(ctx, RBIM_E_HIP, "stream/event creation failed: %s", hipGetErrorString(e)));
+  rc = group_alloc(ctx, g, {}, {}, {}, {}, nullptr, RBIM_START_HOT, 0, 1, d->max_bins, 0, 0, true);
+  if (rc != RBIM_OK) return fail(rc);
+  void* q;
+  e = g.arena.alloc(&q, (size_t)d->n_slots * 24);
+  if (e == hipSuccess) {
+    p->d_sched = (int64_t*)q;
+    e = hipMemsetAsync(q, 0, (size_t)d->n_slots * 24, ctx->stream);  // total 0: inactive
+  }
+  if (e == hipSuccess) e = g.arena.alloc(&q, (size_t)d->n_slots * 4);
+  if (e == hipSuccess) {
+    p->d_map = (int32_t*)q;
+    e = g.arena.alloc(&q, p->stage_bytes);
+  }
+  if (e == hipSuccess) {
+    p->d_stage = (uint8_t*)q;
+    e = hipHostMalloc((void**)&p->h_stage, p->stage_bytes, hipHostMallocDefault);
+  }
+  if (e == hipSuccess) e = hipHostMalloc((void**)&p->h_map, (size_t)d->n_slots * 4, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess)
+    return fail(set_err(ctx, e == hipErrorOutOfMemory ? RBIM_E_OOM : RBIM_E_HIP,
+                        "pool allocation failed: %s", hipGetErrorString(e)));
+  g.set.sched = p->d_sched;
+  // the per-slot state a load clears
+  RegionTable& rt = p->regions;
+  const uint64_t nl = (uint64_t)d->n_slots, N = (uint64_t)g.N;
+  auto add = [&](void* base, uint64_t per_lat, int planes = 1, uint64_t plane_stride = 0) {
+    if (base && per_lat) rt.r[rt.n++] = Region{(uint8_t*)base, per_lat, plane_stride, planes};
+  };
+  add(g.set.acc, (uint64_t)d->max_bins * RBIM_ACC_SLOTS * 8);
+  add(g.set.last, sizeof(rbim_sweep_stats));
+  if (g.fused) {
+    FusedScratch& f = g.fz;
+    const uint64_t tiles = (uint64_t)f.tiles_x * f.tiles_x, ns = (uint64_t)f.nslots;
+    add(f.lab, 2 * N, 2, nl * N * 2);
+    add(f.slots, ns * 8);
+    add(f.guf, ns * 8);
+    add(f.gcnt, ns * 8);
+    add(f.gmn, ns * 4);
+    add(f.gpart, (ns + 255) / 256 * 5 * 8);
+    add(f.wrapw, 4);
+    add(f.xbits, tiles * (2 * f.tile / 64) * 8);
+    add(f.flipbits, ns / 8);
+    add(f.part_int, tiles * 5 * 8, 2, nl * tiles * 5 * 8);
+    add(f.part_em, tiles * 2 * 8);
+    add(f.bst, 6 * 8, 2, nl * 6 * 8);
+  } else if (!g.lds) {
+    TiledScratch& t = g.tile;
+    add(t.part, (uint64_t)t.tiles_x * t.tiles_y * 7 * 8);
+    add(t.wrapf, 8);
+    add(t.pending, 4);
+  }
+  *out = p;
+  return RBIM_OK;
+}
+
+int rbim_pool_load(rbim_pool* p, int32_t n, const int32_t* slots, const int8_t* Jh,
+                   const int8_t* Jv, const uint32_t* thr31, const uint64_t* seed,
+                   const uint32_t* lattice_id, const int8_t* spins0, int32_t start,
+                   int64_t n_therm, int32_t n_bins, int64_t sweeps_per_bin) {
+  if (!p) return set_err(nullptr, RBIM_E_INVAL, "pool is NULL");
+  rbim_ctx* ctx = p->ctx;
+  if (n < 1 || !slots || !Jh || !Jv || !thr31 || !seed || !lattice_id)
+    return set_err(ctx, RBIM_E_INVAL, "NULL argument or n < 1");
+  if (n_bins < 1 || n_bins > p->max_bins)
+    return set_err(ctx, RBIM_E_INVAL, "n_bins must be in 1 .. %d (got %d)", p->max_bins, n_bins);
+  if (sweeps_per_bin < 1) return set_err(ctx, RBIM_E_INVAL, "sweeps_per_bin must be at least 1");
+  if (n_therm < 0) return set_err(ctx, RBIM_E_INVAL, "n_therm must be non-negative");
+  const int64_t total = n_therm + (int64_t)n_bins * sweeps_per_bin;
+  if (total > 0xFFFFFFFFll) return set_err(ctx, RBIM_E_INVAL, "more than 2^32 sweeps per lattice");
+  const int n_slots = p->g.n_lat;
+  for (int k = 0; k < n; ++k) {
+    if (slots[k] < 0 || slots[k] >= n_slots)
+      return set_err(ctx, RBIM_E_INVAL, "slot %d outside [0, %d)", slots[k], n_slots);
+    if (p->loaded[slots[k]])
+      return set_err(ctx, RBIM_E_STATE, "slot %d is in use (release it first)", slots[k]);
+    if (thr31[k] > (1u << 31)) return set_err(ctx, RBIM_E_INVAL, "thr31 exceeds 2^31");
+  }
+  int dup = -1;  // every slot is free (0) here: mark them to find repeats
+  for (int k = 0; k < n && dup < 0; ++k) {
+    if (p->loaded[slots[k]] == 2) dup = slots[k];
+    p->loaded[slots[k]] = 2;
+  }
+  for (int k = 0; k < n; ++k) p->loaded[slots[k]] = 0;
+  if (dup >= 0) return set_err(ctx, RBIM_E_INVAL, "slot %d given twice", dup);
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const int N = p->g.N;
+  const int C = p->chunk;
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const int max_region = [&] {
+    uint64_t m = 0;
+    for (int i = 0; i < p->regions.n; ++i) m = std::max(m, p->regions.r[i].per_lat);
+    return (int)std::min<uint64_t>(64, (m / 4 + 255) / 256 + 1);
+  }();
+  for (int k0 = 0; k0 < n; k0 += C) {
+    const int m = std::min(C, n - k0);
+    // staging layout of m lattices: jbits [m][N] | spins [m][N] | thr | seed | lid | slots
+    const size_t o_sp = al((size_t)m * N), o_thr = o_sp + (spins0 ? al((size_t)m * N) : 0),
+                 o_seed = o_thr + al(4 * (size_t)m), o_lid = o_seed + al(8 * (size_t)m),
+                 o_slot = o_lid + al(4 * (size_t)m), bytes = o_slot + 4 * (size_t)m;
+    HIP_TRY(ctx, hipEventSynchronize(p->staged));  // the previous chunk's copy has run
+    uint8_t* h = p->h_stage;
+    for (int k = 0; k < m; ++k) {
+      const size_t o = (size_t)(k0 + k) * N;
+      pack_jbits(Jh + o, Jv + o, h + (size_t)k * N, N);
+      if (spins0) std::memcpy(h + o_sp + (size_t)k * N, spins0 + o, N);
+    }
+    std::memcpy(h + o_thr, thr31 + k0, 4 * (size_t)m);
+    std::memcpy(h + o_seed, seed + k0, 8 * (size_t)m);
+    std::memcpy(h + o_lid, lattice_id + k0, 4 * (size_t)m);
+    std::memcpy(h + o_slot, slots + k0, 4 * (size_t)m);
+    HIP_TRY(ctx, hipMemcpyAsync(p->d_stage, h, bytes, hipMemcpyHostToDevice, p->stream));
+    HIP_TRY(ctx, hipEventRecord(p->staged, p->stream));
+    const int32_t* d_slots = (const int32_t*)(p->d_stage + o_slot);
+    hipLaunchKernelGGL(pool_reset_kernel, dim3(max_region, m), dim3(256), 0, p->stream, p->regions,
+                       d_slots);
+    HIP_TRY(ctx, hipGetLastError());
+    PoolStage st;
+    st.jbits = p->d_stage;
+    st.spins = spins0 ? (const int8_t*)(p->d_stage + o_sp) : nullptr;
+    st.thr = (const uint32_t*)(p->d_stage + o_thr);
+    st.seed = (const uint64_t*)(p->d_stage + o_seed);
+    st.lid = (const uint32_t*)(p->d_stage + o_lid);
+    st.slots = d_slots;
+    hipLaunchKernelGGL(pool_load_kernel, dim3((N + 255) / 256, m), dim3(256), 0, p->stream, p->g.set,
+                       p->d_sched, p->g.fused ? p->g.fz.fin : nullptr,
+                       p->g.fused ? p->g.fz.cur : nullptr, st, (int)start, n_therm,
+                       sweeps_per_bin, total);
+    HIP_TRY(ctx, hipGetLastError());
+  }
+  for (int k = 0; k < n; ++k) {
+    p->loaded[slots[k]] = 1;
+    p->done[slots[k]] = 0;
+    p->total[slots[k]] = total;
+  }
+  return RBIM_OK;
+}
+
+int rbim_pool_run(rbim_pool* p, int64_t n_sweeps) {
+  if (!p) return set_err(nullptr, RBIM_E_INVAL, "pool is NULL");
+  rbim_ctx* ctx = p->ctx;
+  if (n_sweeps < 0) return set_err(ctx, RBIM_E_INVAL, "n_sweeps must be non-negative");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipEventSynchronize(p->mapped));  // h_map is free again
+  int cnt = 0;
+  int64_t longest = 0;
+  for (int i = 0; i < p->g.n_lat; ++i)
+    if (p->loaded[i] && p->done[i] < p->total[i]) {
+      p->h_map[cnt++] = i;
+      longest = std::max(longest, p->total[i] - p->done[i]);
+    }
+  const int64_t n = std::min(n_sweeps, longest);
+  if (cnt == 0 || n == 0) return RBIM_OK;
+  HIP_TRY(ctx, hipMemcpyAsync(p->d_map, p->h_map, 4 * (size_t)cnt, hipMemcpyHostToDevice, p->stream));
+  HIP_TRY(ctx, hipEventRecord(p->mapped, p->stream));
+  p->g.set.lat_map = p->d_map;
+  p->g.set.n_run = cnt;
+  const int rc = group_run(ctx, p->g, n, p->stream, &p->launches);
+  if (rc != RBIM_OK) return rc;
+  for (int k = 0; k < cnt; ++k) {
+    const int i = p->h_map[k];
+    p->done[i] = std::min(p->done[i] + n, p->total[i]);
+  }
+  return RBIM_OK;
+}
+
+int rbim_pool_query(rbim_pool* p, int32_t* done) {
+  if (!p || !done) return set_err(nullptr, RBIM_E_INVAL, "NULL argument");
+  hipError_t e = hipStreamQuery(p->stream);
+  if (e == hipSuccess) { *done = 1; return RBIM_OK; }
+  if (e == hipErrorNotReady) { *done = 0; return RBIM_OK; }
+  return set_err(p->ctx, RBIM_E_HIP, "stream query failed: %s", hipGetErrorString(e));
+}
+
+int rbim_pool_sync(rbim_pool* p) {
+  if (!p) return set_err(nullptr, RBIM_E_INVAL, "pool is NULL");
+  HIP_TRY(p->ctx, hipStreamSynchronize(p->stream));
+  return RBIM_OK;
+}
+
+int rbim_pool_progress(rbim_pool* p, int32_t n, const int32_t* slots, int64_t* done,
+                       int64_t* total) {
+  if (!p || !slots || n < 0) return set_err(nullptr, RBIM_E_INVAL, "NULL argument");
+  for (int k = 0; k < n; ++k) {
+    if (slots[k] < 0 || slots[k] >= p->g.n_lat)
+      return set_err(p->ctx, RBIM_E_INVAL, "slot %d outside [0, %d)", slots[k], p->g.n_lat);
+    if (done) done[k] = p->loaded[slots[k]] ? p->done[slots[k]] : -1;
+    if (total) total[k] = p->loaded[slots[k]] ? p->total[slots[k]] : -1;
+  }
+  return RBIM_OK;
+}
+
+int rbim_pool_read(rbim_pool* p, int32_t n, const int32_t* slots, uint64_t* moments, int8_t* spins) {
+  if (!p || !slots || n < 0) return set_err(nullptr, RBIM_E_INVAL, "NULL argument");
+  rbim_ctx* ctx = p->ctx;
+  int lo = p->g.n_lat, hi = -1;
+  for (int k = 0; k < n; ++k) {
+    if (slots[k] < 0 || slots[k] >= p->g.n_lat || !p->loaded[slots[k]])
+      return set_err(ctx, RBIM_E_INVAL, "slot %d is not loaded", slots[k]);
+    lo = std::min(lo, slots[k]);
+    hi = std::max(hi, slots[k]);
+  }
+  if (n == 0) return RBIM_OK;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipStreamSynchronize(p->stream));
+  const size_t per = (size_t)p->max_bins * RBIM_ACC_SLOTS;
+  if (moments) {
+    std::vector<uint64_t> h(per * (size_t)(hi - lo + 1));
+    HIP_TRY(ctx, hipMemcpy(h.data(), p->g.set.acc + per * lo, h.size() * 8, hipMemcpyDeviceToHost));
+    for (int k = 0; k < n; ++k) std::memcpy(moments + per * k, &h[per * (slots[k] - lo)], per * 8);
+  }
+  if (spins)
+    for (int k = 0; k < n; ++k)
+      HIP_TRY(ctx, hipMemcpy(spins + (size_t)k * p->g.N, p->g.set.spins + (size_t)slots[k] * p->g.N,
+                             p->g.N, hipMemcpyDeviceToHost));
+  return RBIM_OK;
+}
+
+int rbim_pool_release(rbim_pool* p, int32_t n, const int32_t* slots) {
+  if (!p || !slots || n < 0) return set_err(nullptr, RBIM_E_INVAL, "NULL argument");
+  for (int k = 0; k < n; ++k)
+    if (slots[k] < 0 || slots[k] >= p->g.n_lat)
+      return set_err(p->ctx, RBIM_E_INVAL, "slot %d outside [0, %d)", slots[k], p->g.n_lat);
+  for (int k = 0; k < n; ++k) p->loaded[slots[k]] = 0;
+  return RBIM_OK;
+}
+
+int rbim_pool_destroy(rbim_pool* p) {
+  if (!p) return RBIM_OK;
+  (void)hipSetDevice(p->ctx->device);
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  group_free(p->g);
+  if (p->h_stage) (void)hipHostFree(p->h_stage);
+  if (p->h_map) (void)hipHostFree(p->h_map);
+  if (p->staged) (void)hipEventDestroy(p->staged);
+  if (p->mapped) (void)hipEventDestroy(p->mapped);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
+  delete p;
+  return RBIM_OK;
+}
+
+int rbim_inject_alloc_failure(int64_t after) {
+  g_alloc_countdown.store(after < 0 ? -1 : after);
   return RBIM_OK;
 }
 

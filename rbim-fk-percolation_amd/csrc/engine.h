@@ -26,6 +26,14 @@ struct LatticeSet {
   rbim_sweep_stats* series; // [n_lat][series_cap] ring of measured sweeps, or NULL
   int64_t series_cap;
   int64_t n_therm, spb, n_bins, total;
+  // Slot pools (requests of different schedules sharing launches): per
+  // lattice n_therm, sweeps per bin and total sweeps, [n_lat][3], or NULL when
+  // the scalars above hold for every lattice (n_bins then only strides acc).
+  const int64_t* sched;
+  // Lattices a launch covers: lat_map[0 .. n_run) (slot indices), or NULL for
+  // all n_lat (n_run == n_lat).  Grids span n_run, never n_lat.
+  const int32_t* lat_map;
+  int32_t n_run;
   // given-bits mode (one-lattice parity entry points); NULL in production
   const uint8_t* given_occ; // [N] bit0 horizontal, bit1 vertical
   const uint8_t* given_flip;// [N]
@@ -99,6 +107,21 @@ int tiled_kernels_per_sweep();
 int tiled_tile_size(int L, int pref = 0);
 
 // ---------------------------------------------------------------- helpers
+// Lattice of grid index i (blockIdx of a per-lattice grid dimension).
+__device__ __forceinline__ int run_lat(const LatticeSet& s, int i) {
+  return s.lat_map ? s.lat_map[i] : i;
+}
+// Schedule of lattice lat.
+__device__ __forceinline__ int64_t lat_therm(const LatticeSet& s, int lat) {
+  return s.sched ? s.sched[3 * (size_t)lat] : s.n_therm;
+}
+__device__ __forceinline__ int64_t lat_spb(const LatticeSet& s, int lat) {
+  return s.sched ? s.sched[3 * (size_t)lat + 1] : s.spb;
+}
+__device__ __forceinline__ int64_t lat_total(const LatticeSet& s, int lat) {
+  return s.sched ? s.sched[3 * (size_t)lat + 2] : s.total;
+}
+
 __device__ __forceinline__ uint32_t site_x(uint32_t n, uint32_t magic) {
   return __umulhi(n, magic);
 }
@@ -127,7 +150,7 @@ inline hipError_t ensure_dynamic_lds(const void* kernel, int bytes, std::atomic<
 // Measured sweep gm of lattice lat into its series ring (when kept).
 __device__ __forceinline__ void series_put(const LatticeSet& s, int lat, int64_t gm,
                                            const rbim_sweep_stats& st) {
-  if (s.series) s.series[(size_t)lat * s.series_cap + (gm - s.n_therm) % s.series_cap] = st;
+  if (s.series) s.series[(size_t)lat * s.series_cap + (gm - lat_therm(s, lat)) % s.series_cap] = st;
 }
 
 __device__ __forceinline__ void acc_sweep(uint64_t* a, const rbim_sweep_stats& s) {

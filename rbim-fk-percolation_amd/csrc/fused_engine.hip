@@ -182,7 +182,7 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
   __shared__ uint8_t bits[T * T];  // bit0 spin, bit1 occ_h, bit2 occ_v, bit3 flip draw, bit4/5 J<0
   __shared__ uint8_t hsp[2 * T];
   __shared__ int64_t red[NTH / 64][PARTS];
-  const int lat = blockIdx.y;
+  const int lat = run_lat(s, blockIdx.y);
   if (!lattice_active(s, lat)) return;
   const uint32_t L = (uint32_t)s.L, N = (uint32_t)s.N;
   const uint32_t nt = (uint32_t)f.tiles_x, tiles = nt * nt;
@@ -200,8 +200,9 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
   // E, M of the state this sweep starts from close sweep g-1 (unless a run's
   // closing pass has already folded it); the clusters of sweep g are measured
   // when g is past thermalisation
-  const bool meas_em = g >= 1 && f.fin[lat] == g - 1 && g - 1 >= s.n_therm;
-  const bool meas_cl = g >= s.n_therm;
+  const int64_t n_therm = lat_therm(s, lat);
+  const bool meas_em = g >= 1 && f.fin[lat] == g - 1 && g - 1 >= n_therm;
+  const bool meas_cl = g >= n_therm;
 
   // The state this sweep starts from: spins, labels and couplings of the
   // tile, the halo sites (the column beyond x0+T-1 and the row beyond
@@ -613,7 +614,7 @@ __device__ void fused_fold(const LatticeSet& s, int lat, int64_t gm, int64_t e, 
   }
   st.wrap_x = (int32_t)(wrapw & 1);
   st.wrap_y = (int32_t)((wrapw >> 1) & 1);
-  const int64_t bin = (gm - s.n_therm) / s.spb;
+  const int64_t bin = (gm - lat_therm(s, lat)) / lat_spb(s, lat);
   acc_sweep(s.acc + ((size_t)lat * s.n_bins + bin) * RBIM_ACC_SLOTS, st);
   if (s.last) s.last[lat] = st;
   series_put(s, lat, gm, st);
@@ -717,7 +718,7 @@ __global__ __launch_bounds__(KB_THREADS) void kb_merge(LatticeSet s, FusedScratc
   extern __shared__ uint64_t kbsm[];
   __shared__ int64_t red[KB_THREADS / 64][PARTS];
   __shared__ uint32_t wrapbits;
-  const int lat = blockIdx.x;
+  const int lat = run_lat(s, blockIdx.x);
   if (!lattice_active(s, lat)) return;
   const uint32_t ns = (uint32_t)f.nslots, nt = (uint32_t)f.tiles_x, L = (uint32_t)s.L;
   uint64_t* uf = kbsm;                                 // parent | wx:16 | wy:16
@@ -731,7 +732,7 @@ __global__ __launch_bounds__(KB_THREADS) void kb_merge(LatticeSet s, FusedScratc
   // sweep g-1 is complete (its E, M came from this sweep's KA; its interior
   // clusters, edge clusters and wraps sit in the other parity's partials)
   if (tid == KB_WORK && g >= 1 && f.fin[lat] == g - 1) {
-    if (g - 1 >= s.n_therm) fused_finalize(s, f, lat, g - 1);
+    if (g - 1 >= lat_therm(s, lat)) fused_finalize(s, f, lat, g - 1);
     f.fin[lat] = g;
   }
   for (uint32_t k = tid; worker && k < ns; k += KB_WORK) {
@@ -784,7 +785,7 @@ __global__ __launch_bounds__(KB_THREADS) void kb_merge(LatticeSet s, FusedScratc
   }
   __syncthreads();
   // flip bits per slot; merged clusters measured at their roots
-  const bool meas_cl = g >= s.n_therm;
+  const bool meas_cl = g >= lat_therm(s, lat);
   TileMeas ms = {0, 0, 0, 0ull, 0ull, 0u, 0u};
   uint32_t* fbo = f.flipbits + (size_t)lat * (ns / 32u);
   for (uint32_t k0 = 0; worker && k0 < ns; k0 += KB_WORK) {
@@ -839,7 +840,7 @@ __global__ __launch_bounds__(KB_THREADS) void kb_merge(LatticeSet s, FusedScratc
 template <int T>
 __global__ __launch_bounds__(256) void kg_union(LatticeSet s, FusedScratch f) {
   constexpr uint32_t NS = 4u * T;
-  const int lat = blockIdx.y;
+  const int lat = run_lat(s, blockIdx.y);
   if (!lattice_active(s, lat)) return;
   const uint32_t ns = (uint32_t)f.nslots, nt = (uint32_t)f.tiles_x;
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -884,7 +885,7 @@ __global__ __launch_bounds__(256) void kg_union(LatticeSet s, FusedScratch f) {
 
 // Reps fold their counts and (min site << 1 | flip) into their roots.
 __global__ __launch_bounds__(256) void kg_fold(LatticeSet s, FusedScratch f) {
-  const int lat = blockIdx.y;
+  const int lat = run_lat(s, blockIdx.y);
   if (!lattice_active(s, lat)) return;
   const uint32_t ns = (uint32_t)f.nslots;
   // ns is a multiple of 128: a wavefront's 64 slots are all in or all out
@@ -926,14 +927,14 @@ __global__ __launch_bounds__(256) void kg_fold(LatticeSet s, FusedScratch f) {
 // into per-block partials (n, key1, key2, n_plus1, n_plus2).
 __global__ __launch_bounds__(256) void kg_flip(LatticeSet s, FusedScratch f) {
   __shared__ int64_t red[256 / 64][PARTS];
-  const int lat = blockIdx.y;
+  const int lat = run_lat(s, blockIdx.y);
   if (!lattice_active(s, lat)) return;
   const uint32_t ns = (uint32_t)f.nslots;
   // ns is a multiple of 128: a wavefront's 64 slots are all in or all out
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = k < ns;
   const size_t base = (size_t)lat * ns;
-  const bool meas_cl = s.sweep_ctr[lat] >= s.n_therm;
+  const bool meas_cl = s.sweep_ctr[lat] >= lat_therm(s, lat);
   uint32_t r = 0u, m = 0u;
   if (valid) {
     r = groot(f.guf + base, k);
@@ -981,11 +982,11 @@ constexpr int KG_END_THREADS = 256;
 __global__ __launch_bounds__(KG_END_THREADS) void kg_sweep_end(LatticeSet s, FusedScratch f,
                                                                int nblocks) {
   __shared__ int64_t red[KG_END_THREADS / 64][PARTS];
-  const int lat = blockIdx.x;
+  const int lat = run_lat(s, blockIdx.x);
   if (!lattice_active(s, lat)) return;
   const int tid = threadIdx.x;
   const int64_t g = s.sweep_ctr[lat];
-  if (g >= s.n_therm) {
+  if (g >= lat_therm(s, lat)) {
     int64_t e = 0, m = 0, ncl = 0;
     TileMeas a = {0, 0, 0, 0ull, 0ull, 0u, 0u};
     const int64_t* p = f.gpart + (size_t)lat * nblocks * 5;
@@ -1007,7 +1008,7 @@ __global__ __launch_bounds__(KG_END_THREADS) void kg_sweep_end(LatticeSet s, Fus
     __syncthreads();  // red is reused below
   }
   const bool fold = g >= 1 && f.fin[lat] == g - 1;
-  if (fold && g - 1 >= s.n_therm) fused_finalize_block<KG_END_THREADS>(s, f, lat, g - 1, red);
+  if (fold && g - 1 >= lat_therm(s, lat)) fused_finalize_block<KG_END_THREADS>(s, f, lat, g - 1, red);
   __syncthreads();  // every thread has read the counters it updates
   if (tid == 0) {
     f.wrapw[lat] = 0u;
@@ -1026,9 +1027,9 @@ __global__ __launch_bounds__(NTH) void kf_measure(LatticeSet s, FusedScratch f) 
   __shared__ uint8_t hsp[2 * T];
   __shared__ uint32_t fb_lds[fb_words<BIG>()];
   __shared__ int64_t red[NTH / 64][PARTS];
-  const int lat = blockIdx.y;
+  const int lat = run_lat(s, blockIdx.y);
   const int64_t G = s.sweep_ctr[lat] - 1;
-  if (!(G >= 0 && f.fin[lat] == G && G >= s.n_therm)) return;
+  if (!(G >= 0 && f.fin[lat] == G && G >= lat_therm(s, lat))) return;
   const uint32_t L = (uint32_t)s.L, N = (uint32_t)s.N;
   const uint32_t nt = (uint32_t)f.tiles_x, tiles = nt * nt;
   const uint32_t tile = blockIdx.x, tx = tile % nt, ty = tile / nt;
@@ -1071,7 +1072,7 @@ __global__ __launch_bounds__(NTH) void kf_measure(LatticeSet s, FusedScratch f) 
 template <int T, int NTH, bool BIG>
 __global__ __launch_bounds__(NTH) void kf_settle(LatticeSet s, FusedScratch f) {
   __shared__ uint32_t fb_lds[fb_words<BIG>()];
-  const int lat = blockIdx.y;
+  const int lat = run_lat(s, blockIdx.y);
   const uint32_t L = (uint32_t)s.L, N = (uint32_t)s.N;
   const uint32_t nt = (uint32_t)f.tiles_x;
   const uint32_t tile = blockIdx.x, tx = tile % nt, ty = tile / nt;
@@ -1090,11 +1091,12 @@ __global__ __launch_bounds__(NTH) void kf_settle(LatticeSet s, FusedScratch f) {
 }
 
 __global__ void kf_close(LatticeSet s, FusedScratch f) {
-  const int lat = blockIdx.x * blockDim.x + threadIdx.x;
-  if (lat >= s.n_lat) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= s.n_run) return;
+  const int lat = run_lat(s, i);
   const int64_t G = s.sweep_ctr[lat] - 1;
   if (G >= 0 && f.fin[lat] == G) {
-    if (G >= s.n_therm) fused_finalize(s, f, lat, G);
+    if (G >= lat_therm(s, lat)) fused_finalize(s, f, lat, G);
     f.fin[lat] = G + 1;
   }
   f.cur[lat] = 0;
@@ -1108,19 +1110,19 @@ hipError_t launch_fused_t(const LatticeSet& s, const FusedScratch& f, hipStream_
   const int tiles = f.tiles_x * f.tiles_x;
   if (f.big) {
     const int nb = (f.nslots + 255) / 256;
-    hipLaunchKernelGGL((ka_tile<T, NTH, true>), dim3(tiles, s.n_lat), dim3(NTH), 0, st, s, f);
-    hipLaunchKernelGGL((kg_union<T>), dim3((f.nslots / 2 + 255) / 256, s.n_lat), dim3(256), 0, st, s, f);
-    hipLaunchKernelGGL(kg_fold, dim3(nb, s.n_lat), dim3(256), 0, st, s, f);
-    hipLaunchKernelGGL(kg_flip, dim3(nb, s.n_lat), dim3(256), 0, st, s, f);
-    hipLaunchKernelGGL(kg_sweep_end, dim3(s.n_lat), dim3(KG_END_THREADS), 0, st, s, f, nb);
+    hipLaunchKernelGGL((ka_tile<T, NTH, true>), dim3(tiles, s.n_run), dim3(NTH), 0, st, s, f);
+    hipLaunchKernelGGL((kg_union<T>), dim3((f.nslots / 2 + 255) / 256, s.n_run), dim3(256), 0, st, s, f);
+    hipLaunchKernelGGL(kg_fold, dim3(nb, s.n_run), dim3(256), 0, st, s, f);
+    hipLaunchKernelGGL(kg_flip, dim3(nb, s.n_run), dim3(256), 0, st, s, f);
+    hipLaunchKernelGGL(kg_sweep_end, dim3(s.n_run), dim3(KG_END_THREADS), 0, st, s, f, nb);
     return hipGetLastError();
   }
   static std::atomic<uint64_t> configured{0};
   hipError_t e = ensure_dynamic_lds((const void*)kb_merge<T>,
                                     (int)kb_lds_bytes(RBIM_FUSED_MAX_SLOTS), configured);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((ka_tile<T, NTH, false>), dim3(tiles, s.n_lat), dim3(NTH), 0, st, s, f);
-  hipLaunchKernelGGL((kb_merge<T>), dim3(s.n_lat), dim3(KB_THREADS), kb_lds_bytes(f.nslots), st, s, f);
+  hipLaunchKernelGGL((ka_tile<T, NTH, false>), dim3(tiles, s.n_run), dim3(NTH), 0, st, s, f);
+  hipLaunchKernelGGL((kb_merge<T>), dim3(s.n_run), dim3(KB_THREADS), kb_lds_bytes(f.nslots), st, s, f);
   return hipGetLastError();
 }
 
@@ -1128,9 +1130,9 @@ template <int T, bool BIG>
 hipError_t launch_fused_close_tb(const LatticeSet& s, const FusedScratch& f, hipStream_t st) {
   constexpr int NTH = T == 64 ? 512 : 256;
   const int tiles = f.tiles_x * f.tiles_x;
-  hipLaunchKernelGGL((kf_measure<T, NTH, BIG>), dim3(tiles, s.n_lat), dim3(NTH), 0, st, s, f);
-  hipLaunchKernelGGL((kf_settle<T, NTH, BIG>), dim3(tiles, s.n_lat), dim3(NTH), 0, st, s, f);
-  hipLaunchKernelGGL(kf_close, dim3((s.n_lat + 63) / 64), dim3(64), 0, st, s, f);
+  hipLaunchKernelGGL((kf_measure<T, NTH, BIG>), dim3(tiles, s.n_run), dim3(NTH), 0, st, s, f);
+  hipLaunchKernelGGL((kf_settle<T, NTH, BIG>), dim3(tiles, s.n_run), dim3(NTH), 0, st, s, f);
+  hipLaunchKernelGGL(kf_close, dim3((s.n_run + 63) / 64), dim3(64), 0, st, s, f);
   return hipGetLastError();
 }
 

@@ -260,7 +260,11 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
   uint32_t* cnt = smem + N;
   uint8_t* sb = reinterpret_cast<uint8_t*>(smem + 2 * N);
 
-  const int lat = blockIdx.x;
+  const int lat = run_lat(s, blockIdx.x);
+  const int64_t g0 = s.sweep_ctr[lat];
+  const int64_t gend = min(g0 + (int64_t)sweeps, lat_total(s, lat));
+  if (gend <= g0) return;  // finished (or empty) slot: its state stays as it is
+  const int64_t n_therm = lat_therm(s, lat), spb = lat_spb(s, lat);
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -281,8 +285,6 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
     wrapflag[1] = 0u;
   }
 
-  const int64_t g0 = s.sweep_ctr[lat];
-  const int64_t gend = min(g0 + (int64_t)sweeps, s.total);
   const uint32_t thr = s.thr31[lat];
   const uint64_t seed = s.seed[lat];
   const uint32_t lid = s.lat_id[lat];
@@ -357,12 +359,12 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
     st.wrap_x = (int32_t)(wf & 1u);
     st.wrap_y = (int32_t)((wf >> 1) & 1u);
     if (cur_bin < 0 || gm >= bin_end) {  // measured sweeps arrive in order
-      const int64_t bin = (gm - s.n_therm) / s.spb;
+      const int64_t bin = (gm - n_therm) / spb;
       if (cur_bin >= 0)
         for (int k = 0; k < RBIM_ACC_SLOTS; ++k) gacc[cur_bin * RBIM_ACC_SLOTS + k] = a[k];
       for (int k = 0; k < RBIM_ACC_SLOTS; ++k) a[k] = gacc[bin * RBIM_ACC_SLOTS + k];
       cur_bin = bin;
-      bin_end = s.n_therm + (bin + 1) * s.spb;
+      bin_end = n_therm + (bin + 1) * spb;
     }
     acc_sweep(a, st);
     if (s.last) s.last[lat] = st;
@@ -565,7 +567,7 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
       }
     }
     __syncthreads();
-    pending = g >= s.n_therm;
+    pending = g >= n_therm;
     // an unmeasured sweep's wrap bits are dropped; its slot is reused two
     // sweeps later, after two more barriers
     if (!pending && tid == 0) wrapflag[g & 1] = 0u;
@@ -588,7 +590,7 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
 
   if (tid == fold_tid && cur_bin >= 0)
     for (int k = 0; k < RBIM_ACC_SLOTS; ++k) gacc[cur_bin * RBIM_ACC_SLOTS + k] = a[k];
-  if (tid == 0) s.sweep_ctr[lat] = gend > g0 ? gend : g0;
+  if (tid == 0) s.sweep_ctr[lat] = gend;
   for (uint32_t n = tid; n < N; n += NT) gsp[n] = (sb[n] & 1u) ? (int8_t)1 : (int8_t)-1;
 }
 
@@ -606,7 +608,7 @@ hipError_t launch_lds_t(const LatticeSet& s, int sweeps, hipStream_t st) {
   hipError_t e = ensure_dynamic_lds((const void*)lds_sweep_kernel<NT, LC>,
                                     (int)lds_bytes(RBIM_LDS_MAX_N), configured);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((lds_sweep_kernel<NT, LC>), dim3(s.n_lat), dim3(NT), lds_bytes(s.N), st, s,
+  hipLaunchKernelGGL((lds_sweep_kernel<NT, LC>), dim3(s.n_run), dim3(NT), lds_bytes(s.N), st, s,
                      sweeps);
   return hipGetLastError();
 }

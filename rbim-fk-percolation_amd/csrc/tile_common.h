@@ -98,7 +98,7 @@ __device__ __forceinline__ void wave_or_wraps(P* flag, uint32_t w) {
 }
 
 __device__ __forceinline__ bool lattice_active(const LatticeSet& s, int lat) {
-  return s.sweep_ctr[lat] < s.total;
+  return s.sweep_ctr[lat] < lat_total(s, lat);
 }
 
 // Per-tile partial of one sweep's measurement: E, M, n_clusters and the two

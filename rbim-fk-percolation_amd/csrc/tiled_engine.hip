@@ -111,7 +111,7 @@ __global__ __launch_bounds__(NTH) void k1_tile(LatticeSet s, TiledScratch t) {
   __shared__ uint32_t par[T * T];
   __shared__ uint32_t lcnt[T * T];
   __shared__ uint8_t bits[T * T];  // bit0 spin, bit1 occ_h, bit2 occ_v
-  const int lat = blockIdx.y;
+  const int lat = run_lat(s, blockIdx.y);
   if (!lattice_active(s, lat)) return;
   const int L = s.L, N = s.N;
   const int tx = blockIdx.x % t.tiles_x, ty = blockIdx.x / t.tiles_x;
@@ -218,7 +218,7 @@ __global__ __launch_bounds__(NTH, 8) void k1_tile_full(LatticeSet s, TiledScratc
   __shared__ uint32_t par[T * T];
   __shared__ uint32_t lcnt[T * T];
   __shared__ uint8_t bits[T * T];  // bit0 spin, bit1 occ_h, bit2 occ_v
-  const int lat = blockIdx.y;
+  const int lat = run_lat(s, blockIdx.y);
   if (!lattice_active(s, lat)) return;
   const uint32_t L = (uint32_t)s.L, N = (uint32_t)s.N;
   const uint32_t tx = blockIdx.x % t.tiles_x, ty = blockIdx.x / t.tiles_x;
@@ -383,7 +383,7 @@ __global__ __launch_bounds__(NTH, 8) void k1_tile_full(LatticeSet s, TiledScratc
 // ----------------------------------------------------------------- K2
 template <int T>
 __global__ void k2_merge(LatticeSet s, TiledScratch t) {
-  const int lat = blockIdx.y;
+  const int lat = run_lat(s, blockIdx.y);
   if (!lattice_active(s, lat)) return;
   const int L = s.L;
   const int nh = t.tiles_x * L, nv = t.tiles_y * L;
@@ -448,7 +448,7 @@ template <int T, int NTH>
 __global__ __launch_bounds__(NTH) void k3_flip(LatticeSet s, TiledScratch t) {
   __shared__ uint32_t rmap[T * T];  // global root | flip << 31, at local roots
   constexpr int PER = T * T / NTH;
-  const int lat = blockIdx.y;
+  const int lat = run_lat(s, blockIdx.y);
   if (!lattice_active(s, lat)) return;
   const uint32_t L = (uint32_t)s.L, N = (uint32_t)s.N;
   const uint32_t tx = blockIdx.x % t.tiles_x, ty = blockIdx.x / t.tiles_x;
@@ -517,7 +517,7 @@ __global__ __launch_bounds__(NTH) void k4_measure(LatticeSet s, TiledScratch t) 
   // measures the settled state after the last sweep of a run (earlier sweeps
   // are measured inside the next sweep's K1)
   __shared__ int64_t red[NTH / 64][PARTS];
-  const int lat = blockIdx.y;
+  const int lat = run_lat(s, blockIdx.y);
   if (t.pending[lat] == 0) return;
   const int L = s.L, N = s.N;
   const int tx = blockIdx.x % t.tiles_x, ty = blockIdx.x / t.tiles_x;
@@ -574,7 +574,7 @@ __device__ void finalize_sweep(const LatticeSet& s, const TiledScratch& t, int l
   st.wrap_x = (int32_t)(*wf & 1u);
   st.wrap_y = (int32_t)((*wf >> 1) & 1u);
   *wf = 0u;
-  const int64_t bin = (gm - s.n_therm) / s.spb;
+  const int64_t bin = (gm - lat_therm(s, lat)) / lat_spb(s, lat);
   acc_sweep(s.acc + ((size_t)lat * s.n_bins + bin) * RBIM_ACC_SLOTS, st);
   if (s.last) s.last[lat] = st;
   series_put(s, lat, gm, st);
@@ -583,12 +583,13 @@ __device__ void finalize_sweep(const LatticeSet& s, const TiledScratch& t, int l
 // After K1..K3 of sweep g: finalize sweep g-1 (measured in K1 of g), mark
 // sweep g pending (or drop its wrap bits) and advance the counter.
 __global__ void k5_finalize(LatticeSet s, TiledScratch t) {
-  const int lat = blockIdx.x * blockDim.x + threadIdx.x;
-  if (lat >= s.n_lat) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= s.n_run) return;
+  const int lat = run_lat(s, i);
   const int64_t g = s.sweep_ctr[lat];
-  if (g >= s.total) return;
+  if (g >= lat_total(s, lat)) return;
   if (t.pending[lat]) finalize_sweep(s, t, lat, g - 1);
-  const bool measured = g >= s.n_therm;
+  const bool measured = g >= lat_therm(s, lat);
   t.pending[lat] = measured ? 1 : 0;
   if (!measured) t.wrapf[2 * lat + (g & 1)] = 0u;
   s.sweep_ctr[lat] = g + 1;
@@ -596,8 +597,10 @@ __global__ void k5_finalize(LatticeSet s, TiledScratch t) {
 
 // End of a run: finalize the pending sweep measured by k4_measure.
 __global__ void k5_flush(LatticeSet s, TiledScratch t) {
-  const int lat = blockIdx.x * blockDim.x + threadIdx.x;
-  if (lat >= s.n_lat || t.pending[lat] == 0) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= s.n_run) return;
+  const int lat = run_lat(s, i);
+  if (t.pending[lat] == 0) return;
   finalize_sweep(s, t, lat, s.sweep_ctr[lat] - 1);
   t.pending[lat] = 0;
 }
@@ -678,20 +681,20 @@ template <int T, int NTH>
 hipError_t launch_tiled_t(const LatticeSet& s, const TiledScratch& t, hipStream_t st) {
   const int ntiles = t.tiles_x * t.tiles_y;
   if (s.L % T == 0)
-    hipLaunchKernelGGL((k1_tile_full<T, NTH>), dim3(ntiles, s.n_lat), dim3(NTH), 0, st, s, t);
+    hipLaunchKernelGGL((k1_tile_full<T, NTH>), dim3(ntiles, s.n_run), dim3(NTH), 0, st, s, t);
   else
-    hipLaunchKernelGGL((k1_tile<T, NTH>), dim3(ntiles, s.n_lat), dim3(NTH), 0, st, s, t);
+    hipLaunchKernelGGL((k1_tile<T, NTH>), dim3(ntiles, s.n_run), dim3(NTH), 0, st, s, t);
   const int nb = (t.tiles_x + t.tiles_y) * s.L;
-  hipLaunchKernelGGL((k2_merge<T>), dim3((nb + 255) / 256, s.n_lat), dim3(256), 0, st, s, t);
-  hipLaunchKernelGGL((k3_flip<T, NTH>), dim3(ntiles, s.n_lat), dim3(NTH), 0, st, s, t);
-  hipLaunchKernelGGL(k5_finalize, dim3((s.n_lat + 63) / 64), dim3(64), 0, st, s, t);
+  hipLaunchKernelGGL((k2_merge<T>), dim3((nb + 255) / 256, s.n_run), dim3(256), 0, st, s, t);
+  hipLaunchKernelGGL((k3_flip<T, NTH>), dim3(ntiles, s.n_run), dim3(NTH), 0, st, s, t);
+  hipLaunchKernelGGL(k5_finalize, dim3((s.n_run + 63) / 64), dim3(64), 0, st, s, t);
   return hipGetLastError();
 }
 
 template <int T, int NTH>
 hipError_t launch_flush_t(const LatticeSet& s, const TiledScratch& t, hipStream_t st) {
-  hipLaunchKernelGGL((k4_measure<T, NTH>), dim3(t.tiles_x * t.tiles_y, s.n_lat), dim3(NTH), 0, st, s, t);
-  hipLaunchKernelGGL(k5_flush, dim3((s.n_lat + 63) / 64), dim3(64), 0, st, s, t);
+  hipLaunchKernelGGL((k4_measure<T, NTH>), dim3(t.tiles_x * t.tiles_y, s.n_run), dim3(NTH), 0, st, s, t);
+  hipLaunchKernelGGL(k5_flush, dim3((s.n_run + 63) / 64), dim3(64), 0, st, s, t);
   return hipGetLastError();
 }
 

@@ -26,6 +26,7 @@ __all__ = [
     "SweepStats",
     "BatchDesc",
     "BatchInfo",
+    "PoolDesc",
     "library_path",
     "lib",
     "context",
@@ -99,6 +100,18 @@ class BatchDesc(ctypes.Structure):
     ]
 
 
+class PoolDesc(ctypes.Structure):
+    _fields_ = [
+        ("L", ctypes.c_int32),
+        ("n_slots", ctypes.c_int32),
+        ("max_bins", ctypes.c_int32),
+        ("engine", ctypes.c_int32),
+        ("threads", ctypes.c_int32),
+        ("sweeps_per_launch", ctypes.c_int32),
+        ("tile", ctypes.c_int32),
+    ]
+
+
 class BatchInfo(ctypes.Structure):
     _fields_ = [
         ("n_groups", ctypes.c_int32),
@@ -140,6 +153,18 @@ SIGNATURES = {
     "rbim_batch_info_get": ([_P, _P], ctypes.c_int),
     "rbim_batch_series": ([_P, _P, _P], ctypes.c_int),
     "rbim_batch_destroy": ([_P], ctypes.c_int),
+    "rbim_pool_footprint": ([_P, _P], ctypes.c_int),
+    "rbim_pool_create": ([_P, _P, ctypes.POINTER(_P)], ctypes.c_int),
+    "rbim_pool_load": ([_P, _i32, _P, _P, _P, _P, _P, _P, _P, _i32, _i64, _i32, _i64],
+                       ctypes.c_int),
+    "rbim_pool_run": ([_P, _i64], ctypes.c_int),
+    "rbim_pool_query": ([_P, _P], ctypes.c_int),
+    "rbim_pool_sync": ([_P], ctypes.c_int),
+    "rbim_pool_progress": ([_P, _i32, _P, _P, _P], ctypes.c_int),
+    "rbim_pool_read": ([_P, _i32, _P, _P, _P], ctypes.c_int),
+    "rbim_pool_release": ([_P, _i32, _P], ctypes.c_int),
+    "rbim_pool_destroy": ([_P], ctypes.c_int),
+    "rbim_inject_alloc_failure": ([_i64], ctypes.c_int),
 }
 
 _lib = None

@@ -34,7 +34,7 @@ def test_library_exports_every_declared_symbol():
 def test_abi_version():
     from rbim import _hip
 
-    assert _hip.lib().rbim_abi_version() == 2
+    assert _hip.lib().rbim_abi_version() == 3
 
 
 def test_descriptor_validation_on_host():
