@@ -109,12 +109,12 @@ def run_schedule(K, therm_fraction=11):
 
 # ------------------------------------------------------------- GPU timing
 def gpu_batched(specs, K, W, engine=0, sweeps_per_launch=0, threads=0, streams=0, tile=0,
-                therm_fraction=11):
+                therm_fraction=11, single_launch=False):
     from rbim.engine import Batch
 
     n_therm, n_bins, spb = run_schedule(K, therm_fraction)
     kw = dict(engine=engine, sweeps_per_launch=sweeps_per_launch, threads=threads,
-              streams=streams, tile=tile)
+              streams=streams, tile=tile, single_launch=single_launch)
     if W > 0:
         wn, wb, ws = run_schedule(W)
         with Batch(specs, n_therm=wn, n_bins=wb, sweeps_per_bin=ws, **kw) as warm:
@@ -129,7 +129,7 @@ def gpu_batched(specs, K, W, engine=0, sweeps_per_launch=0, threads=0, streams=0
     return {"ms": ms, "wall_s": wall, "site_updates": sites * K, "sites": sites,
             "launches": int(info.kernel_launches), "threads": int(info.threads),
             "sweeps_per_launch": int(info.sweeps_per_launch), "groups": int(info.n_groups),
-            "checksum": int(acc[:, :, 17].sum())}
+            "checksum": int(np.bitwise_xor.reduce(acc.reshape(-1) * np.uint64(0x9E3779B97F4A7C15)))}
 
 
 def gpu_plain(spec, K):
@@ -370,20 +370,38 @@ def measure_cfg1(args):
 
 
 def measure_cfg3(args):
+    """The ragged scan as one launch per L (each on its own stream) and as one
+    launch over every L (lds_ragged_kernel: a page of lattices of one L per
+    1024-thread workgroup)."""
     specs = workload_specs("ragged")
-    gpu_batched(specs, 20, 0)  # load the kernels of every L
-    res = gpu_batched(specs, 11000, 0, therm_fraction=11)
-    sites = res["sites"]
-    by_L = {L: sum(s.L * s.L for s in specs if s.L == L) for L in (16, 32, 64, 128)}
-    return {
+    out = {
         "workload": "ragged scan: L in {16,32,64,128} x p in {0,0.05,0.10,0.12} x 24 T in "
                     "linspace(1.4,2.8) x 16 realizations = 6144 lattices, 1000 + 10000 sweeps",
-        "lattices": len(specs), "value": res["site_updates"] / (res["ms"] / 1e3),
-        "unit": "site-updates/s", "wall_s": res["ms"] / 1e3, "groups": res["groups"],
-        "site_share": {str(L): by_L[L] / sites for L in by_L},
-        "roofline": roofline("lds_sweep_kernel (every L, concurrent groups)", sites * 11000,
-                             res["ms"], 1, "lds_sweep_kernel_ragged.md"),
+        "lattices": len(specs), "unit": "site-updates/s",
     }
+    by_L = {L: sum(s.L * s.L for s in specs if s.L == L) for L in (16, 32, 64, 128)}
+    for single in (False, True):
+        gpu_batched(specs, 20, 0, single_launch=single)  # load the kernels
+        res = gpu_batched(specs, 11000, 0, therm_fraction=11, single_launch=single)
+        launches = max(1, res["launches"])
+        sites = res["sites"]
+        item = {
+            "value": res["site_updates"] / (res["ms"] / 1e3), "wall_s": res["ms"] / 1e3,
+            "groups": res["groups"], "launches": launches, "checksum": res["checksum"],
+            "roofline": (roofline("lds_ragged_kernel (every L in one launch)",
+                                  sites * 11000 / launches, res["ms"] / launches, launches,
+                                  "lds_ragged_kernel_ragged_scan.md")
+                         if single else
+                         roofline("lds_sweep_kernel (one launch sequence per L, concurrent)",
+                                  sites * 11000, res["ms"], 1, "lds_sweep_kernel_ragged_scan.md")),
+        }
+        out["single_launch" if single else "per_L_streams"] = item
+    out["site_share"] = {str(L): by_L[L] / sum(by_L.values()) for L in by_L}
+    out["moments_identical"] = out["single_launch"]["checksum"] == out["per_L_streams"]["checksum"]
+    best = max(("per_L_streams", "single_launch"), key=lambda k: out[k]["value"])
+    out["value"] = out[best]["value"]
+    out["default"] = "per_L_streams"
+    return out
 
 
 def measure_cfg4(args):
@@ -474,9 +492,9 @@ def measure_cpu_anchor(args):
     return out
 
 
-def measure_workload(name, steps):
+def measure_workload(name, steps, single_launch=False):
     specs = workload_specs(name)
-    res = gpu_batched(specs, steps, 0)
+    res = gpu_batched(specs, steps, 0, single_launch=single_launch)
     value = res["site_updates"] / (res["ms"] / 1e3)
     return {"workload": name, "lattices": len(specs), "steps": steps, "value": value,
             "unit": "site-updates/s", "ms_per_step": res["ms"] / steps,
@@ -516,6 +534,8 @@ def main():
                     help="sections to run (default: all)")
     ap.add_argument("--workload", choices=["ragged", "large"], default=None,
                     help="only time the ragged (L 16..128 mix) or large (L 512, 1024) scan")
+    ap.add_argument("--single-launch", action="store_true",
+                    help="with --workload ragged: every L in one launch")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (rocprof counter "
                          "calibration) and exit")
@@ -532,7 +552,7 @@ def main():
         print(json.dumps({"copy_bytes_per_dispatch": 4 * n, "dispatches": 3}))
         return
     if args.workload:
-        print(json.dumps(measure_workload(args.workload, args.steps)), flush=True)
+        print(json.dumps(measure_workload(args.workload, args.steps, args.single_launch)), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU (asked for --gpus {args.gpus}): the lattices are "

@@ -114,17 +114,24 @@ typedef struct {
   int32_t threads;            /* LDS kernel workgroup size (64..1024); 0 -> default:
                                  per L, doubled for L <= 32 while the batch would give
                                  the GPU's SIMDs fewer than four wavefronts each    */
-  int32_t streams;            /* streams each set of lattices of one L is split
-                                 over (every distinct L also gets its own stream);
-                                 0 -> default: 2 for tiled sets (L > 128), and for
-                                 LDS sets that fill the GPU more than once with a
-                                 partial last round; else 1                       */
+  int32_t streams;            /* parts each set of lattices of one L is split into
+                                 (parts run concurrently, on at most 4 streams per
+                                 batch -- the device's hardware queues -- the largest
+                                 parts first on streams of their own); 0 -> default:
+                                 2 for tiled sets (L > 128), and for LDS sets that
+                                 fill the GPU more than once with a partial last
+                                 round; else 1                                    */
   int32_t series;             /* > 0: keep the per-sweep stats of the last `series`
                                  measured sweeps of every lattice in a device ring
                                  (time series for autocorrelation / equilibration
                                  checks); 0 -> off                                */
   int32_t tile;               /* tile side of the tiled engines: 32 or 64 (used when
                                  it divides L); 0 -> 64 when it divides L, else 32 */
+  int32_t single_launch;      /* 1: the LDS-resident lattices (every side one of
+                                 16, 32, 64, 128) run in one launch of 1024-thread
+                                 workgroups, each a page of lattices of one side
+                                 (1 of L = 128, 2 of 64, 4 of 32, 16 of 16); 0 -> one
+                                 launch sequence per L, each on its own stream    */
 } rbim_batch_desc;
 
 typedef struct {

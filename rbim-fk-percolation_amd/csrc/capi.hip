@@ -400,6 +400,8 @@ void group_free(Group& g) {
 // kernels are alternately ALU-bound (activation + tile labelling) and
 // latency-bound (cross-tile merge, flip), so two halves interleave on the CUs.
 constexpr int kTiledStreams = 2;
+// Streams of one batch at most: the hardware queues a process gets by default.
+constexpr int kMaxStreams = 4;
 
 bool valid_L(int L) { return L >= 3 && L <= 16384; }
 
@@ -696,6 +698,10 @@ size_t pool_stage_bytes(int N, int chunk) {
 struct rbim_batch {
   rbim_ctx* ctx = nullptr;
   hipStream_t stream = nullptr;
+  std::vector<hipStream_t> streams;  // the groups' streams (groups may share one)
+  bool ragged = false;               // LDS groups run in one launch (lds_ragged_kernel)
+  int ragged_lead = -1;              // the LDS group whose turn queues that launch
+  int ragged_per_launch = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, fork = nullptr;
   std::vector<Group> groups;
   int n_lat = 0;
@@ -944,6 +950,8 @@ static int check_desc(rbim_ctx* ctx, const rbim_batch_desc* d) {
     return set_err(ctx, RBIM_E_INVAL, "tile must be 0 (by L), 32 or 64");
   if (d->streams < 0 || d->streams > 16)
     return set_err(ctx, RBIM_E_INVAL, "streams must be 0 (default) .. 16");
+  if (d->single_launch != 0 && d->single_launch != 1)
+    return set_err(ctx, RBIM_E_INVAL, "single_launch must be 0 or 1");
   if (d->threads != 0 && d->threads != 64 && d->threads != 128 && d->threads != 256 &&
       d->threads != 512 && d->threads != 1024)
     return set_err(ctx, RBIM_E_INVAL, "threads must be 64, 128, 256, 512 or 1024");
@@ -1035,6 +1043,27 @@ int rbim_batch_create(rbim_ctx* ctx, const rbim_batch_desc* d, rbim_batch** out)
   int n_cu = 0;
   (void)hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device);
   if (d->threads == 0) default_lds_threads(b->groups, n_cu);
+  // one launch for every LDS set when asked for and every LDS side is one of
+  // the ragged kernel's (its page shapes fix the workgroup sizes)
+  b->ragged = d->single_launch != 0;
+  int lds_sets = 0;
+  for (Group& g : b->groups)
+    if (g.lds) {
+      ++lds_sets;
+      if (ragged_slot(g.L) < 0) b->ragged = false;
+    }
+  b->ragged = b->ragged && lds_sets > 0;
+  if (b->ragged) {
+    b->ragged_per_launch = d->sweeps_per_launch;
+    for (Group& g : b->groups)
+      if (g.lds) {
+        g.threads = ragged_threads(g.L);
+        if (d->sweeps_per_launch <= 0)
+          b->ragged_per_launch = b->ragged_per_launch > 0
+                                     ? std::min(b->ragged_per_launch, default_lds_sweeps(g.L))
+                                     : default_lds_sweeps(g.L);
+      }
+  }
   // split groups into contiguous chunks, one stream each: tiled groups in two
   // (ALU- and latency-bound kernels of the halves interleave); LDS groups in
   // two when they fill the GPU more than once and the last round would run
@@ -1043,7 +1072,9 @@ int rbim_batch_create(rbim_ctx* ctx, const rbim_batch_desc* d, rbim_batch** out)
     std::vector<Group> split;
     for (Group& g : b->groups) {
       int want = d->streams > 0 ? d->streams : kTiledStreams;
-      if (g.lds && d->streams <= 0) {
+      if (g.lds && b->ragged) {
+        want = 1;
+      } else if (g.lds && d->streams <= 0) {
         const long cap = (long)lds_blocks_per_cu(g.L, g.threads) * n_cu;
         const long n = (long)g.idx.size();
         want = cap > 0 && n > cap && n % cap != 0 ? 2 : 1;
@@ -1065,12 +1096,46 @@ int rbim_batch_create(rbim_ctx* ctx, const rbim_batch_desc* d, rbim_batch** out)
     }
     b->groups = std::move(split);
   }
+  // Streams: at most kMaxStreams (the device's hardware queues; more streams
+  // would share queues anyway), the largest groups first on streams of their
+  // own; with one launch for the LDS sets, those share one stream.
+  {
+    std::vector<int> order;
+    for (int i = 0; i < (int)b->groups.size(); ++i) {
+      Group& g = b->groups[i];
+      if (b->ragged && g.lds) {
+        if (b->ragged_lead < 0) b->ragged_lead = i;
+        if (b->ragged_lead != i) continue;
+      }
+      order.push_back(i);
+    }
+    auto work = [&](int i) {
+      int64_t w = 0;  // a ragged launch carries the sites of every LDS set
+      for (Group& g : b->groups)
+        if (&g == &b->groups[i] || (b->ragged && g.lds && b->groups[i].lds))
+          w += (int64_t)g.N * (int64_t)g.idx.size();
+      return w;
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) > work(y); });
+    const int n_streams = std::min<int>((int)order.size(), kMaxStreams);
+    for (int k = 0; k < n_streams; ++k) {
+      hipStream_t st = nullptr;
+      hipError_t se = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+      if (se != hipSuccess) {
+        rbim_batch_destroy(b);
+        return set_err(ctx, RBIM_E_HIP, "stream creation failed: %s", hipGetErrorString(se));
+      }
+      b->streams.push_back(st);
+    }
+    for (int k = 0; k < (int)order.size(); ++k) b->groups[order[k]].stream = b->streams[k % n_streams];
+    for (Group& g : b->groups)
+      if (b->ragged && g.lds) g.stream = b->groups[b->ragged_lead].stream;
+  }
   for (Group& g : b->groups) {
-    hipError_t se = hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking);
-    if (se == hipSuccess) se = hipEventCreateWithFlags(&g.done, hipEventDisableTiming);
+    hipError_t se = hipEventCreateWithFlags(&g.done, hipEventDisableTiming);
     if (se != hipSuccess) {
       rbim_batch_destroy(b);
-      return set_err(ctx, RBIM_E_HIP, "stream/event creation failed: %s", hipGetErrorString(se));
+      return set_err(ctx, RBIM_E_HIP, "event creation failed: %s", hipGetErrorString(se));
     }
   }
   for (Group& g : b->groups) {
@@ -1107,7 +1172,7 @@ int rbim_batch_run(rbim_batch* b, int64_t n_sweeps) {
     return set_err(ctx, RBIM_E_STATE, "run of %lld sweeps passes the end (%lld of %lld issued)",
                    (long long)n_sweeps, (long long)b->issued, (long long)b->total);
   HIP_TRY(ctx, hipSetDevice(ctx->device));
-  if (b->groups.size() == 1) {
+  if (b->groups.size() == 1 && !b->ragged) {
     int rc = group_run(ctx, b->groups[0], n_sweeps, b->stream, &b->launches);
     if (rc != RBIM_OK) return rc;
   } else {
@@ -1118,6 +1183,11 @@ int rbim_batch_run(rbim_batch* b, int64_t n_sweeps) {
     // stream's full queue before the next group had any work on the device.
     HIP_TRY(ctx, hipEventRecord(b->fork, b->stream));
     for (Group& g : b->groups) HIP_TRY(ctx, hipStreamWaitEvent(g.stream, b->fork, 0));
+    // one launch for the LDS sets: the lead LDS group's turn queues it
+    const LatticeSet* sets[4] = {nullptr, nullptr, nullptr, nullptr};
+    if (b->ragged)
+      for (Group& g : b->groups)
+        if (g.lds) sets[ragged_slot(g.L)] = &g.set;
     std::vector<int64_t> left(b->groups.size(), n_sweeps);
     for (bool more = n_sweeps > 0; more;) {
       more = false;
@@ -1125,8 +1195,18 @@ int rbim_batch_run(rbim_batch* b, int64_t n_sweeps) {
         if (left[i] <= 0) continue;
         Group& g = b->groups[i];
         int64_t q = 0;
-        const int rc = group_chunk(ctx, g, left[i], g.stream, &b->launches, &q);
-        if (rc != RBIM_OK) return rc;
+        if (b->ragged && g.lds) {
+          if ((int)i != b->ragged_lead) {
+            left[i] = 0;
+            continue;
+          }
+          q = std::min<int64_t>(left[i], b->ragged_per_launch);
+          HIP_TRY(ctx, launch_lds_ragged(sets, (int)q, g.stream));
+          ++b->launches;
+        } else {
+          const int rc = group_chunk(ctx, g, left[i], g.stream, &b->launches, &q);
+          if (rc != RBIM_OK) return rc;
+        }
         left[i] -= q;
         more = more || left[i] > 0;
       }
@@ -1236,7 +1316,12 @@ int rbim_batch_destroy(rbim_batch* b) {
   if (!b) return RBIM_OK;
   (void)hipSetDevice(b->ctx->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  for (Group& g : b->groups) group_free(g);
+  for (hipStream_t st : b->streams) (void)hipStreamSynchronize(st);
+  for (Group& g : b->groups) {
+    g.stream = nullptr;  // the batch's, destroyed below
+    group_free(g);
+  }
+  for (hipStream_t st : b->streams) (void)hipStreamDestroy(st);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->fork) (void)hipEventDestroy(b->fork);

@@ -60,6 +60,12 @@ struct TiledScratch {
 hipError_t launch_init_spins(const LatticeSet& s, int start, hipStream_t st);
 hipError_t launch_lds_sweeps(const LatticeSet& s, int threads, int sweeps, hipStream_t st);
 size_t lds_bytes(int N);
+// One launch over LDS sets of L = 16, 32, 64, 128 (sets[k] of side 16 << k, or
+// NULL): pages of 16384 sites per 1024-thread workgroup.  ragged_slot(L): k,
+// or -1 for a side it does not hold.
+int ragged_slot(int L);
+int ragged_threads(int L);  // threads per lattice in its page
+hipError_t launch_lds_ragged(const LatticeSet* const sets[4], int sweeps, hipStream_t st);
 // Workgroups of the LDS kernel one CU holds at once (0 if unknown).
 int lds_blocks_per_cu(int L, int threads);
 // Two-kernel sweep for lattices of several full tiles whose tile-edge slots fit

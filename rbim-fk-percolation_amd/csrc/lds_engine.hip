@@ -239,33 +239,54 @@ __device__ __forceinline__ Pair make_pair(uint32_t q, uint32_t N, uint32_t Lr, u
   return p;
 }
 
-template <int NT, int LC>
-__global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t sweeps) {
+// LDS words of one lattice of N sites: forest, counters, state bytes.
+__host__ __device__ constexpr uint32_t lds_words(uint32_t N) { return 2u * N + ((N + 15u) & ~15u) / 4u; }
+
+// The sweeps of P lattices in one workgroup, NT threads (whole wavefronts)
+// each: the workgroup's `page` holds lattices page * P .. page * P + P - 1 of
+// the launch's list, each in its own LDS region, every one run exactly as a
+// workgroup of NT threads of its own would run it.  The workgroup's barriers
+// are shared, so the P lattices sweep in step: they must share their sweep
+// counter and schedule (true of every lattice of a batch).  A page past the
+// end of the list keeps its idle lattices in step on neutral LDS (all spins
+// up, no bond drawn) and writes nothing.
+template <int NT, int LC, int P>
+__device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, int page) {
   constexpr int NW = NT / 64;
   constexpr bool ALL = LC > 0 && (LC % 2) == 0 && ((LC * LC / 2) % NT) == 0;
   // exactly two pairs per lane: phase C walks the four roots together
   constexpr bool MERGED = ALL && LC * LC / 2 == 2 * NT;
+  static_assert(P == 1 || LC > 0, "lattices share a workgroup only at a compile-time L");
   using Key = ClusterKey<(LC > 0 && LC <= 32)>;
   using KT = typename Key::T;
-  extern __shared__ uint32_t smem[];
-  __shared__ uint32_t red_c[NW][3];  // satisfied bonds, up spins, roots
-  __shared__ KT red_k[NW][2];        // two largest cluster keys
-  __shared__ uint32_t wrapflag[2];   // by sweep parity
-  __shared__ uint64_t a[RBIM_ACC_SLOTS];  // running sums of the open bin (lane 0)
+  extern __shared__ uint32_t smem_all[];
+  __shared__ uint32_t red_c_all[P][NW][3];  // satisfied bonds, up spins, roots
+  __shared__ KT red_k_all[P][NW][2];        // two largest cluster keys
+  __shared__ uint32_t wrapflag_all[P][2];   // by sweep parity
+  __shared__ uint64_t a_all[P][RBIM_ACC_SLOTS];  // running sums of the open bin (fold lane)
 
+  // wavefront-uniform (NT is a multiple of 64): the lattice's LDS base is scalar
+  const int sub = P > 1 ? __builtin_amdgcn_readfirstlane((int)threadIdx.x / NT) : 0;
   const uint32_t N = LC ? (uint32_t)(LC * LC) : (uint32_t)s.N;
   const uint32_t L = LC ? (uint32_t)LC : (uint32_t)s.L;
   const uint32_t magic = s.magic;
+  uint32_t* smem = smem_all + (size_t)sub * lds_words(N);
+  uint32_t (&red_c)[NW][3] = red_c_all[sub];
+  KT (&red_k)[NW][2] = red_k_all[sub];
+  uint32_t (&wrapflag)[2] = wrapflag_all[sub];
+  uint64_t (&a)[RBIM_ACC_SLOTS] = a_all[sub];
   uint32_t* uf = smem;
   uint32_t* cnt = smem + N;
   uint8_t* sb = reinterpret_cast<uint8_t*>(smem + 2 * N);
 
-  const int lat = run_lat(s, blockIdx.x);
+  const int ri = page * P + sub;
+  const bool valid = P == 1 || ri < s.n_run;
+  const int lat = run_lat(s, valid ? ri : page * P);  // an idle lattice follows the page's first
   const int64_t g0 = s.sweep_ctr[lat];
   const int64_t gend = min(g0 + (int64_t)sweeps, lat_total(s, lat));
   if (gend <= g0) return;  // finished (or empty) slot: its state stays as it is
   const int64_t n_therm = lat_therm(s, lat), spb = lat_spb(s, lat);
-  const int tid = threadIdx.x;
+  const int tid = P > 1 ? (int)threadIdx.x % NT : (int)threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const uint64_t below = (1ull << lane) - 1ull;
@@ -279,13 +300,13 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
   int8_t* gsp = s.spins + (size_t)lat * N;
   const uint8_t* gj = s.jbits + (size_t)lat * N;
   for (uint32_t n = tid; n < N; n += NT)
-    sb[n] = (uint8_t)((gsp[n] > 0 ? 1u : 0u) | ((uint32_t)gj[n] & 3u) << 1);
+    sb[n] = valid ? (uint8_t)((gsp[n] > 0 ? 1u : 0u) | ((uint32_t)gj[n] & 3u) << 1) : (uint8_t)1u;
   if (tid == 0) {
     wrapflag[0] = 0u;
     wrapflag[1] = 0u;
   }
 
-  const uint32_t thr = s.thr31[lat];
+  const uint32_t thr = valid ? s.thr31[lat] : 0u;
   const uint64_t seed = s.seed[lat];
   const uint32_t lid = s.lat_id[lat];
   const bool given = s.given_occ != nullptr;
@@ -331,6 +352,7 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
   };
   // One lane: fold the published partials of sweep gm into its bin.
   auto finalize = [&](int64_t gm) {
+    if (!valid) return;
     uint32_t sat = 0u, up = 0u, roots = 0u;
     KT f1 = 0u, f2 = 0u;
     for (int w = 0; w < NW; ++w) {
@@ -588,10 +610,51 @@ __global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t swe
     if (tid == fold_tid) finalize(gend - 1);
   }
 
+  if (!valid) return;
   if (tid == fold_tid && cur_bin >= 0)
     for (int k = 0; k < RBIM_ACC_SLOTS; ++k) gacc[cur_bin * RBIM_ACC_SLOTS + k] = a[k];
   if (tid == 0) s.sweep_ctr[lat] = gend;
   for (uint32_t n = tid; n < N; n += NT) gsp[n] = (sb[n] & 1u) ? (int8_t)1 : (int8_t)-1;
+}
+
+template <int NT, int LC>
+__global__ __launch_bounds__(NT) void lds_sweep_kernel(LatticeSet s, int32_t sweeps) {
+  lds_sweeps<NT, LC, 1>(s, sweeps, blockIdx.x);
+}
+
+// One launch over the LDS-resident lattices of several sides: workgroups of
+// 1024 threads, each holding a page of lattices of one side -- one of L = 128,
+// two of L = 64 (512 threads each), four of L = 32 (256 threads), sixteen of
+// L = 16 (a wavefront each: the per-side kernels' own workgroup sizes, which
+// keep every side's loop within the 128 registers of a 1024-thread
+// workgroup) -- so one dynamic LDS size serves every side.  set[k] holds the
+// lattices of side RAGGED_L[k]; page0[k] is its first page.  Workgroups take
+// the pages from the last (page = grid size - 1 - workgroup id), so they go out
+// largest first (L = 128, 64, 32, 16: 16384, 8192, 4096 and 4096 sites) and the
+// small pages fill the last round of workgroups.
+constexpr int RAGGED_L[4] = {16, 32, 64, 128};
+constexpr int RAGGED_P[4] = {16, 4, 2, 1};
+struct RaggedLaunch {
+  LatticeSet set[4];
+  int32_t page0[4];
+};
+
+__global__ __launch_bounds__(1024) void lds_ragged_kernel(RaggedLaunch r, int32_t sweeps) {
+  const int b = (int)(gridDim.x - 1u - blockIdx.x);
+  // (largest side first: the order the register allocator handles best)
+  if (b >= r.page0[3]) {
+    lds_sweeps<1024, 128, 1>(r.set[3], sweeps, b - r.page0[3]);
+    return;
+  }
+  if (b >= r.page0[2]) {
+    lds_sweeps<512, 64, 2>(r.set[2], sweeps, b - r.page0[2]);
+    return;
+  }
+  if (b >= r.page0[1]) {
+    lds_sweeps<256, 32, 4>(r.set[1], sweeps, b - r.page0[1]);
+    return;
+  }
+  lds_sweeps<64, 16, 16>(r.set[0], sweeps, b - r.page0[0]);
 }
 
 __global__ void init_spins_kernel(LatticeSet s, int start) {
@@ -626,8 +689,37 @@ hipError_t launch_lds_l(const LatticeSet& s, int sweeps, hipStream_t st) {
 
 }  // namespace
 
-size_t lds_bytes(int N) {
-  return (size_t)N * 8u + (((size_t)N + 15u) & ~(size_t)15u);
+size_t lds_bytes(int N) { return (size_t)lds_words((uint32_t)N) * 4u; }
+
+int ragged_slot(int L) {
+  for (int k = 0; k < 4; ++k)
+    if (RAGGED_L[k] == L) return k;
+  return -1;
+}
+
+int ragged_threads(int L) {
+  const int k = ragged_slot(L);
+  return k < 0 ? 0 : 1024 / RAGGED_P[k];
+}
+
+hipError_t launch_lds_ragged(const LatticeSet* const sets[4], int sweeps, hipStream_t st) {
+  static std::atomic<uint64_t> configured{0};
+  RaggedLaunch r{};
+  int pages = 0;
+  size_t bytes = 0;
+  for (int k = 0; k < 4; ++k) {
+    r.page0[k] = pages;
+    if (!sets[k]) continue;
+    r.set[k] = *sets[k];
+    pages += (r.set[k].n_run + RAGGED_P[k] - 1) / RAGGED_P[k];
+    bytes = std::max(bytes, (size_t)RAGGED_P[k] * lds_bytes(RAGGED_L[k] * RAGGED_L[k]));
+  }
+  if (pages == 0) return hipSuccess;
+  hipError_t e = ensure_dynamic_lds((const void*)lds_ragged_kernel,
+                                    (int)lds_bytes(RBIM_LDS_MAX_N), configured);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(lds_ragged_kernel, dim3(pages), dim3(1024), bytes, st, r, sweeps);
+  return hipGetLastError();
 }
 
 hipError_t launch_lds_sweeps(const LatticeSet& s, int threads, int sweeps, hipStream_t st) {

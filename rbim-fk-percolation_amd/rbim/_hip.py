@@ -97,6 +97,7 @@ class BatchDesc(ctypes.Structure):
         ("streams", ctypes.c_int32),
         ("series", ctypes.c_int32),
         ("tile", ctypes.c_int32),
+        ("single_launch", ctypes.c_int32),
     ]
 
 

@@ -9,8 +9,8 @@ keeps the fastest:
 * tiled kernels (L > 128): tile side (32, or 64 when it divides L) and sweeps
   per captured hipGraph.
 
-Results go to a JSON cache keyed by GPU architecture, L and a lattice-count
-bucket (``$RBIM_AUTOTUNE_CACHE``, default ``~/.cache/rbim-hip/autotune.json``);
+Results go to a JSON cache keyed by GPU architecture, engine kind (LDS or
+tiled), L and a lattice-count bucket (``$RBIM_AUTOTUNE_CACHE``, default ``~/.cache/rbim-hip/autotune.json``);
 :class:`rbim.engine.Batch` consults it when it is given no explicit shape.
 Every shape gives bit-identical results (the random streams depend only on
 seed, lattice, sweep and site), so tuning never changes a result.
@@ -48,8 +48,19 @@ def _bucket(n_lattices: int) -> int:
     return b
 
 
-def _key(L: int, n_lattices: int, arch: str = "gfx950") -> str:
-    return f"{arch}/L{L}/n{_bucket(n_lattices)}"
+def _kind(L: int, engine: int = 0) -> Optional[str]:
+    """Engine a tuned shape applies to: "lds" (the LDS-resident kernel, L <= 128),
+    "tiled" (the tiled sweeps the auto engine picks above), or None for an
+    engine choice the tuner never timed."""
+    from rbim import _hip
+
+    if L <= 128:
+        return "lds" if engine in (_hip.ENGINE_AUTO, _hip.ENGINE_BATCHED) else None
+    return "tiled" if engine in (_hip.ENGINE_AUTO, _hip.ENGINE_PLAIN) else None
+
+
+def _key(L: int, n_lattices: int, kind: str, arch: str = "gfx950") -> str:
+    return f"{arch}/{kind}/L{L}/n{_bucket(n_lattices)}"
 
 
 def _load() -> Dict[str, dict]:
@@ -60,16 +71,20 @@ def _load() -> Dict[str, dict]:
         return {}
 
 
-def lookup(L: int, n_lattices: int) -> Optional[dict]:
-    """Tuned {"threads", "sweeps_per_launch", "tile"} for this shape, or None."""
+def lookup(L: int, n_lattices: int, engine: int = 0) -> Optional[dict]:
+    """Tuned {"threads", "sweeps_per_launch", "tile"} for this shape on this
+    engine choice, or None."""
+    kind = _kind(L, engine)
+    if kind is None:
+        return None
     with _lock:
-        return _load().get(_key(L, n_lattices))
+        return _load().get(_key(L, n_lattices, kind))
 
 
 def _store(L: int, n_lattices: int, entry: dict) -> None:
     with _lock:
         data = _load()
-        data[_key(L, n_lattices)] = entry
+        data[_key(L, n_lattices, _kind(L))] = entry
         os.makedirs(os.path.dirname(cache_path()), exist_ok=True)
         tmp = cache_path() + ".tmp"
         with open(tmp, "w") as fh:

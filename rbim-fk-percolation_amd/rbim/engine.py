@@ -162,7 +162,7 @@ class Batch:
     def __init__(self, lattices: Sequence[LatticeSpec], *, n_therm: int, n_bins: int,
                  sweeps_per_bin: int, start: str = "hot", engine: int = _hip.ENGINE_AUTO,
                  sweeps_per_launch: int = 0, threads: int = 0, streams: int = 0,
-                 series: int = 0, tile: int = 0):
+                 series: int = 0, tile: int = 0, single_launch: bool = False):
         from rbim.clusters import activation_threshold
 
         if start not in ("hot", "cold"):
@@ -190,10 +190,11 @@ class Batch:
             self._spins0 = np.ascontiguousarray(np.concatenate([
                 (np.asarray(s.spins0, np.int8).reshape(-1) if s.spins0 is not None
                  else np.ones(s.L * s.L, np.int8)) for s in lattices]))
-        if threads == 0 and sweeps_per_launch == 0 and tile == 0 and len(set(self.Ls.tolist())) == 1:
+        if (threads == 0 and sweeps_per_launch == 0 and tile == 0 and not single_launch
+                and len(set(self.Ls.tolist())) == 1):
             from rbim import autotune
 
-            tuned = autotune.lookup(int(self.Ls[0]), self.n)
+            tuned = autotune.lookup(int(self.Ls[0]), self.n, int(engine))
             if tuned:
                 threads, sweeps_per_launch = int(tuned["threads"]), int(tuned["sweeps_per_launch"])
                 tile = int(tuned.get("tile", 0))
@@ -216,6 +217,7 @@ class Batch:
         desc.threads = int(threads)
         desc.streams = int(streams)
         desc.tile = int(tile)
+        desc.single_launch = 1 if single_launch else 0
         if series < 0:
             raise ValueError("series must be >= 0")
         self.series_capacity = int(series)

@@ -271,6 +271,40 @@ def test_random_shapes_bit_exact(hip, coracle, engine):
             assert np.array_equal(acc[i], acc_ref), (engine, start, spec.L)
 
 
+def test_single_launch_ragged_batch_bit_exact(hip, coracle):
+    """One launch over L = 16, 32, 64, 128 (pages of 16, 4, 2 and 1 lattices
+    per workgroup, last pages part-filled) beside a tiled lattice of L = 160:
+    trajectories equal the oracle's and the one-launch-per-L batch's."""
+    from rbim.engine import Batch, LatticeSpec
+
+    rng = np.random.default_rng(2026)
+    specs = []
+    for L, count in ((16, 17), (32, 5), (64, 3), (128, 2), (160, 1)):
+        for _ in range(count):
+            J_h, J_v = random_disorder(rng, L, float(rng.uniform(0.0, 0.15)))
+            specs.append(LatticeSpec(L=L, J_h=J_h, J_v=J_v, beta=1.0 / float(rng.uniform(1.2, 3.5)),
+                                     seed=int(rng.integers(0, 2**63)),
+                                     lattice_id=int(rng.integers(0, 2**31))))
+    rng.shuffle(specs)
+    n_therm, spb, n_bins = 3, 4, 2
+    total = n_therm + spb * n_bins
+    out = {}
+    for single in (True, False):
+        with Batch(specs, n_therm=n_therm, n_bins=n_bins, sweeps_per_bin=spb, single_launch=single,
+                   sweeps_per_launch=4 if single else 0) as batch:
+            batch.run(5)
+            batch.run(total - 5)
+            out[single] = (batch.moments(), batch.spins(), batch.info())
+    acc, spins, info = out[True]
+    assert info.lds_groups == 4 and info.n_groups == 5  # the one tiled lattice: one part
+    assert np.array_equal(acc, out[False][0])
+    for i, spec in enumerate(specs):
+        s_ref, acc_ref = _oracle_trajectory(coracle, spec, total, n_therm, spb, n_bins)
+        assert np.array_equal(spins[i], s_ref), spec.L
+        assert np.array_equal(acc[i], acc_ref), spec.L
+        assert np.array_equal(out[False][1][i], s_ref), spec.L
+
+
 def test_lds_sets_split_over_streams_are_invariant(hip):
     """An LDS set split over streams -- explicitly, or by default when it fills
     the GPU more than once with a part-empty last round (1100 lattices at L=64,

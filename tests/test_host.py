@@ -356,3 +356,26 @@ def test_crossing_interpolation():
     for a, b, x in pairs:
         assert first_crossing(t, noisy[int(a)], noisy[int(b)]) == x
     assert mean_crossing(size_pair_crossings(t, {8: t})) is None
+
+
+def test_agreement_rule_on_synthetic_scores():
+    from agreement import agreement
+
+    """The acceptance rule itself: N(0, 1) scores pass nearly always, a shift
+    of one observable, a few wild points or inflated scatter fail."""
+    rng = np.random.default_rng(8)
+    names = [f"o{i}" for i in range(16)]
+
+    def rows(z):
+        return [(names[i % 16], float(i // 16), 0.0, 0.0, float(v)) for i, v in enumerate(z)]
+
+    passes = sum(all(p for p, _, _ in agreement(rows(rng.normal(size=384))).values())
+                 for _ in range(200))
+    assert passes >= 198
+    shifted = rng.normal(size=384)
+    shifted[::16] += 2.5  # one observable off by 2.5 sigma at every temperature
+    assert not agreement(rows(shifted))["chi2"][0]
+    wild = rng.normal(size=384)
+    wild[:7] = 3.4
+    assert not agreement(rows(wild))["beyond3"][0]
+    assert not agreement(rows(np.concatenate([rng.normal(size=383), [6.0]])))["max"][0]
