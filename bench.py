@@ -237,7 +237,7 @@ def cpu_baseline(specs, site_updates_per_thread, per_thread):
 def fused_tile_size(L):
     """Tile side of the edge-slot sweep for L (0: L uses the multi-kernel tiled
     sweep); mirrors fused_tile_size() of the library."""
-    return 64 if L >= 64 and L % 64 == 0 else 32 if L >= 32 and L % 32 == 0 else 0
+    return 64 if L >= 64 and L % 64 == 0 else 32 if L >= 32 else 0
 
 
 def measure_point(L, args, steps, warmup, plain_steps, with_cpu):

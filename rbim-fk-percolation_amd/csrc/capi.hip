@@ -144,7 +144,7 @@ size_t group_bytes(int L, int n_lat, bool lds, bool fused, int n_bins, int serie
              (size_t)n_lat * (size_t)std::max(series, 0) * sizeof(rbim_sweep_stats);
   if (fused) {
     const int T = fused_tile_size(L, tile_pref);
-    const size_t tiles = (size_t)(L / T) * (L / T), ns = tiles * 4 * T;
+    const size_t nt = (size_t)fused_tiles_per_side(L, T), tiles = nt * nt, ns = tiles * 4 * T;
     b += N * n_lat * (1 + 4) + (size_t)n_lat * (tiles * (2 * T / 64) * 8 + ns / 8 +
                                                  tiles * (2 * 5 + 2) * 8 + 2 * 6 * 8 + 8 + 4);
     if (fused_merge_in_hbm(L, T))
@@ -252,7 +252,7 @@ int group_alloc(rbim_ctx* ctx, Group& g, const std::vector<uint8_t>& jbits,
   if (g.fused) {
     FusedScratch& f = g.fz;
     f.tile = fused_tile_size(g.L, g.tile_pref);
-    f.tiles_x = g.L / f.tile;
+    f.tiles_x = fused_tiles_per_side(g.L, f.tile);
     const size_t tiles = (size_t)f.tiles_x * f.tiles_x;
     f.nslots = (int32_t)(tiles * 4 * f.tile);
     const size_t nl = (size_t)g.n_lat;

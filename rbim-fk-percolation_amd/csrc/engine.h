@@ -91,8 +91,10 @@ struct FusedScratch {
   int32_t tile, tiles_x, nslots, big;
 };
 // Tile side for L (pref: 32 or 64 when it divides L, else the default: 64
-// when it divides L, else 32); 0 when L is not a multiple of 32.
+// when it divides L, else 32 -- with a cut last tile row and column when 32
+// does not divide L either); 0 for L <= 32 not a multiple of 32.
 int fused_tile_size(int L, int pref = 0);
+int fused_tiles_per_side(int L, int T);
 bool fused_merge_in_hbm(int L, int T);
 int fused_kernels_per_sweep(const FusedScratch& f);
 hipError_t launch_fused_sweep(const LatticeSet& s, const FusedScratch& f, hipStream_t st);

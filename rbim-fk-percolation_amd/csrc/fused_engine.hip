@@ -73,6 +73,21 @@ __device__ __forceinline__ uint32_t slot_li(uint32_t k) {
   return lx * T + ly;
 }
 
+// Edge slot k of a tile whose valid part is wx x wy sites (wx = wy = T for a
+// whole tile): its local index, and whether the position holds a site (the
+// sides of a cut tile are shorter than T).
+template <int T>
+__device__ __forceinline__ uint32_t slot_li_cut(uint32_t k, uint32_t wx, uint32_t wy, bool& ok) {
+  const uint32_t side = k / T, i = k % T;
+  ok = side < 2u ? i < wy : i < wx;
+  const uint32_t lx = side == 0 ? 0u : side == 1 ? wx - 1u : i;
+  const uint32_t ly = side == 2 ? 0u : side == 3 ? wy - 1u : i;
+  return lx * T + ly;
+}
+
+// Slot entry of a position of a cut tile that holds no site.
+constexpr uint64_t SLOT_ABSENT = 1ull << 63;
+
 __device__ __forceinline__ uint32_t flip_of(const uint32_t* fb, uint32_t slot) {
   return (fb[slot >> 5] >> (slot & 31u)) & 1u;
 }
@@ -122,20 +137,22 @@ __device__ __forceinline__ const uint32_t* flipbits_of(const FusedScratch& f, in
 template <bool BIG>
 constexpr int fb_words() { return BIG ? 1 : RBIM_FUSED_MAX_SLOTS / 32; }
 
-// Spin bits of the column beyond x0+T-1 (hsp[0..T)) and the row beyond
-// y0+T-1 (hsp[T..2T)), pending flips applied (sites of the neighbour tiles).
+// Spin bits of the column beyond x0+wx-1 (hsp[0..wy)) and the row beyond
+// y0+wy-1 (hsp[T..T+wx)), pending flips applied (sites of the neighbour tiles);
+// wx x wy is the tile's valid part.
 template <int T>
 __device__ __forceinline__ void load_halo(const LatticeSet& s, const FusedScratch& f,
                                           const FusedView& v, const uint32_t* fb, uint32_t tx,
-                                          uint32_t ty, uint8_t* hsp) {
+                                          uint32_t ty, uint32_t wx, uint32_t wy, uint8_t* hsp) {
   const uint32_t tid = threadIdx.x;
   if (tid >= 2u * T) return;
   const uint32_t L = (uint32_t)s.L, nt = (uint32_t)f.tiles_x;
   const uint32_t x0 = tx * T, y0 = ty * T;
   const bool right = tid < (uint32_t)T;
   const uint32_t i = right ? tid : tid - T;
-  const uint32_t hx = right ? (x0 + T == L ? 0u : x0 + T) : x0 + i;
-  const uint32_t hy = right ? y0 + i : (y0 + T == L ? 0u : y0 + T);
+  if (i >= (right ? wy : wx)) return;
+  const uint32_t hx = right ? (x0 + wx == L ? 0u : x0 + wx) : x0 + i;
+  const uint32_t hy = right ? y0 + i : (y0 + wy == L ? 0u : y0 + wy);
   const uint32_t ntile = right ? ty * nt + (tx + 1u == nt ? 0u : tx + 1u)
                                : (ty + 1u == nt ? 0u : ty + 1u) * nt + tx;
   const uint32_t hn = hx * L + hy;
@@ -170,7 +187,11 @@ __device__ __forceinline__ void troot16_4(const uint32_t* par, uint32_t (&x)[4])
 }
 
 // ------------------------------------------------------------------- KA
-template <int T, int NTH, bool BIG>
+// PART: 0 -- L a multiple of T, every tile whole; 1 -- L even, the last tile
+// row and column cut (wx, wy < T; site pairs stay whole and aligned); 2 -- L
+// odd, cut as well, so the last pair of a cut row holds one site and a pair of
+// a row starting at an odd site straddles two Philox pairs (two draws).
+template <int T, int NTH, bool BIG, int PART>
 __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
   constexpr int ITERS = T * T / 2 / NTH;
   constexpr uint32_t NS = 4u * T;  // edge slots per tile
@@ -188,6 +209,10 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
   const uint32_t nt = (uint32_t)f.tiles_x, tiles = nt * nt;
   const uint32_t tile = blockIdx.x, tx = tile % nt, ty = tile / nt;
   const uint32_t x0 = tx * T, y0 = ty * T;
+  // the tile's valid part (the last tile row / column is cut when T does not
+  // divide L); sites beyond it are absent: no loads, stores, bonds or counts
+  const uint32_t wx = PART ? min((uint32_t)T, L - x0) : (uint32_t)T;
+  const uint32_t wy = PART ? min((uint32_t)T, L - y0) : (uint32_t)T;
   const size_t base = (size_t)lat * N;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint64_t below = (1ull << lane) - 1ull;
@@ -224,10 +249,36 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
   for (int it = 0; it < ITERS; ++it) {
     const uint32_t ql = (uint32_t)(it * NTH + tid);
     const uint32_t lx = ql / (T / 2), ly0 = (ql % (T / 2)) * 2u;
-    const uint32_t n0 = (x0 + lx) * L + y0 + ly0;  // even: the pair loads are aligned
-    sp2[it] = *reinterpret_cast<const uint16_t*>(v.sin + n0);
-    lb2[it] = *reinterpret_cast<const uint32_t*>(v.lin + n0);
-    j2[it] = *reinterpret_cast<const uint16_t*>(jb + n0);
+    const uint32_t n0 = (x0 + lx) * L + y0 + ly0;  // even L: the pair loads are aligned
+    if (PART == 0) {
+      sp2[it] = *reinterpret_cast<const uint16_t*>(v.sin + n0);
+      lb2[it] = *reinterpret_cast<const uint32_t*>(v.lin + n0);
+      j2[it] = *reinterpret_cast<const uint16_t*>(jb + n0);
+    } else {
+      // absent sites read as spin +1, no label, J = +1
+      const bool v0 = lx < wx && ly0 < wy, v1 = lx < wx && ly0 + 1u < wy;
+      sp2[it] = 0x0101u;
+      lb2[it] = 0u;
+      j2[it] = 0u;
+      if (PART == 1) {
+        if (v0) {
+          sp2[it] = *reinterpret_cast<const uint16_t*>(v.sin + n0);
+          lb2[it] = *reinterpret_cast<const uint32_t*>(v.lin + n0);
+          j2[it] = *reinterpret_cast<const uint16_t*>(jb + n0);
+        }
+      } else {
+        if (v0) {
+          sp2[it] = (uint32_t)(uint8_t)v.sin[n0] | 0x0100u;
+          lb2[it] = v.lin[n0];
+          j2[it] = jb[n0];
+        }
+        if (v1) {
+          sp2[it] = (sp2[it] & 0xFFu) | ((uint32_t)(uint8_t)v.sin[n0 + 1u] << 8);
+          lb2[it] |= (uint32_t)v.lin[n0 + 1u] << 16;
+          j2[it] |= (uint32_t)jb[n0 + 1u] << 8;
+        }
+      }
+    }
   }
   const bool halo = tid < 2 * T;  // tid < T: right column, else upper row
   int8_t hs = 1;
@@ -235,10 +286,12 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
   if (halo) {
     const bool right = tid < T;
     const uint32_t i = right ? (uint32_t)tid : (uint32_t)tid - T;
-    const uint32_t hx = right ? (x0 + T == L ? 0u : x0 + T) : x0 + i;
-    const uint32_t hy = right ? y0 + i : (y0 + T == L ? 0u : y0 + T);
-    hs = v.sin[hx * L + hy];
-    hl = v.lin[hx * L + hy];
+    const uint32_t hx = right ? (x0 + wx == L ? 0u : x0 + wx) : x0 + i;
+    const uint32_t hy = right ? y0 + i : (y0 + wy == L ? 0u : y0 + wy);
+    if (PART == 0 || i < (right ? wy : wx)) {
+      hs = v.sin[hx * L + hy];
+      hl = v.lin[hx * L + hy];
+    }
   }
   // spin bit of a site, its edge cluster's pending flip applied (wbase: the
   // first flip word of the site's tile); the shuffle runs on every lane
@@ -271,28 +324,49 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
     const uint32_t li0 = ql << 1, li1 = li0 + 1u;
     const uint32_t n0 = (x0 + lx) * L + y0 + ly0;
     const uint32_t b0 = bits[li0], b1 = bits[li1];
-    const uint32_t sr0 = lx + 1u < T ? bits[li0 + T] & 1u : hsp[ly0];
-    const uint32_t sr1 = lx + 1u < T ? bits[li1 + T] & 1u : hsp[ly0 + 1u];
-    const uint32_t su1 = ly0 + 2u < T ? bits[li1 + 1u] & 1u : hsp[T + lx];
-    // satisfied: J s_a s_b = +1, i.e. equal spins xor J < 0
-    const uint32_t sh0 = ~(b0 ^ sr0 ^ (b0 >> 4)) & 1u;
-    const uint32_t sv0 = ~(b0 ^ b1 ^ (b0 >> 5)) & 1u;
-    const uint32_t sh1 = ~(b1 ^ sr1 ^ (b1 >> 4)) & 1u;
-    const uint32_t sv1 = ~(b1 ^ su1 ^ (b1 >> 5)) & 1u;
+    // the x neighbour is in the tile below row wx-1, the y neighbour of n1 below
+    // column wy-1 (of n0: n1 itself, unless n0 is a cut row's last site)
+    const bool v0 = !PART || (lx < wx && ly0 < wy), v1 = !PART || (lx < wx && ly0 + 1u < wy);
+    const uint32_t sr0 = lx + 1u < wx ? bits[li0 + T] & 1u : hsp[ly0];
+    const uint32_t sr1 = lx + 1u < wx ? bits[li1 + T] & 1u : hsp[ly0 + 1u];
+    const uint32_t su0 = PART == 2 && ly0 + 1u >= wy ? hsp[T + lx] : b1 & 1u;
+    const uint32_t su1 = ly0 + 2u < wy ? bits[li1 + 1u] & 1u : hsp[T + lx];
+    // satisfied: J s_a s_b = +1, i.e. equal spins xor J < 0 (absent: never)
+    const uint32_t sh0 = v0 ? ~(b0 ^ sr0 ^ (b0 >> 4)) & 1u : 0u;
+    const uint32_t sv0 = v0 ? ~(b0 ^ su0 ^ (b0 >> 5)) & 1u : 0u;
+    const uint32_t sh1 = v1 ? ~(b1 ^ sr1 ^ (b1 >> 4)) & 1u : 0u;
+    const uint32_t sv1 = v1 ? ~(b1 ^ su1 ^ (b1 >> 5)) & 1u : 0u;
     if (meas_em) {
-      me += 4 - 2 * (int32_t)(sh0 + sv0 + sh1 + sv1);
-      mm += 2 * (int32_t)((b0 & 1u) + (b1 & 1u)) - 2;
+      if (PART == 0) {
+        me += 4 - 2 * (int32_t)(sh0 + sv0 + sh1 + sv1);
+        mm += 2 * (int32_t)((b0 & 1u) + (b1 & 1u)) - 2;
+      } else {
+        if (v0) me += 2 - 2 * (int32_t)(sh0 + sv0), mm += 2 * (int32_t)(b0 & 1u) - 1;
+        if (v1) me += 2 - 2 * (int32_t)(sh1 + sv1), mm += 2 * (int32_t)(b1 & 1u) - 1;
+      }
     }
+    // horizontal / vertical draws of the two sites: words x, y of the even
+    // and z, w of the odd site of a Philox pair; at odd L a pair starting at
+    // an odd site takes z, w of its pair and x, y of the next one
     const rbim_u32x4 r = rbim_sweep_draws(seed, lid, (uint32_t)g, n0 >> 1);
-    const uint32_t oh0 = sh0 & ((r.x >> 1) < thr ? 1u : 0u);
-    const uint32_t ov0 = sv0 & ((r.y >> 1) < thr ? 1u : 0u);
-    const uint32_t oh1 = sh1 & ((r.z >> 1) < thr ? 1u : 0u);
-    const uint32_t ov1 = sv1 & ((r.w >> 1) < thr ? 1u : 0u);
+    uint32_t dh0 = r.x, dv0 = r.y, dh1 = r.z, dv1 = r.w;
+    if (PART == 2 && (n0 & 1u)) {
+      const rbim_u32x4 r2 = rbim_sweep_draws(seed, lid, (uint32_t)g, (n0 >> 1) + 1u);
+      dh0 = r.z;
+      dv0 = r.w;
+      dh1 = r2.x;
+      dv1 = r2.y;
+    }
+    const uint32_t oh0 = sh0 & ((dh0 >> 1) < thr ? 1u : 0u);
+    const uint32_t ov0 = sv0 & ((dv0 >> 1) < thr ? 1u : 0u);
+    const uint32_t oh1 = sh1 & ((dh1 >> 1) < thr ? 1u : 0u);
+    const uint32_t ov1 = sv1 & ((dv1 >> 1) < thr ? 1u : 0u);
     // bit 0 (the spin other lanes read as a neighbour) is rewritten unchanged
-    bits[li0] = (uint8_t)((b0 & 1u) | (oh0 << 1) | (ov0 << 2) | ((r.x & 1u) << 3));
-    bits[li1] = (uint8_t)((b1 & 1u) | (oh1 << 1) | (ov1 << 2) | ((r.z & 1u) << 3));
-    const bool internal = ov0 != 0u;                       // n0 -> n1, same row and tile
-    const bool outlink = ov1 != 0u && ly0 + 1u != T - 1u;  // n1 -> n1+1 inside the tile
+    bits[li0] = (uint8_t)((b0 & 1u) | (oh0 << 1) | (ov0 << 2) | ((dh0 & 1u) << 3));
+    bits[li1] = (uint8_t)((b1 & 1u) | (oh1 << 1) | (ov1 << 2) | ((dh1 & 1u) << 3));
+    // n0 -> n1, same row and tile (for a cut row's last site, the bond leaves)
+    const bool internal = ov0 != 0u && (PART != 2 || ly0 + 1u < wy);
+    const bool outlink = ov1 != 0u && ly0 + 2u < wy;  // n1 -> n1+1 inside the tile
     const uint32_t chunk = 2u * (uint32_t)(it * NTH + wave * 64);
     const uint32_t start0 = run_start(__ballot(internal && outlink), __ballot(outlink), below, chunk);
     par[li0] = start0;
@@ -320,7 +394,7 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
     const uint32_t li0 = ql << 1, li1 = li0 + 1u;
     const uint32_t b0 = bits[li0], b1 = bits[li1];
     uint32_t req = 0u;
-    if (lx + 1u < T) {
+    if (lx + 1u < wx) {
       const bool first = ly0 == 0u;  // no plaquette to the left
       const uint32_t lf = first ? li0 : li0 - 1u;
       const uint32_t left = bits[lf], left_up = bits[lf + T];
@@ -329,7 +403,7 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
       const uint32_t h1 = (b1 >> 1) & 1u & ~((b0 >> 1) & (b0 >> 2) & (bits[li0 + T] >> 2));
       req = h0 | (h1 << 1);
     }
-    req |= ((b1 >> 2) & 1u & ((lane == 63 && ly0 + 1u != T - 1u) ? 1u : 0u)) << 2;
+    req |= ((b1 >> 2) & 1u & ((lane == 63 && ly0 + 2u < wy) ? 1u : 0u)) << 2;
     // the wave's queue lives in lcnt, which is cleared before it is used
     const uint32_t qa[3] = {li0, li1, li1};
     const uint32_t qb[3] = {li0 + T, li1 + T, li1 + 1u};
@@ -342,22 +416,25 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
   __syncthreads();
   // roots: counts (size << 16 | n_plus before the flip); every site is pointed
   // straight at its root (the forest is settled, so this is only compression)
-  auto count_and_point = [&](uint32_t li0, uint32_t b0, uint32_t b1, uint32_t r0, uint32_t r1) {
+  // (v0, v1: the sites are present; absent ones add to no count)
+  auto count_and_point = [&](uint32_t li0, uint32_t b0, uint32_t b1, uint32_t r0, uint32_t r1,
+                             bool v0, bool v1) {
     const uint32_t li1 = li0 + 1u;
     // the first lane's cluster (often most of the wave) is summed by ballots
     // and added by one atomic
     const uint32_t R = __builtin_amdgcn_readfirstlane(r0);
-    const bool in0 = r0 == R, in1 = r1 == R;
+    const bool in0 = v0 && r0 == R, in1 = v1 && r1 == R;
     const uint64_t m0 = __ballot(in0), m1 = __ballot(in1);
     const uint64_t u0 = __ballot(in0 && (b0 & 1u)), u1 = __ballot(in1 && (b1 & 1u));
     if (lane == 0)
       atomicAdd(&lcnt[R], (uint32_t)(__popcll(m0) + __popcll(m1)) << 16 |
                               (uint32_t)(__popcll(u0) + __popcll(u1)));
-    if (!in0 && !in1 && r0 == r1) {
+    const bool o0 = v0 && !in0, o1 = v1 && !in1;
+    if (o0 && o1 && r0 == r1) {
       atomicAdd(&lcnt[r0], 0x20000u | ((b0 & 1u) + (b1 & 1u)));
     } else {
-      if (!in0) atomicAdd(&lcnt[r0], 0x10000u | (b0 & 1u));
-      if (!in1) atomicAdd(&lcnt[r1], 0x10000u | (b1 & 1u));
+      if (o0) atomicAdd(&lcnt[r0], 0x10000u | (b0 & 1u));
+      if (o1) atomicAdd(&lcnt[r1], 0x10000u | (b1 & 1u));
     }
     // every site now points straight at its root; a root's word becomes
     // root | rep << 16 with rep = "none" until an edge slot claims it (the
@@ -371,26 +448,37 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
     for (int it = 0; it < ITERS; it += 2) {
       const uint32_t la = (uint32_t)(it * NTH + tid) << 1, lb = la + 2u * NTH;
       const uint32_t ba0 = bits[la], ba1 = bits[la + 1u], bb0 = bits[lb], bb1 = bits[lb + 1u];
+      bool va0 = true, va1 = true, vb0 = true, vb1 = true;
+      if (PART) {
+        const uint32_t xa = la / T, ya = la % T, xb = lb / T, yb = lb % T;
+        va0 = xa < wx && ya < wy, va1 = xa < wx && ya + 1u < wy;
+        vb0 = xb < wx && yb < wy, vb1 = xb < wx && yb + 1u < wy;
+      }
       // li1 shares li0's run when the in-row bond is occupied
-      uint32_t r[4] = {la, (ba0 & 4u) ? la : la + 1u, lb, (bb0 & 4u) ? lb : lb + 1u};
+      uint32_t r[4] = {la, (ba0 & 4u) && va1 ? la : la + 1u, lb, (bb0 & 4u) && vb1 ? lb : lb + 1u};
       troot16_4(par, r);
-      count_and_point(la, ba0, ba1, r[0], r[1]);
-      count_and_point(lb, bb0, bb1, r[2], r[3]);
+      count_and_point(la, ba0, ba1, r[0], r[1], va0, va1);
+      count_and_point(lb, bb0, bb1, r[2], r[3], vb0, vb1);
     }
   } else {
 #pragma unroll 1
     for (int it = 0; it < ITERS; ++it) {
       const uint32_t li0 = (uint32_t)(it * NTH + tid) << 1, li1 = li0 + 1u;
       const uint32_t b0 = bits[li0], b1 = bits[li1];
-      uint32_t r0 = li0, r1 = (b0 & 4u) ? li0 : li1;  // li1 shares li0's run when linked
+      const bool v0 = !PART || (li0 / T < wx && li0 % T < wy);
+      const bool v1 = !PART || (li0 / T < wx && li0 % T + 1u < wy);
+      uint32_t r0 = li0, r1 = (b0 & 4u) && v1 ? li0 : li1;  // li1 shares li0's run when linked
       troot16_2(par, r0, r1);
-      count_and_point(li0, b0, b1, r0, r1);
+      count_and_point(li0, b0, b1, r0, r1, v0, v1);
     }
   }
   __syncthreads();
   // rep slot of every edge cluster: par[root] = root | min slot << 16
   for (uint32_t k = tid; k < NS; k += NTH) {
-    const uint32_t r = par[slot_li<T>(k)] & 0xFFFFu;
+    bool ok = true;
+    const uint32_t li = PART ? slot_li_cut<T>(k, wx, wy, ok) : slot_li<T>(k);
+    if (!ok) continue;
+    const uint32_t r = par[li] & 0xFFFFu;
     atomicMin(&par[r], r | (k << 16));
   }
   __syncthreads();
@@ -406,6 +494,7 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
     const uint32_t lx = ql / (T / 2), ly0 = (ql % (T / 2)) * 2u;
     const uint32_t li0 = ql << 1;
     const uint32_t n0 = (x0 + lx) * L + y0 + ly0;
+    const bool vs[2] = {!PART || (lx < wx && ly0 < wy), !PART || (lx < wx && ly0 + 1u < wy)};
     uint32_t sp2 = 0u, lb2 = 0u;
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
@@ -420,15 +509,26 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
       const uint32_t ns = edge ? sbit : sbit ^ fl;
       sp2 |= (ns ? 0x01u : 0xFFu) << (8 * j);
       lb2 |= (edge ? (LAB_EDGE | rep) : 0u) << (16 * j);
-      if (meas_cl && !edge && r == li) {  // an interior cluster, at its root
+      if (meas_cl && !edge && r == li && vs[j]) {  // an interior cluster, at its root
         const uint32_t key = ((lcnt[r] >> 16) << 12) | (0xFFFu - li);
         ++incl;
         ik2 = max(ik2, min(ik1, key));
         ik1 = max(ik1, key);
       }
     }
-    *reinterpret_cast<uint16_t*>(v.sout + n0) = (uint16_t)sp2;
-    *reinterpret_cast<uint32_t*>(v.lout + n0) = lb2;
+    if (PART == 0 || (PART == 1 && vs[0])) {
+      *reinterpret_cast<uint16_t*>(v.sout + n0) = (uint16_t)sp2;
+      *reinterpret_cast<uint32_t*>(v.lout + n0) = lb2;
+    } else if (PART == 2) {
+      if (vs[0]) {
+        v.sout[n0] = (int8_t)(sp2 & 0xFFu);
+        v.lout[n0] = (uint16_t)(lb2 & 0xFFFFu);
+      }
+      if (vs[1]) {
+        v.sout[n0 + 1u] = (int8_t)(sp2 >> 8);
+        v.lout[n0 + 1u] = (uint16_t)(lb2 >> 16);
+      }
+    }
   }
   // edge slots for KB, and the occupation of the bonds leaving the tile
   if (BIG) {
@@ -436,7 +536,14 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
     // carrying their cluster's counts and (min site << 1 | flip)
     const size_t sb = (size_t)lat * f.nslots + (size_t)tile * NS;
     for (uint32_t k = tid; k < NS; k += NTH) {
-      const uint32_t r = par[slot_li<T>(k)] & 0xFFFFu;
+      bool ok = true;
+      const uint32_t li = PART ? slot_li_cut<T>(k, wx, wy, ok) : slot_li<T>(k);
+      if (!ok) {  // no site here: a slot of its own, never folded or counted
+        f.guf[sb + k] = gpack(tile * NS + k, 0, 0);
+        f.gmn[sb + k] = 0xFFFFFFFFu;
+        continue;
+      }
+      const uint32_t r = par[li] & 0xFFFFu;
       const uint32_t rep = par[r] >> 16;
       f.guf[sb + k] = gpack(tile * NS + rep, 0, 0);
       if (rep == k) {
@@ -450,16 +557,23 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
   } else {
     uint64_t* sl = f.slots + (size_t)lat * f.nslots + (size_t)tile * NS;
     for (uint32_t k = tid; k < NS; k += NTH) {
-      const uint32_t r = par[slot_li<T>(k)] & 0xFFFFu;
+      bool ok = true;
+      const uint32_t li = PART ? slot_li_cut<T>(k, wx, wy, ok) : slot_li<T>(k);
+      if (!ok) {
+        sl[k] = SLOT_ABSENT;
+        continue;
+      }
+      const uint32_t r = par[li] & 0xFFFFu;
       const uint32_t rep = par[r] >> 16;
       const uint32_t c = lcnt[r];
       sl[k] = slot_entry(rep, r, (bits[r] >> 3) & 1u, c >> 16, c & 0xFFFFu);
     }
   }
   if (tid < 2 * T) {
-    // tid < T: +x bond of (x0+T-1, y0+tid); else +y bond of (x0+tid-T, y0+T-1)
-    const uint32_t occ = tid < T ? (bits[(T - 1) * T + tid] >> 1) & 1u
-                                 : (bits[(tid - T) * T + T - 1] >> 2) & 1u;
+    // tid < T: +x bond of (x0+wx-1, y0+tid); else +y bond of (x0+tid-T, y0+wy-1)
+    const uint32_t i = tid < T ? (uint32_t)tid : (uint32_t)tid - T;
+    uint32_t occ = tid < T ? (bits[(wx - 1u) * T + i] >> 1) & 1u : (bits[i * T + wy - 1u] >> 2) & 1u;
+    if (PART && i >= (tid < T ? wy : wx)) occ = 0u;
     const uint64_t word = __ballot(occ != 0u);
     if (lane == 0) f.xbits[((size_t)lat * tiles + tile) * (2 * T / 64) + wave] = word;
   }
@@ -737,6 +851,11 @@ __global__ __launch_bounds__(KB_THREADS) void kb_merge(LatticeSet s, FusedScratc
   }
   for (uint32_t k = tid; worker && k < ns; k += KB_WORK) {
     const uint64_t e = ent[k];
+    if (e & SLOT_ABSENT) {  // a position of a cut tile without a site
+      uf[k] = kpack(k, 0, 0);
+      mn[k] = 0xFFFFFFFFu;
+      continue;
+    }
     const uint32_t t = k / NS, loc = k % NS, rep = (uint32_t)(e & 0xFFFu);
     uf[k] = kpack(t * NS + rep, 0, 0);
     if (rep == loc) {
@@ -795,7 +914,7 @@ __global__ __launch_bounds__(KB_THREADS) void kb_merge(LatticeSet s, FusedScratc
       const uint32_t r = kroot(uf, k);
       const uint32_t m = mn[r];
       fl = m & 1u;
-      if (meas_cl && r == k) {
+      if (meas_cl && r == k && m != 0xFFFFFFFFu) {  // (absent slots are no cluster)
         const uint64_t c = cnt[r];
         const uint32_t size = (uint32_t)(c >> 32), np = (uint32_t)c;
         ++ms.ncl;
@@ -949,7 +1068,7 @@ __global__ __launch_bounds__(256) void kg_flip(LatticeSet s, FusedScratch f) {
   }
   if (!meas_cl) return;  // block-uniform
   TileMeas ms = {0, 0, 0, 0ull, 0ull, 0u, 0u};
-  if (valid && r == k) {  // roots are reps
+  if (valid && r == k && m != 0xFFFFFFFFu) {  // roots are reps (absent slots: no cluster)
     const uint64_t c = f.gcnt[base + k];
     const uint32_t size = (uint32_t)(c >> 32), np = (uint32_t)c;
     ++ms.ncl;
@@ -1021,7 +1140,7 @@ __global__ __launch_bounds__(KG_END_THREADS) void kg_sweep_end(LatticeSet s, Fus
 // ------------------------------------------------------------ run closing
 // E, M of the current state (pending flips applied), when the last sweep
 // still has to be folded.
-template <int T, int NTH, bool BIG>
+template <int T, int NTH, bool BIG, int PART>
 __global__ __launch_bounds__(NTH) void kf_measure(LatticeSet s, FusedScratch f) {
   __shared__ uint8_t bits[T * T];
   __shared__ uint8_t hsp[2 * T];
@@ -1034,12 +1153,15 @@ __global__ __launch_bounds__(NTH) void kf_measure(LatticeSet s, FusedScratch f) 
   const uint32_t nt = (uint32_t)f.tiles_x, tiles = nt * nt;
   const uint32_t tile = blockIdx.x, tx = tile % nt, ty = tile / nt;
   const uint32_t x0 = tx * T, y0 = ty * T;
+  const uint32_t wx = PART ? min((uint32_t)T, L - x0) : (uint32_t)T;
+  const uint32_t wy = PART ? min((uint32_t)T, L - y0) : (uint32_t)T;
   const FusedView v = fused_view(s, f, lat);
   const uint8_t* jb = s.jbits + (size_t)lat * N;
   const uint32_t* fb = flipbits_of<BIG>(f, lat, fb_lds, NTH);
   __syncthreads();
-  load_halo<T>(s, f, v, fb, tx, ty, hsp);
+  load_halo<T>(s, f, v, fb, tx, ty, wx, wy, hsp);
   for (uint32_t li = threadIdx.x; li < T * T; li += NTH) {
+    if (PART && (li / T >= wx || li % T >= wy)) continue;  // absent site
     const uint32_t n = (x0 + li / T) * L + y0 + li % T;
     bits[li] = (uint8_t)(true_spin<T>(v.sin[n], v.lin[n], tile, fb) | (jb[n] & 3u) << 4);
   }
@@ -1047,9 +1169,10 @@ __global__ __launch_bounds__(NTH) void kf_measure(LatticeSet s, FusedScratch f) 
   int32_t e = 0, m = 0;
   for (uint32_t li = threadIdx.x; li < T * T; li += NTH) {
     const uint32_t lx = li / T, ly = li % T;
+    if (PART && (lx >= wx || ly >= wy)) continue;
     const uint32_t b = bits[li];
-    const uint32_t sr = lx + 1u < T ? bits[li + T] & 1u : hsp[ly];
-    const uint32_t su = ly + 1u < T ? bits[li + 1u] & 1u : hsp[T + lx];
+    const uint32_t sr = lx + 1u < wx ? bits[li + T] & 1u : hsp[ly];
+    const uint32_t su = ly + 1u < wy ? bits[li + 1u] & 1u : hsp[T + lx];
     const uint32_t sh = ~(b ^ sr ^ (b >> 4)) & 1u, sv = ~(b ^ su ^ (b >> 5)) & 1u;
     e += 2 - 2 * (int32_t)(sh + sv);
     m += 2 * (int32_t)(b & 1u) - 1;
@@ -1069,7 +1192,7 @@ __global__ __launch_bounds__(NTH) void kf_measure(LatticeSet s, FusedScratch f) 
 }
 
 // Pending flips applied into buffer 0; labels cleared.
-template <int T, int NTH, bool BIG>
+template <int T, int NTH, bool BIG, int PART>
 __global__ __launch_bounds__(NTH) void kf_settle(LatticeSet s, FusedScratch f) {
   __shared__ uint32_t fb_lds[fb_words<BIG>()];
   const int lat = run_lat(s, blockIdx.y);
@@ -1081,8 +1204,11 @@ __global__ __launch_bounds__(NTH) void kf_settle(LatticeSet s, FusedScratch f) {
   int8_t* s0 = s.spins + (size_t)lat * N;
   uint16_t* l0 = f.lab + (size_t)lat * N;
   const uint32_t* fb = flipbits_of<BIG>(f, lat, fb_lds, NTH);
+  const uint32_t wx = PART ? min((uint32_t)T, L - x0) : (uint32_t)T;
+  const uint32_t wy = PART ? min((uint32_t)T, L - y0) : (uint32_t)T;
   __syncthreads();
   for (uint32_t li = threadIdx.x; li < T * T; li += NTH) {
+    if (PART && (li / T >= wx || li % T >= wy)) continue;  // absent site
     const uint32_t n = (x0 + li / T) * L + y0 + li % T;
     const uint32_t b = true_spin<T>(v.sin[n], v.lin[n], tile, fb);
     s0[n] = b ? (int8_t)1 : (int8_t)-1;
@@ -1104,13 +1230,13 @@ __global__ void kf_close(LatticeSet s, FusedScratch f) {
 
 size_t kb_lds_bytes(int nslots) { return (size_t)nslots * (8 + 8 + 4); }
 
-template <int T>
+template <int T, int PART>
 hipError_t launch_fused_t(const LatticeSet& s, const FusedScratch& f, hipStream_t st) {
   constexpr int NTH = T == 64 ? 512 : 256;
   const int tiles = f.tiles_x * f.tiles_x;
   if (f.big) {
     const int nb = (f.nslots + 255) / 256;
-    hipLaunchKernelGGL((ka_tile<T, NTH, true>), dim3(tiles, s.n_run), dim3(NTH), 0, st, s, f);
+    hipLaunchKernelGGL((ka_tile<T, NTH, true, PART>), dim3(tiles, s.n_run), dim3(NTH), 0, st, s, f);
     hipLaunchKernelGGL((kg_union<T>), dim3((f.nslots / 2 + 255) / 256, s.n_run), dim3(256), 0, st, s, f);
     hipLaunchKernelGGL(kg_fold, dim3(nb, s.n_run), dim3(256), 0, st, s, f);
     hipLaunchKernelGGL(kg_flip, dim3(nb, s.n_run), dim3(256), 0, st, s, f);
@@ -1121,24 +1247,30 @@ hipError_t launch_fused_t(const LatticeSet& s, const FusedScratch& f, hipStream_
   hipError_t e = ensure_dynamic_lds((const void*)kb_merge<T>,
                                     (int)kb_lds_bytes(RBIM_FUSED_MAX_SLOTS), configured);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((ka_tile<T, NTH, false>), dim3(tiles, s.n_run), dim3(NTH), 0, st, s, f);
+  hipLaunchKernelGGL((ka_tile<T, NTH, false, PART>), dim3(tiles, s.n_run), dim3(NTH), 0, st, s, f);
   hipLaunchKernelGGL((kb_merge<T>), dim3(s.n_run), dim3(KB_THREADS), kb_lds_bytes(f.nslots), st, s, f);
   return hipGetLastError();
 }
 
-template <int T, bool BIG>
+template <int T, bool BIG, int PART>
 hipError_t launch_fused_close_tb(const LatticeSet& s, const FusedScratch& f, hipStream_t st) {
   constexpr int NTH = T == 64 ? 512 : 256;
   const int tiles = f.tiles_x * f.tiles_x;
-  hipLaunchKernelGGL((kf_measure<T, NTH, BIG>), dim3(tiles, s.n_run), dim3(NTH), 0, st, s, f);
-  hipLaunchKernelGGL((kf_settle<T, NTH, BIG>), dim3(tiles, s.n_run), dim3(NTH), 0, st, s, f);
+  hipLaunchKernelGGL((kf_measure<T, NTH, BIG, PART>), dim3(tiles, s.n_run), dim3(NTH), 0, st, s, f);
+  hipLaunchKernelGGL((kf_settle<T, NTH, BIG, PART>), dim3(tiles, s.n_run), dim3(NTH), 0, st, s, f);
   hipLaunchKernelGGL(kf_close, dim3((s.n_run + 63) / 64), dim3(64), 0, st, s, f);
   return hipGetLastError();
 }
 
-template <int T>
+template <int T, int PART>
 hipError_t launch_fused_close_t(const LatticeSet& s, const FusedScratch& f, hipStream_t st) {
-  return f.big ? launch_fused_close_tb<T, true>(s, f, st) : launch_fused_close_tb<T, false>(s, f, st);
+  return f.big ? launch_fused_close_tb<T, true, PART>(s, f, st)
+               : launch_fused_close_tb<T, false, PART>(s, f, st);
+}
+
+// cut tiles (L no multiple of the tile side) only ever have T = 32
+int fused_part(const LatticeSet& s, const FusedScratch& f) {
+  return s.L % f.tile == 0 ? 0 : (s.L & 1) ? 2 : 1;
 }
 
 }  // namespace
@@ -1147,19 +1279,33 @@ int fused_tile_size(int L, int pref) {
   if ((pref == 32 || pref == 64) && L >= pref && L % pref == 0) return pref;
   if (L >= 64 && L % 64 == 0) return 64;
   if (L >= 32 && L % 32 == 0) return 32;
+  if (L > 32) return 32;  // whole tiles of 32 and a cut last row and column
   return 0;
 }
 
+int fused_tiles_per_side(int L, int T) { return T > 0 ? (L + T - 1) / T : 0; }
+
 bool fused_merge_in_hbm(int L, int T) {
-  return T > 0 && (size_t)(L / T) * (L / T) * 4 * T > RBIM_FUSED_MAX_SLOTS;
+  const size_t nt = (size_t)fused_tiles_per_side(L, T);
+  return T > 0 && nt * nt * 4 * T > RBIM_FUSED_MAX_SLOTS;
 }
 
 int fused_kernels_per_sweep(const FusedScratch& f) { return f.big ? 5 : 2; }
 
 hipError_t launch_fused_sweep(const LatticeSet& s, const FusedScratch& f, hipStream_t st) {
-  return f.tile == 64 ? launch_fused_t<64>(s, f, st) : launch_fused_t<32>(s, f, st);
+  if (f.tile == 64) return launch_fused_t<64, 0>(s, f, st);
+  switch (fused_part(s, f)) {
+    case 1: return launch_fused_t<32, 1>(s, f, st);
+    case 2: return launch_fused_t<32, 2>(s, f, st);
+    default: return launch_fused_t<32, 0>(s, f, st);
+  }
 }
 
 hipError_t launch_fused_close(const LatticeSet& s, const FusedScratch& f, hipStream_t st) {
-  return f.tile == 64 ? launch_fused_close_t<64>(s, f, st) : launch_fused_close_t<32>(s, f, st);
+  if (f.tile == 64) return launch_fused_close_t<64, 0>(s, f, st);
+  switch (fused_part(s, f)) {
+    case 1: return launch_fused_close_t<32, 1>(s, f, st);
+    case 2: return launch_fused_close_t<32, 2>(s, f, st);
+    default: return launch_fused_close_t<32, 0>(s, f, st);
+  }
 }

@@ -366,14 +366,17 @@ def test_run_past_end_is_refused(hip):
             batch.run(1)
 
 
-@pytest.mark.parametrize("Ls", [(32, 64, 96), (128, 160, 256), (320,), (288, 384)])
+@pytest.mark.parametrize("Ls", [(32, 64, 96), (128, 160, 256), (320,), (288, 384),
+                                (33, 47, 100, 129), (200,), (333,), (226, 131)])
 def test_edge_slot_sweep_equals_hbm_forest_sweep(hip, coracle, Ls):
-    """The edge-slot sweep (ENGINE_PLAIN when L is a multiple of 32: slots merged
-    in LDS -- 320: 6400 slots, the most one workgroup holds -- or in HBM for
-    288 = 9 x 9 tiles of 32 and 384 = 6 x 6 tiles of 64)
-    and the multi-kernel HBM-forest sweep (ENGINE_TILED) give identical
-    trajectories, moment sums and per-sweep series, across split runs; and both
-    equal the oracle."""
+    """The edge-slot sweep (ENGINE_PLAIN for L > 32: slots merged in LDS --
+    320: 6400 slots, the most one workgroup holds -- or in HBM for 288 = 9 x 9
+    tiles of 32 and 384 = 6 x 6 tiles of 64; for L no multiple of 32, tiles of
+    32 with a cut last row and column: 33, 47, 100, 129, 200, 226 and 131 merge
+    in LDS, 333 = 10 x 32 + 13 in HBM; odd L take two Philox calls for a site
+    pair that straddles two) and the multi-kernel HBM-forest sweep
+    (ENGINE_TILED) give identical trajectories, moment sums and per-sweep
+    series, across split runs; and both equal the oracle."""
     from rbim._hip import ENGINE_TILED
     from rbim.engine import Batch
 
