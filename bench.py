@@ -234,10 +234,11 @@ def cpu_baseline(specs, site_updates_per_thread, per_thread):
 
 
 # ---------------------------------------------------------------- sections
-def fused_tile_size(L):
-    """Tile side of the edge-slot sweep for L (0: L uses the multi-kernel tiled
-    sweep); mirrors fused_tile_size() of the library."""
-    return 64 if L >= 64 and L % 64 == 0 else 32 if L >= 32 else 0
+def uses_edge_slot(L):
+    """Whether the default tiled engine runs L on the edge-slot sweep (every
+    L >= 32, on tiles of 32 or 64 with the last row and column cut when the
+    side does not divide L); smaller L take the multi-kernel tiled sweep."""
+    return L >= 32
 
 
 def measure_point(L, args, steps, warmup, plain_steps, with_cpu):
@@ -260,7 +261,7 @@ def measure_point(L, args, steps, warmup, plain_steps, with_cpu):
     else:
         # the tiled path's launch: one sweep of its kernels over all lattices
         profile = "edge_slot_sweep_L256_batched.md" if (L, n_lat) == (256, 1280) and default_shape else None
-        kernel = ("edge-slot sweep: ka_tile + kb_merge" if fused_tile_size(L) and args.engine != 3
+        kernel = ("edge-slot sweep: ka_tile + kb_merge" if uses_edge_slot(L) and args.engine != 3
                   else "tiled sweep: k1_tile_full + k2_merge + k3_flip + k5_finalize")
         roof = roofline(kernel, res["sites"], res["ms"] / steps, steps, profile,
                         "traffic_bytes_per_sweep")

@@ -90,9 +90,10 @@ struct FusedScratch {
   uint32_t* wrapw;     // [n_lat] wrap bits of the sweep
   int32_t tile, tiles_x, nslots, big;
 };
-// Tile side for L (pref: 32 or 64 when it divides L, else the default: 64
-// when it divides L, else 32 -- with a cut last tile row and column when 32
-// does not divide L either); 0 for L <= 32 not a multiple of 32.
+// Tile side for L >= 32 (pref: 32, or 64 for L >= 64; else 64 when it
+// divides L, otherwise 32 while tiles of 32 merge their slots in LDS, 64
+// beyond), the last tile row and column cut when the side does not divide L;
+// 0 for L < 32.
 int fused_tile_size(int L, int pref = 0);
 int fused_tiles_per_side(int L, int T);
 bool fused_merge_in_hbm(int L, int T);

@@ -74,19 +74,35 @@ __device__ __forceinline__ uint32_t slot_li(uint32_t k) {
 }
 
 // Edge slot k of a tile whose valid part is wx x wy sites (wx = wy = T for a
-// whole tile): its local index, and whether the position holds a site (the
-// sides of a cut tile are shorter than T).
+// whole tile): its local index in rows of 2^rsh sites, and whether the
+// position holds a site (the sides of a cut tile are shorter than T).
 template <int T>
-__device__ __forceinline__ uint32_t slot_li_cut(uint32_t k, uint32_t wx, uint32_t wy, bool& ok) {
+__device__ __forceinline__ uint32_t slot_li_cut(uint32_t k, uint32_t wx, uint32_t wy, uint32_t rsh,
+                                                bool& ok) {
   const uint32_t side = k / T, i = k % T;
   ok = side < 2u ? i < wy : i < wx;
   const uint32_t lx = side == 0 ? 0u : side == 1 ? wx - 1u : i;
   const uint32_t ly = side == 2 ? 0u : side == 3 ? wy - 1u : i;
-  return lx * T + ly;
+  return (lx << rsh) + ly;
 }
 
 // Slot entry of a position of a cut tile that holds no site.
 constexpr uint64_t SLOT_ABSENT = 1ull << 63;
+
+// Site pairs per row of KA's lane map, as a shift.  A whole row holds T / 2
+// pairs; a tile whose rows are cut to wy < T sites packs each row into the
+// smallest power of two of pairs that holds them, so that lanes, and whole
+// wavefronts, follow the sites that are present.  A row never spans two
+// wavefronts (T / 2 <= 64), and a wavefront never spans more than T rows (at
+// least 64 / T pairs a row), so the rows of a wavefront that starts inside
+// the tile end inside it.
+template <int T>
+__device__ __forceinline__ uint32_t pair_shift(uint32_t wy) {
+  constexpr uint32_t lo = T >= 64 ? 0u : (uint32_t)__builtin_ctz(64 / T);
+  const uint32_t pairs = (wy + 1u) >> 1;
+  const uint32_t sh = pairs <= 1u ? 0u : 32u - (uint32_t)__builtin_clz(pairs - 1u);
+  return sh > lo ? sh : lo;
+}
 
 __device__ __forceinline__ uint32_t flip_of(const uint32_t* fb, uint32_t slot) {
   return (fb[slot >> 5] >> (slot & 31u)) & 1u;
@@ -187,22 +203,34 @@ __device__ __forceinline__ void troot16_4(const uint32_t* par, uint32_t (&x)[4])
 }
 
 // ------------------------------------------------------------------- KA
-// PART: 0 -- L a multiple of T, every tile whole; 1 -- L even, the last tile
-// row and column cut (wx, wy < T; site pairs stay whole and aligned); 2 -- L
-// odd, cut as well, so the last pair of a cut row holds one site and a pair of
-// a row starting at an odd site straddles two Philox pairs (two draws).
+// LDS of one tile pass.
+template <int T, int NTH>
+struct KaShared {
+  uint32_t par[T * T];
+  __align__(16) uint32_t lcnt[T * T];
+  uint8_t bits[T * T];  // bit0 spin, bit1 occ_h, bit2 occ_v, bit3 flip draw, bit4/5 J<0
+  uint8_t hsp[2 * T];
+  int64_t red[NTH / 64][PARTS];
+};
+
+// PART: 0 -- a whole tile (wx = wy = T) of a lattice of even L; 1 -- L even,
+// the last tile row and column cut (wx, wy < T; site pairs stay whole and
+// aligned); 2 -- L odd, cut as well, so the last pair of a cut row holds one
+// site and a pair of a row starting at an odd site straddles two Philox pairs
+// (two draws).
 template <int T, int NTH, bool BIG, int PART>
-__global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
+__device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch& f,
+                                        KaShared<T, NTH>& sm) {
   constexpr int ITERS = T * T / 2 / NTH;
   constexpr uint32_t NS = 4u * T;  // edge slots per tile
   constexpr uint32_t QW = T * T / (NTH / 64);  // union queue entries per wave (>= 192)
   static_assert(QW >= 192, "a wave queues up to 3 unions per lane");
   static_assert(QW % 4u == 0u, "queues are cleared with 16-byte stores");
-  __shared__ uint32_t par[T * T];
-  __shared__ __align__(16) uint32_t lcnt[T * T];
-  __shared__ uint8_t bits[T * T];  // bit0 spin, bit1 occ_h, bit2 occ_v, bit3 flip draw, bit4/5 J<0
-  __shared__ uint8_t hsp[2 * T];
-  __shared__ int64_t red[NTH / 64][PARTS];
+  uint32_t* par = sm.par;
+  uint32_t* lcnt = sm.lcnt;
+  uint8_t* bits = sm.bits;
+  uint8_t* hsp = sm.hsp;
+  int64_t (*red)[PARTS] = sm.red;
   const int lat = run_lat(s, blockIdx.y);
   if (!lattice_active(s, lat)) return;
   const uint32_t L = (uint32_t)s.L, N = (uint32_t)s.N;
@@ -216,6 +244,27 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
   const size_t base = (size_t)lat * N;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint64_t below = (1ull << lane) - 1ull;
+  // lane map: queue position ql = it * NTH + tid holds the site pair (lx, ly0),
+  // (lx, ly0 + 1), local index li0 = lx * RS + ly0 in rows of RS = 2^rsh sites
+  // (T in a whole tile, where li0 = 2 ql; a tile with cut rows packs them to
+  // twice its pairs per row, so a wavefront's LDS accesses spread over the
+  // banks as in a whole tile).  In a cut tile the wavefronts whose pairs all
+  // lie past row wx - 1 skip every phase that works pair by pair (their sites
+  // are absent, and no present site reads the local state of an absent row).
+  // Slot entries and site numbers use the tile's own rows (stride T).
+  const uint32_t psh = PART ? pair_shift<T>(wy) : (uint32_t)__builtin_ctz(T / 2);
+  const uint32_t rsh = psh + 1u, RS = 1u << rsh;
+  const uint32_t wv = (uint32_t)__builtin_amdgcn_readfirstlane(wave);
+  auto row_of = [&](uint32_t ql) -> uint32_t { return ql >> psh; };
+  auto col_of = [&](uint32_t ql) -> uint32_t { return (ql & ((1u << psh) - 1u)) << 1; };
+  auto li_of = [&](uint32_t ql) -> uint32_t { return PART ? (row_of(ql) << rsh) + col_of(ql) : ql << 1; };
+  // a local index as the tile-row index lx * T + ly
+  auto tile_li = [&](uint32_t li) -> uint32_t {
+    return PART ? (li >> rsh) * T + (li & (RS - 1u)) : li;
+  };
+  auto skip = [&](int it) -> bool {
+    return PART && (((uint32_t)it * NTH + wv * 64u) >> psh) >= wx;
+  };
   const int64_t g = s.sweep_ctr[lat];
   const uint32_t thr = s.thr31[lat];
   const uint64_t seed = s.seed[lat];
@@ -248,7 +297,7 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
 #pragma unroll
   for (int it = 0; it < ITERS; ++it) {
     const uint32_t ql = (uint32_t)(it * NTH + tid);
-    const uint32_t lx = ql / (T / 2), ly0 = (ql % (T / 2)) * 2u;
+    const uint32_t lx = row_of(ql), ly0 = col_of(ql);
     const uint32_t n0 = (x0 + lx) * L + y0 + ly0;  // even L: the pair loads are aligned
     if (PART == 0) {
       sp2[it] = *reinterpret_cast<const uint16_t*>(v.sin + n0);
@@ -260,6 +309,7 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
       sp2[it] = 0x0101u;
       lb2[it] = 0u;
       j2[it] = 0u;
+      if (skip(it)) continue;
       if (PART == 1) {
         if (v0) {
           sp2[it] = *reinterpret_cast<const uint16_t*>(v.sin + n0);
@@ -303,7 +353,8 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
   };
 #pragma unroll
   for (int it = 0; it < ITERS; ++it) {
-    const uint32_t li0 = (uint32_t)(it * NTH + tid) << 1;
+    if (skip(it)) continue;
+    const uint32_t li0 = li_of((uint32_t)(it * NTH + tid));
     const uint32_t t0 = spin_bit((int8_t)(sp2[it] & 0xFFu), lb2[it] & 0xFFFFu, 0u);
     const uint32_t t1 = spin_bit((int8_t)(sp2[it] >> 8), lb2[it] >> 16, 0u);
     bits[li0] = (uint8_t)(t0 | (j2[it] & 3u) << 4);
@@ -319,16 +370,17 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
   int32_t me = 0, mm = 0;
 #pragma unroll
   for (int it = 0; it < ITERS; ++it) {
+    if (skip(it)) continue;
     const uint32_t ql = (uint32_t)(it * NTH + tid);
-    const uint32_t lx = ql / (T / 2), ly0 = (ql % (T / 2)) * 2u;
-    const uint32_t li0 = ql << 1, li1 = li0 + 1u;
+    const uint32_t lx = row_of(ql), ly0 = col_of(ql);
+    const uint32_t li0 = li_of(ql), li1 = li0 + 1u;
     const uint32_t n0 = (x0 + lx) * L + y0 + ly0;
     const uint32_t b0 = bits[li0], b1 = bits[li1];
     // the x neighbour is in the tile below row wx-1, the y neighbour of n1 below
     // column wy-1 (of n0: n1 itself, unless n0 is a cut row's last site)
     const bool v0 = !PART || (lx < wx && ly0 < wy), v1 = !PART || (lx < wx && ly0 + 1u < wy);
-    const uint32_t sr0 = lx + 1u < wx ? bits[li0 + T] & 1u : hsp[ly0];
-    const uint32_t sr1 = lx + 1u < wx ? bits[li1 + T] & 1u : hsp[ly0 + 1u];
+    const uint32_t sr0 = lx + 1u < wx ? bits[li0 + RS] & 1u : hsp[ly0];
+    const uint32_t sr1 = lx + 1u < wx ? bits[li1 + RS] & 1u : hsp[ly0 + 1u];
     const uint32_t su0 = PART == 2 && ly0 + 1u >= wy ? hsp[T + lx] : b1 & 1u;
     const uint32_t su1 = ly0 + 2u < wy ? bits[li1 + 1u] & 1u : hsp[T + lx];
     // satisfied: J s_a s_b = +1, i.e. equal spins xor J < 0 (absent: never)
@@ -367,7 +419,9 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
     // n0 -> n1, same row and tile (for a cut row's last site, the bond leaves)
     const bool internal = ov0 != 0u && (PART != 2 || ly0 + 1u < wy);
     const bool outlink = ov1 != 0u && ly0 + 2u < wy;  // n1 -> n1+1 inside the tile
-    const uint32_t chunk = 2u * (uint32_t)(it * NTH + wave * 64);
+    // local index of the wave's lane 0 as seen from this lane's row (a run
+    // never leaves its row)
+    const uint32_t chunk = PART ? li0 - 2u * (uint32_t)lane : 2u * (uint32_t)(it * NTH + wave * 64);
     const uint32_t start0 = run_start(__ballot(internal && outlink), __ballot(outlink), below, chunk);
     par[li0] = start0;
     par[li1] = internal ? start0 : li1;
@@ -389,24 +443,25 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
   // wave's chunk
 #pragma unroll 1
   for (int it = 0; it < ITERS; ++it) {
+    if (skip(it)) continue;
     const uint32_t ql = (uint32_t)(it * NTH + tid);
-    const uint32_t lx = ql / (T / 2), ly0 = (ql % (T / 2)) * 2u;
-    const uint32_t li0 = ql << 1, li1 = li0 + 1u;
+    const uint32_t lx = row_of(ql), ly0 = col_of(ql);
+    const uint32_t li0 = li_of(ql), li1 = li0 + 1u;
     const uint32_t b0 = bits[li0], b1 = bits[li1];
     uint32_t req = 0u;
     if (lx + 1u < wx) {
       const bool first = ly0 == 0u;  // no plaquette to the left
       const uint32_t lf = first ? li0 : li0 - 1u;
-      const uint32_t left = bits[lf], left_up = bits[lf + T];
+      const uint32_t left = bits[lf], left_up = bits[lf + RS];
       const uint32_t h0 = (b0 >> 1) & 1u &
                           ~((left >> 1) & (left >> 2) & (left_up >> 2) & (first ? 0u : 1u));
-      const uint32_t h1 = (b1 >> 1) & 1u & ~((b0 >> 1) & (b0 >> 2) & (bits[li0 + T] >> 2));
+      const uint32_t h1 = (b1 >> 1) & 1u & ~((b0 >> 1) & (b0 >> 2) & (bits[li0 + RS] >> 2));
       req = h0 | (h1 << 1);
     }
     req |= ((b1 >> 2) & 1u & ((lane == 63 && ly0 + 2u < wy) ? 1u : 0u)) << 2;
     // the wave's queue lives in lcnt, which is cleared before it is used
     const uint32_t qa[3] = {li0, li1, li1};
-    const uint32_t qb[3] = {li0 + T, li1 + T, li1 + 1u};
+    const uint32_t qb[3] = {li0 + RS, li1 + RS, li1 + 1u};
     wave_union_queue(par, lcnt + wave * QW, req, qa, qb, below);
   }
   {  // this wave's queue, cleared with 16-byte stores (QW is a multiple of 4)
@@ -446,14 +501,23 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
     // two pairs per step: four root walks in flight together
 #pragma unroll 1
     for (int it = 0; it < ITERS; it += 2) {
-      const uint32_t la = (uint32_t)(it * NTH + tid) << 1, lb = la + 2u * NTH;
-      const uint32_t ba0 = bits[la], ba1 = bits[la + 1u], bb0 = bits[lb], bb1 = bits[lb + 1u];
+      if (skip(it)) continue;  // and so is it + 1
+      const uint32_t qla = (uint32_t)(it * NTH + tid), qlb = qla + NTH;
+      const uint32_t la = li_of(qla), lb = li_of(qlb);
+      const uint32_t ba0 = bits[la], ba1 = bits[la + 1u];
       bool va0 = true, va1 = true, vb0 = true, vb1 = true;
       if (PART) {
-        const uint32_t xa = la / T, ya = la % T, xb = lb / T, yb = lb % T;
+        const uint32_t xa = row_of(qla), ya = col_of(qla), xb = row_of(qlb), yb = col_of(qlb);
         va0 = xa < wx && ya < wy, va1 = xa < wx && ya + 1u < wy;
         vb0 = xb < wx && yb < wy, vb1 = xb < wx && yb + 1u < wy;
+        if (skip(it + 1)) {  // the second pair's wavefront is past the cut
+          uint32_t r0 = la, r1 = (ba0 & 4u) && va1 ? la : la + 1u;
+          troot16_2(par, r0, r1);
+          count_and_point(la, ba0, ba1, r0, r1, va0, va1);
+          continue;
+        }
       }
+      const uint32_t bb0 = bits[lb], bb1 = bits[lb + 1u];
       // li1 shares li0's run when the in-row bond is occupied
       uint32_t r[4] = {la, (ba0 & 4u) && va1 ? la : la + 1u, lb, (bb0 & 4u) && vb1 ? lb : lb + 1u};
       troot16_4(par, r);
@@ -463,10 +527,12 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
   } else {
 #pragma unroll 1
     for (int it = 0; it < ITERS; ++it) {
-      const uint32_t li0 = (uint32_t)(it * NTH + tid) << 1, li1 = li0 + 1u;
+      if (skip(it)) continue;
+      const uint32_t ql = (uint32_t)(it * NTH + tid);
+      const uint32_t li0 = li_of(ql), li1 = li0 + 1u;
       const uint32_t b0 = bits[li0], b1 = bits[li1];
-      const bool v0 = !PART || (li0 / T < wx && li0 % T < wy);
-      const bool v1 = !PART || (li0 / T < wx && li0 % T + 1u < wy);
+      const bool v0 = !PART || (row_of(ql) < wx && col_of(ql) < wy);
+      const bool v1 = !PART || (row_of(ql) < wx && col_of(ql) + 1u < wy);
       uint32_t r0 = li0, r1 = (b0 & 4u) && v1 ? li0 : li1;  // li1 shares li0's run when linked
       troot16_2(par, r0, r1);
       count_and_point(li0, b0, b1, r0, r1, v0, v1);
@@ -476,7 +542,7 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
   // rep slot of every edge cluster: par[root] = root | min slot << 16
   for (uint32_t k = tid; k < NS; k += NTH) {
     bool ok = true;
-    const uint32_t li = PART ? slot_li_cut<T>(k, wx, wy, ok) : slot_li<T>(k);
+    const uint32_t li = PART ? slot_li_cut<T>(k, wx, wy, rsh, ok) : slot_li<T>(k);
     if (!ok) continue;
     const uint32_t r = par[li] & 0xFFFFu;
     atomicMin(&par[r], r | (k << 16));
@@ -490,9 +556,10 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
   uint32_t ik1 = 0u, ik2 = 0u, incl = 0u;
 #pragma unroll
   for (int it = 0; it < ITERS; ++it) {
+    if (skip(it)) continue;
     const uint32_t ql = (uint32_t)(it * NTH + tid);
-    const uint32_t lx = ql / (T / 2), ly0 = (ql % (T / 2)) * 2u;
-    const uint32_t li0 = ql << 1;
+    const uint32_t lx = row_of(ql), ly0 = col_of(ql);
+    const uint32_t li0 = li_of(ql);
     const uint32_t n0 = (x0 + lx) * L + y0 + ly0;
     const bool vs[2] = {!PART || (lx < wx && ly0 < wy), !PART || (lx < wx && ly0 + 1u < wy)};
     uint32_t sp2 = 0u, lb2 = 0u;
@@ -537,7 +604,7 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
     const size_t sb = (size_t)lat * f.nslots + (size_t)tile * NS;
     for (uint32_t k = tid; k < NS; k += NTH) {
       bool ok = true;
-      const uint32_t li = PART ? slot_li_cut<T>(k, wx, wy, ok) : slot_li<T>(k);
+      const uint32_t li = PART ? slot_li_cut<T>(k, wx, wy, rsh, ok) : slot_li<T>(k);
       if (!ok) {  // no site here: a slot of its own, never folded or counted
         f.guf[sb + k] = gpack(tile * NS + k, 0, 0);
         f.gmn[sb + k] = 0xFFFFFFFFu;
@@ -549,7 +616,8 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
       if (rep == k) {
         const uint32_t c = lcnt[r];
         f.gcnt[sb + k] = ((uint64_t)(c >> 16) << 32) | (c & 0xFFFFu);
-        f.gmn[sb + k] = (((x0 + r / T) * L + y0 + r % T) << 1) | ((bits[r] >> 3) & 1u);
+        const uint32_t rt = tile_li(r);
+        f.gmn[sb + k] = (((x0 + rt / T) * L + y0 + rt % T) << 1) | ((bits[r] >> 3) & 1u);
       } else {
         f.gmn[sb + k] = 0xFFFFFFFFu;
       }
@@ -558,7 +626,7 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
     uint64_t* sl = f.slots + (size_t)lat * f.nslots + (size_t)tile * NS;
     for (uint32_t k = tid; k < NS; k += NTH) {
       bool ok = true;
-      const uint32_t li = PART ? slot_li_cut<T>(k, wx, wy, ok) : slot_li<T>(k);
+      const uint32_t li = PART ? slot_li_cut<T>(k, wx, wy, rsh, ok) : slot_li<T>(k);
       if (!ok) {
         sl[k] = SLOT_ABSENT;
         continue;
@@ -566,13 +634,14 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
       const uint32_t r = par[li] & 0xFFFFu;
       const uint32_t rep = par[r] >> 16;
       const uint32_t c = lcnt[r];
-      sl[k] = slot_entry(rep, r, (bits[r] >> 3) & 1u, c >> 16, c & 0xFFFFu);
+      sl[k] = slot_entry(rep, tile_li(r), (bits[r] >> 3) & 1u, c >> 16, c & 0xFFFFu);
     }
   }
   if (tid < 2 * T) {
     // tid < T: +x bond of (x0+wx-1, y0+tid); else +y bond of (x0+tid-T, y0+wy-1)
     const uint32_t i = tid < T ? (uint32_t)tid : (uint32_t)tid - T;
-    uint32_t occ = tid < T ? (bits[(wx - 1u) * T + i] >> 1) & 1u : (bits[i * T + wy - 1u] >> 2) & 1u;
+    uint32_t occ = tid < T ? (bits[((wx - 1u) << rsh) + i] >> 1) & 1u
+                           : (bits[(i << rsh) + wy - 1u] >> 2) & 1u;
     if (PART && i >= (tid < T ? wy : wx)) occ = 0u;
     const uint64_t word = __ballot(occ != 0u);
     if (lane == 0) f.xbits[((size_t)lat * tiles + tile) * (2 * T / 64) + wave] = word;
@@ -608,7 +677,8 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
         if (ks[q] != 0u) {
           const uint32_t li = 0xFFFu - (ks[q] & 0xFFFu);
           const uint32_t c = lcnt[li], size = c >> 16, np = c & 0xFFFFu;
-          const uint32_t site = (x0 + li / T) * L + y0 + li % T;
+          const uint32_t lt = tile_li(li);
+          const uint32_t site = (x0 + lt / T) * L + y0 + lt % T;
           key = (int64_t)(((uint64_t)size << 32) | (uint64_t)(0xFFFFFFFFu - site));
           np_after = ((bits[li] >> 3) & 1u) ? size - np : np;
         }
@@ -617,6 +687,26 @@ __global__ __launch_bounds__(NTH) void ka_tile(LatticeSet s, FusedScratch f) {
       }
     }
   }
+}
+
+// Eight wavefronts per SIMD: four 512-thread workgroups of 64 x 64 tiles (their
+// LDS allows four per CU) or eight of 32 x 32 tiles per CU, which caps the
+// scalar registers below what the cut-tile kernels would otherwise take (100,
+// enough for seven).
+template <int T, int NTH, bool BIG, int PART>
+__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(8, 8))) void ka_tile(
+    LatticeSet s, FusedScratch f) {
+  __shared__ KaShared<T, NTH> sm;
+  if constexpr (PART == 1) {
+    // the whole tiles of a cut lattice (all but the last tile row and column)
+    // take the whole-tile code: the choice is the same for the whole workgroup
+    const uint32_t nt = (uint32_t)f.tiles_x;
+    if (blockIdx.x % nt + 1u < nt && blockIdx.x / nt + 1u < nt) {
+      ka_body<T, NTH, BIG, 0>(s, f, sm);
+      return;
+    }
+  }
+  ka_body<T, NTH, BIG, PART>(s, f, sm);
 }
 
 // ------------------------------------------------------------------- KB
@@ -1268,19 +1358,22 @@ hipError_t launch_fused_close_t(const LatticeSet& s, const FusedScratch& f, hipS
                : launch_fused_close_tb<T, false, PART>(s, f, st);
 }
 
-// cut tiles (L no multiple of the tile side) only ever have T = 32
 int fused_part(const LatticeSet& s, const FusedScratch& f) {
   return s.L % f.tile == 0 ? 0 : (s.L & 1) ? 2 : 1;
 }
 
 }  // namespace
 
+// Measured on 1280-lattice scans: where 64 does not divide L, tiles of 32 are
+// as fast or faster while their slots still merge in LDS (L <= 224: 1.04e11
+// site-updates/s at L = 224 against 1.00e11 on cut tiles of 64, 8.6e10 against
+// 8.0e10 at L = 200), cut tiles of 64 beyond (L = 288: 1.21e11 against 8.0e10
+// for 32 x 32 tiles merged in HBM, L = 333: 7.6e10 against 6.8e10).
 int fused_tile_size(int L, int pref) {
-  if ((pref == 32 || pref == 64) && L >= pref && L % pref == 0) return pref;
-  if (L >= 64 && L % 64 == 0) return 64;
-  if (L >= 32 && L % 32 == 0) return 32;
-  if (L > 32) return 32;  // whole tiles of 32 and a cut last row and column
-  return 0;
+  if (L < 32) return 0;
+  if (pref == 32 || (pref == 64 && L >= 64)) return pref;  // cut tiles when T does not divide L
+  if (L % 64 == 0) return 64;
+  return fused_merge_in_hbm(L, 32) ? 64 : 32;
 }
 
 int fused_tiles_per_side(int L, int T) { return T > 0 ? (L + T - 1) / T : 0; }
@@ -1293,19 +1386,23 @@ bool fused_merge_in_hbm(int L, int T) {
 int fused_kernels_per_sweep(const FusedScratch& f) { return f.big ? 5 : 2; }
 
 hipError_t launch_fused_sweep(const LatticeSet& s, const FusedScratch& f, hipStream_t st) {
-  if (f.tile == 64) return launch_fused_t<64, 0>(s, f, st);
-  switch (fused_part(s, f)) {
-    case 1: return launch_fused_t<32, 1>(s, f, st);
-    case 2: return launch_fused_t<32, 2>(s, f, st);
-    default: return launch_fused_t<32, 0>(s, f, st);
-  }
+  const int part = fused_part(s, f);
+  if (f.tile == 64)
+    return part == 1 ? launch_fused_t<64, 1>(s, f, st)
+         : part == 2 ? launch_fused_t<64, 2>(s, f, st)
+                     : launch_fused_t<64, 0>(s, f, st);
+  return part == 1 ? launch_fused_t<32, 1>(s, f, st)
+       : part == 2 ? launch_fused_t<32, 2>(s, f, st)
+                   : launch_fused_t<32, 0>(s, f, st);
 }
 
 hipError_t launch_fused_close(const LatticeSet& s, const FusedScratch& f, hipStream_t st) {
-  if (f.tile == 64) return launch_fused_close_t<64, 0>(s, f, st);
-  switch (fused_part(s, f)) {
-    case 1: return launch_fused_close_t<32, 1>(s, f, st);
-    case 2: return launch_fused_close_t<32, 2>(s, f, st);
-    default: return launch_fused_close_t<32, 0>(s, f, st);
-  }
+  const int part = fused_part(s, f);
+  if (f.tile == 64)
+    return part == 1 ? launch_fused_close_t<64, 1>(s, f, st)
+         : part == 2 ? launch_fused_close_t<64, 2>(s, f, st)
+                     : launch_fused_close_t<64, 0>(s, f, st);
+  return part == 1 ? launch_fused_close_t<32, 1>(s, f, st)
+       : part == 2 ? launch_fused_close_t<32, 2>(s, f, st)
+                   : launch_fused_close_t<32, 0>(s, f, st);
 }

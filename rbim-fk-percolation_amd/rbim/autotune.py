@@ -6,8 +6,8 @@ keeps the fastest:
 
 * LDS-resident kernel (L <= 128): workgroup size, i.e. wavefronts per lattice
   (64 ... 1024 threads), and sweeps per launch;
-* tiled kernels (L > 128): tile side (32, or 64 when it divides L) and sweeps
-  per captured hipGraph.
+* tiled kernels (L > 128): tile side (32 or 64, the last tile row and column
+  cut when it does not divide L) and sweeps per captured hipGraph.
 
 Results go to a JSON cache keyed by GPU architecture, engine kind (LDS or
 tiled), L and a lattice-count bucket (``$RBIM_AUTOTUNE_CACHE``, default ``~/.cache/rbim-hip/autotune.json``);
@@ -116,7 +116,7 @@ def tune(L: int, n_lattices: int = 1280, sweeps: int = 2000, store: bool = True)
     if lds:
         shapes = [(t, k, 0) for t in LDS_THREADS for k in LDS_SWEEPS if k <= sweeps]
     else:
-        tiles = [T for T in (32, 64) if L % T == 0] or [0]
+        tiles = [T for T in (32, 64) if L >= T]  # cut tiles where T does not divide L
         shapes = [(0, k, T) for T in tiles for k in GRAPH_SWEEPS if k <= sweeps]
     trials = []
     for threads, per_launch, tile in shapes:

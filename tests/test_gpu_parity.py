@@ -367,16 +367,17 @@ def test_run_past_end_is_refused(hip):
 
 
 @pytest.mark.parametrize("Ls", [(32, 64, 96), (128, 160, 256), (320,), (288, 384),
-                                (33, 47, 100, 129), (200,), (333,), (226, 131)])
+                                (33, 47, 100, 129), (200,), (333,), (226, 131), (352,)])
 def test_edge_slot_sweep_equals_hbm_forest_sweep(hip, coracle, Ls):
-    """The edge-slot sweep (ENGINE_PLAIN for L > 32: slots merged in LDS --
-    320: 6400 slots, the most one workgroup holds -- or in HBM for 288 = 9 x 9
-    tiles of 32 and 384 = 6 x 6 tiles of 64; for L no multiple of 32, tiles of
-    32 with a cut last row and column: 33, 47, 100, 129, 200, 226 and 131 merge
-    in LDS, 333 = 10 x 32 + 13 in HBM; odd L take two Philox calls for a site
-    pair that straddles two) and the multi-kernel HBM-forest sweep
-    (ENGINE_TILED) give identical trajectories, moment sums and per-sweep
-    series, across split runs; and both equal the oracle."""
+    """The edge-slot sweep (ENGINE_PLAIN for L >= 32) on its default tiles and
+    on tiles of 32 and of 64 forced, and the multi-kernel HBM-forest sweep
+    (ENGINE_TILED), give identical trajectories, moment sums and per-sweep
+    series, across split runs; and both equal the oracle.  The shapes cover
+    whole tiles, tiles with a cut last row and column (every L that the side
+    does not divide; odd L take two Philox calls for a site pair that
+    straddles two, and cut tiles of 64 pack their rows), slots merged in LDS
+    (up to 6400: 320 = 10 x 10 tiles of 32) and in HBM (288 = 9 x 9 and 333 =
+    11 x 11 tiles of 32, 333 = 6 x 6 and 352, 384 = 6 x 6 tiles of 64)."""
     from rbim._hip import ENGINE_TILED
     from rbim.engine import Batch
 
@@ -384,9 +385,10 @@ def test_edge_slot_sweep_equals_hbm_forest_sweep(hip, coracle, Ls):
     n_therm, spb, n_bins = 5, 4, 3
     total = n_therm + spb * n_bins
     out = []
-    for engine, per_launch in [(PLAIN, 3), (ENGINE_TILED, 4), (PLAIN, 1)]:
+    for engine, per_launch, tile in [(PLAIN, 3, 0), (ENGINE_TILED, 4, 0), (PLAIN, 1, 0),
+                                     (PLAIN, 2, 32), (PLAIN, 2, 64)]:
         with Batch(specs, n_therm=n_therm, n_bins=n_bins, sweeps_per_bin=spb, engine=engine,
-                   sweeps_per_launch=per_launch, series=spb * n_bins) as batch:
+                   sweeps_per_launch=per_launch, tile=tile, series=spb * n_bins) as batch:
             batch.run(2)   # the first run ends inside thermalisation
             batch.run(7)   # this one inside a bin
             batch.run()
