@@ -145,7 +145,7 @@ size_t group_bytes(int L, int n_lat, bool lds, bool fused, int n_bins, int serie
   if (fused) {
     const int T = fused_tile_size(L, tile_pref);
     const size_t nt = (size_t)fused_tiles_per_side(L, T), tiles = nt * nt, ns = tiles * 4 * T;
-    b += N * n_lat * (1 + 4) + (size_t)n_lat * (tiles * (2 * T / 64) * 8 + ns / 8 +
+    b += N * n_lat * 4 + (size_t)n_lat * (tiles * T * T / 4 + tiles * (2 * T / 64) * 8 + ns / 8 +
                                                  tiles * (2 * 5 + 2) * 8 + 2 * 6 * 8 + 8 + 4);
     if (fused_merge_in_hbm(L, T))
       b += (size_t)n_lat * (ns * (8 + 8 + 4) + (ns + 255) / 256 * 5 * 8 + 4);
@@ -260,8 +260,9 @@ int group_alloc(rbim_ctx* ctx, Group& g, const std::vector<uint8_t>& jbits,
       hipError_t e = g.arena.alloc(q, bytes);
       return e == hipSuccess ? hipMemsetAsync(*q, 0, bytes, ctx->stream) : e;
     };
-    HIP_TRY(ctx, zalloc(&p, NN)); f.spins_alt = (int8_t*)p;
-    HIP_TRY(ctx, zalloc(&p, 2 * NN * 2)); f.lab = (uint16_t*)p;
+    HIP_TRY(ctx, zalloc(&p, 2 * NN * 2)); f.st = (uint16_t*)p;
+    HIP_TRY(ctx, zalloc(&p, nl * (size_t)f.tiles_x * f.tiles_x * f.tile * f.tile / 4));
+    f.jt = (uint32_t*)p;
     f.big = fused_merge_in_hbm(g.L, f.tile) ? 1 : 0;
     if (f.big) {
       HIP_TRY(ctx, zalloc(&p, nl * f.nslots * 8)); f.guf = (uint64_t*)p;
@@ -374,8 +375,19 @@ int group_close(rbim_ctx* ctx, Group& g, hipStream_t stream, int64_t* launches) 
   return RBIM_OK;
 }
 
+// Before the first sweep of a run of a group: the edge-slot sweep's state
+// words from the group's spins.
+int group_open(rbim_ctx* ctx, Group& g, hipStream_t stream, int64_t* launches) {
+  if (!g.fused) return RBIM_OK;
+  HIP_TRY(ctx, launch_fused_open(g.set, g.fz, stream));
+  ++*launches;
+  return RBIM_OK;
+}
+
 // Queue `n` sweeps of one group on `stream`.
 int group_run(rbim_ctx* ctx, Group& g, int64_t n, hipStream_t stream, int64_t* launches) {
+  const int rc0 = group_open(ctx, g, stream, launches);
+  if (rc0 != RBIM_OK) return rc0;
   while (n > 0) {
     int64_t q = 0;
     const int rc = group_chunk(ctx, g, n, stream, launches, &q);
@@ -1182,7 +1194,11 @@ int rbim_batch_run(rbim_batch* b, int64_t n_sweeps) {
     // host that queued one group's whole run first would block on that
     // stream's full queue before the next group had any work on the device.
     HIP_TRY(ctx, hipEventRecord(b->fork, b->stream));
-    for (Group& g : b->groups) HIP_TRY(ctx, hipStreamWaitEvent(g.stream, b->fork, 0));
+    for (Group& g : b->groups) {
+      HIP_TRY(ctx, hipStreamWaitEvent(g.stream, b->fork, 0));
+      const int rc = group_open(ctx, g, g.stream, &b->launches);
+      if (rc != RBIM_OK) return rc;
+    }
     // one launch for the LDS sets: the lead LDS group's turn queues it
     const LatticeSet* sets[4] = {nullptr, nullptr, nullptr, nullptr};
     if (b->ragged)
@@ -1459,7 +1475,8 @@ int rbim_pool_create(rbim_ctx* ctx, const rbim_pool_desc* d, rbim_pool** out) {
   if (g.fused) {
     FusedScratch& f = g.fz;
     const uint64_t tiles = (uint64_t)f.tiles_x * f.tiles_x, ns = (uint64_t)f.nslots;
-    add(f.lab, 2 * N, 2, nl * N * 2);
+    add(f.st, 2 * N, 2, nl * N * 2);
+    add(f.jt, tiles * f.tile * f.tile / 4);
     add(f.slots, ns * 8);
     add(f.guf, ns * 8);
     add(f.gcnt, ns * 8);

@@ -72,8 +72,10 @@ int lds_blocks_per_cu(int L, int threads);
 // one workgroup's LDS (fused_engine.hip); device pointers, [n_lat][N] unless noted.
 #define RBIM_FUSED_MAX_SLOTS 6400  // slots merged in one workgroup's LDS
 struct FusedScratch {
-  int8_t* spins_alt;   // second spin buffer
-  uint16_t* lab;       // [2][n_lat][N] 0, or 0x8000 | rep slot of the site's edge cluster
+  uint16_t* st;        // [2][n_lat][N] state words: spin << 15 | edge << 14 | the rep slot of
+                       // the site's edge cluster (edge clusters' flips are pending)
+  uint32_t* jt;        // [n_lat][tiles][T*T/16] couplings tile by tile, 2 bits a site
+                       // (J_h < 0, J_v < 0) at local index lx * T + ly (0 where absent)
   uint64_t* slots;     // [n_lat][nslots] edge-slot entries of the last tile pass
   uint64_t* xbits;     // [n_lat][tiles][2T/64] occupied bonds leaving each tile (+x, +y)
   uint32_t* flipbits;  // [n_lat][nslots/32] flip bit of each slot's cluster
@@ -98,6 +100,8 @@ int fused_tile_size(int L, int pref = 0);
 int fused_tiles_per_side(int L, int T);
 bool fused_merge_in_hbm(int L, int T);
 int fused_kernels_per_sweep(const FusedScratch& f);
+// Before the first sweep of a run: the state words from the batch's spins.
+hipError_t launch_fused_open(const LatticeSet& s, const FusedScratch& f, hipStream_t st);
 hipError_t launch_fused_sweep(const LatticeSet& s, const FusedScratch& f, hipStream_t st);
 // Folds the last sweep of a run and applies its pending flips into buffer 0.
 hipError_t launch_fused_close(const LatticeSet& s, const FusedScratch& f, hipStream_t st);

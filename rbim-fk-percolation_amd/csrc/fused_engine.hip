@@ -52,7 +52,12 @@
 
 namespace {
 
-constexpr uint32_t LAB_EDGE = 0x8000u;
+// State word of a site between tile passes: its spin (1: +1), whether it
+// belongs to an edge cluster, whose flip is pending, and that cluster's rep
+// slot in the site's tile.
+constexpr uint32_t ST_SPIN = 0x8000u;
+constexpr uint32_t ST_EDGE = 0x4000u;
+constexpr uint32_t ST_SLOT = 0x3FFFu;
 constexpr uint32_t NONE16 = 0xFFFFu;
 
 // Slot entry: rep (0..11) | min local index (12..23) | flip (24) |
@@ -108,32 +113,26 @@ __device__ __forceinline__ uint32_t flip_of(const uint32_t* fb, uint32_t slot) {
   return (fb[slot >> 5] >> (slot & 31u)) & 1u;
 }
 
-// Buffers of the current state of lattice lat.
+// State planes of lattice lat: the current one and the one a tile pass writes.
 struct FusedView {
-  const int8_t* sin;
-  int8_t* sout;
-  const uint16_t* lin;
-  uint16_t* lout;
+  const uint16_t* in;
+  uint16_t* out;
 };
 
 __device__ __forceinline__ FusedView fused_view(const LatticeSet& s, const FusedScratch& f, int lat) {
-  const size_t base = (size_t)lat * s.N;
   const int cur = f.cur[lat];
   FusedView v;
-  v.sin = (cur ? f.spins_alt : s.spins) + base;
-  v.sout = (cur ? s.spins : f.spins_alt) + base;
-  v.lin = f.lab + ((size_t)cur * s.n_lat + lat) * s.N;
-  v.lout = f.lab + ((size_t)(1 - cur) * s.n_lat + lat) * s.N;
+  v.in = f.st + ((size_t)cur * s.n_lat + lat) * s.N;
+  v.out = f.st + ((size_t)(1 - cur) * s.n_lat + lat) * s.N;
   return v;
 }
 
-// Spin bit (1: +1) of site n of tile `tile` in the current state, pending
+// Spin bit (1: +1) of a site of tile `tile` from its state word w, pending
 // edge-cluster flip applied.
 template <int T>
-__device__ __forceinline__ uint32_t true_spin(int8_t v, uint32_t lb, uint32_t tile,
-                                              const uint32_t* fb) {
-  uint32_t b = v > 0 ? 1u : 0u;
-  if (lb & LAB_EDGE) b ^= flip_of(fb, tile * 4u * T + (lb & 0x7FFFu));
+__device__ __forceinline__ uint32_t true_spin(uint32_t w, uint32_t tile, const uint32_t* fb) {
+  uint32_t b = (w >> 15) & 1u;
+  if (w & ST_EDGE) b ^= flip_of(fb, tile * 4u * T + (w & ST_SLOT));
   return b;
 }
 
@@ -172,7 +171,7 @@ __device__ __forceinline__ void load_halo(const LatticeSet& s, const FusedScratc
   const uint32_t ntile = right ? ty * nt + (tx + 1u == nt ? 0u : tx + 1u)
                                : (ty + 1u == nt ? 0u : ty + 1u) * nt + tx;
   const uint32_t hn = hx * L + hy;
-  hsp[tid] = (uint8_t)true_spin<T>(v.sin[hn], v.lin[hn], ntile, fb);
+  hsp[tid] = (uint8_t)true_spin<T>(v.in[hn], ntile, fb);
 }
 
 // Roots of x and y in the settled tile forest, walked together, reading
@@ -241,7 +240,6 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
   // divide L); sites beyond it are absent: no loads, stores, bonds or counts
   const uint32_t wx = PART ? min((uint32_t)T, L - x0) : (uint32_t)T;
   const uint32_t wy = PART ? min((uint32_t)T, L - y0) : (uint32_t)T;
-  const size_t base = (size_t)lat * N;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint64_t below = (1ull << lane) - 1ull;
   // lane map: queue position ql = it * NTH + tid holds the site pair (lx, ly0),
@@ -270,7 +268,7 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
   const uint64_t seed = s.seed[lat];
   const uint32_t lid = s.lat_id[lat];
   const FusedView v = fused_view(s, f, lat);
-  const uint8_t* jb = s.jbits + base;
+  const uint32_t* jt = f.jt + ((size_t)lat * tiles + tile) * (T * T / 16);
   // E, M of the state this sweep starts from close sweep g-1 (unless a run's
   // closing pass has already folded it); the clusters of sweep g are measured
   // when g is past thermalisation
@@ -293,75 +291,64 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
     const uint32_t src = lane < (int)WPT ? tile : lane < (int)(2u * WPT) ? nt_r : nt_u;
     fw = f.flipbits[(size_t)lat * ((uint32_t)f.nslots / 32u) + src * WPT + w];
   }
-  uint32_t sp2[ITERS], lb2[ITERS], j2[ITERS];
+  // the pair's two state words, and its word of couplings (the pair's four
+  // bits sit at bit 2 (lx T + ly0) mod 32: a pair never straddles two words)
+  uint32_t st2[ITERS], j2[ITERS];
 #pragma unroll
   for (int it = 0; it < ITERS; ++it) {
     const uint32_t ql = (uint32_t)(it * NTH + tid);
     const uint32_t lx = row_of(ql), ly0 = col_of(ql);
     const uint32_t n0 = (x0 + lx) * L + y0 + ly0;  // even L: the pair loads are aligned
+    const uint32_t lt = lx * T + ly0;
     if (PART == 0) {
-      sp2[it] = *reinterpret_cast<const uint16_t*>(v.sin + n0);
-      lb2[it] = *reinterpret_cast<const uint32_t*>(v.lin + n0);
-      j2[it] = *reinterpret_cast<const uint16_t*>(jb + n0);
+      st2[it] = *reinterpret_cast<const uint32_t*>(v.in + n0);
+      j2[it] = jt[lt >> 4];
     } else {
       // absent sites read as spin +1, no label, J = +1
       const bool v0 = lx < wx && ly0 < wy, v1 = lx < wx && ly0 + 1u < wy;
-      sp2[it] = 0x0101u;
-      lb2[it] = 0u;
+      st2[it] = ST_SPIN | ST_SPIN << 16;
       j2[it] = 0u;
       if (skip(it)) continue;
+      if (v0) j2[it] = jt[lt >> 4];
       if (PART == 1) {
-        if (v0) {
-          sp2[it] = *reinterpret_cast<const uint16_t*>(v.sin + n0);
-          lb2[it] = *reinterpret_cast<const uint32_t*>(v.lin + n0);
-          j2[it] = *reinterpret_cast<const uint16_t*>(jb + n0);
-        }
+        if (v0) st2[it] = *reinterpret_cast<const uint32_t*>(v.in + n0);
       } else {
-        if (v0) {
-          sp2[it] = (uint32_t)(uint8_t)v.sin[n0] | 0x0100u;
-          lb2[it] = v.lin[n0];
-          j2[it] = jb[n0];
-        }
-        if (v1) {
-          sp2[it] = (sp2[it] & 0xFFu) | ((uint32_t)(uint8_t)v.sin[n0 + 1u] << 8);
-          lb2[it] |= (uint32_t)v.lin[n0 + 1u] << 16;
-          j2[it] |= (uint32_t)jb[n0 + 1u] << 8;
-        }
+        if (v0) st2[it] = (uint32_t)v.in[n0] | ST_SPIN << 16;
+        if (v1) st2[it] = (st2[it] & 0xFFFFu) | (uint32_t)v.in[n0 + 1u] << 16;
       }
     }
   }
   const bool halo = tid < 2 * T;  // tid < T: right column, else upper row
-  int8_t hs = 1;
-  uint32_t hl = 0u;
+  uint32_t hw = ST_SPIN;
   if (halo) {
     const bool right = tid < T;
     const uint32_t i = right ? (uint32_t)tid : (uint32_t)tid - T;
     const uint32_t hx = right ? (x0 + wx == L ? 0u : x0 + wx) : x0 + i;
     const uint32_t hy = right ? y0 + i : (y0 + wy == L ? 0u : y0 + wy);
-    if (PART == 0 || i < (right ? wy : wx)) {
-      hs = v.sin[hx * L + hy];
-      hl = v.lin[hx * L + hy];
-    }
+    if (PART == 0 || i < (right ? wy : wx)) hw = v.in[hx * L + hy];
   }
-  // spin bit of a site, its edge cluster's pending flip applied (wbase: the
-  // first flip word of the site's tile); the shuffle runs on every lane
-  auto spin_bit = [&](int8_t sp, uint32_t lb, uint32_t wbase) -> uint32_t {
-    const uint32_t slot = lb & 0x7FFFu;
+  // spin bit of a site from its state word w, its edge cluster's pending flip
+  // applied (wbase: the first flip word of the site's tile); the shuffle runs
+  // on every lane
+  auto spin_bit = [&](uint32_t w, uint32_t wbase) -> uint32_t {
+    const uint32_t slot = w & ST_SLOT;
     const uint32_t word = (uint32_t)__shfl((int)fw, (int)(wbase + (slot >> 5)));
-    const uint32_t flip = (lb & LAB_EDGE) ? (word >> (slot & 31u)) & 1u : 0u;
-    return (sp > 0 ? 1u : 0u) ^ flip;
+    const uint32_t flip = (w & ST_EDGE) ? (word >> (slot & 31u)) & 1u : 0u;
+    return ((w >> 15) & 1u) ^ flip;
   };
 #pragma unroll
   for (int it = 0; it < ITERS; ++it) {
     if (skip(it)) continue;
-    const uint32_t li0 = li_of((uint32_t)(it * NTH + tid));
-    const uint32_t t0 = spin_bit((int8_t)(sp2[it] & 0xFFu), lb2[it] & 0xFFFFu, 0u);
-    const uint32_t t1 = spin_bit((int8_t)(sp2[it] >> 8), lb2[it] >> 16, 0u);
-    bits[li0] = (uint8_t)(t0 | (j2[it] & 3u) << 4);
-    bits[li0 + 1u] = (uint8_t)(t1 | ((j2[it] >> 8) & 3u) << 4);
+    const uint32_t ql = (uint32_t)(it * NTH + tid);
+    const uint32_t li0 = li_of(ql);
+    const uint32_t j4 = j2[it] >> (((row_of(ql) * T + col_of(ql)) & 15u) * 2u);
+    const uint32_t t0 = spin_bit(st2[it] & 0xFFFFu, 0u);
+    const uint32_t t1 = spin_bit(st2[it] >> 16, 0u);
+    bits[li0] = (uint8_t)(t0 | (j4 & 3u) << 4);
+    bits[li0 + 1u] = (uint8_t)(t1 | ((j4 >> 2) & 3u) << 4);
   }
   {
-    const uint32_t hb = spin_bit(hs, hl, tid < T ? WPT : 2u * WPT);
+    const uint32_t hb = spin_bit(hw, tid < T ? WPT : 2u * WPT);
     if (halo) hsp[tid] = (uint8_t)hb;
   }
   __syncthreads();
@@ -562,7 +549,7 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
     const uint32_t li0 = li_of(ql);
     const uint32_t n0 = (x0 + lx) * L + y0 + ly0;
     const bool vs[2] = {!PART || (lx < wx && ly0 < wy), !PART || (lx < wx && ly0 + 1u < wy)};
-    uint32_t sp2 = 0u, lb2 = 0u;
+    uint32_t st2 = 0u;  // the pair's new state words
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const uint32_t li = li0 + (uint32_t)j;
@@ -574,8 +561,7 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
       const uint32_t sbit = bits[li] & 1u;
       const bool edge = rep != NONE16;
       const uint32_t ns = edge ? sbit : sbit ^ fl;
-      sp2 |= (ns ? 0x01u : 0xFFu) << (8 * j);
-      lb2 |= (edge ? (LAB_EDGE | rep) : 0u) << (16 * j);
+      st2 |= ((ns ? ST_SPIN : 0u) | (edge ? ST_EDGE | rep : 0u)) << (16 * j);
       if (meas_cl && !edge && r == li && vs[j]) {  // an interior cluster, at its root
         const uint32_t key = ((lcnt[r] >> 16) << 12) | (0xFFFu - li);
         ++incl;
@@ -584,17 +570,10 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
       }
     }
     if (PART == 0 || (PART == 1 && vs[0])) {
-      *reinterpret_cast<uint16_t*>(v.sout + n0) = (uint16_t)sp2;
-      *reinterpret_cast<uint32_t*>(v.lout + n0) = lb2;
+      *reinterpret_cast<uint32_t*>(v.out + n0) = st2;
     } else if (PART == 2) {
-      if (vs[0]) {
-        v.sout[n0] = (int8_t)(sp2 & 0xFFu);
-        v.lout[n0] = (uint16_t)(lb2 & 0xFFFFu);
-      }
-      if (vs[1]) {
-        v.sout[n0 + 1u] = (int8_t)(sp2 >> 8);
-        v.lout[n0 + 1u] = (uint16_t)(lb2 >> 16);
-      }
+      if (vs[0]) v.out[n0] = (uint16_t)(st2 & 0xFFFFu);
+      if (vs[1]) v.out[n0 + 1u] = (uint16_t)(st2 >> 16);
     }
   }
   // edge slots for KB, and the occupation of the bonds leaving the tile
@@ -1253,7 +1232,7 @@ __global__ __launch_bounds__(NTH) void kf_measure(LatticeSet s, FusedScratch f) 
   for (uint32_t li = threadIdx.x; li < T * T; li += NTH) {
     if (PART && (li / T >= wx || li % T >= wy)) continue;  // absent site
     const uint32_t n = (x0 + li / T) * L + y0 + li % T;
-    bits[li] = (uint8_t)(true_spin<T>(v.sin[n], v.lin[n], tile, fb) | (jb[n] & 3u) << 4);
+    bits[li] = (uint8_t)(true_spin<T>(v.in[n], tile, fb) | (jb[n] & 3u) << 4);
   }
   __syncthreads();
   int32_t e = 0, m = 0;
@@ -1281,7 +1260,7 @@ __global__ __launch_bounds__(NTH) void kf_measure(LatticeSet s, FusedScratch f) 
   }
 }
 
-// Pending flips applied into buffer 0; labels cleared.
+// The batch's spins from the current state, pending flips applied.
 template <int T, int NTH, bool BIG, int PART>
 __global__ __launch_bounds__(NTH) void kf_settle(LatticeSet s, FusedScratch f) {
   __shared__ uint32_t fb_lds[fb_words<BIG>()];
@@ -1292,7 +1271,6 @@ __global__ __launch_bounds__(NTH) void kf_settle(LatticeSet s, FusedScratch f) {
   const uint32_t x0 = tx * T, y0 = ty * T;
   const FusedView v = fused_view(s, f, lat);
   int8_t* s0 = s.spins + (size_t)lat * N;
-  uint16_t* l0 = f.lab + (size_t)lat * N;
   const uint32_t* fb = flipbits_of<BIG>(f, lat, fb_lds, NTH);
   const uint32_t wx = PART ? min((uint32_t)T, L - x0) : (uint32_t)T;
   const uint32_t wy = PART ? min((uint32_t)T, L - y0) : (uint32_t)T;
@@ -1300,10 +1278,35 @@ __global__ __launch_bounds__(NTH) void kf_settle(LatticeSet s, FusedScratch f) {
   for (uint32_t li = threadIdx.x; li < T * T; li += NTH) {
     if (PART && (li / T >= wx || li % T >= wy)) continue;  // absent site
     const uint32_t n = (x0 + li / T) * L + y0 + li % T;
-    const uint32_t b = true_spin<T>(v.sin[n], v.lin[n], tile, fb);
-    s0[n] = b ? (int8_t)1 : (int8_t)-1;
-    l0[n] = 0u;
+    s0[n] = true_spin<T>(v.in[n], tile, fb) ? (int8_t)1 : (int8_t)-1;
   }
+}
+
+// Before a run's first sweep: state plane 0 (the current one between runs)
+// from the batch's spins, with no site in an edge cluster, and the couplings
+// tile by tile (16 sites of one tile row per word).
+__global__ void kf_open(LatticeSet s, FusedScratch f) {
+  const int lat = run_lat(s, blockIdx.y);
+  const size_t base = (size_t)lat * s.N;
+  const int8_t* sp = s.spins + base;
+  const uint8_t* jb = s.jbits + base;
+  uint16_t* st = f.st + base;
+  const uint32_t N = (uint32_t)s.N, L = (uint32_t)s.L, T = (uint32_t)f.tile;
+  const uint32_t nt = (uint32_t)f.tiles_x, per_tile = T * T / 16u, words = nt * nt * per_tile;
+  uint32_t* jt = f.jt + (size_t)lat * words;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += stride)
+    st[n] = sp[n] > 0 ? (uint16_t)ST_SPIN : (uint16_t)0u;
+  for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < words; w += stride) {
+    const uint32_t tile = w / per_tile, lt = (w % per_tile) * 16u;
+    const uint32_t x = (tile % nt) * T + lt / T, y0 = (tile / nt) * T + lt % T;
+    uint32_t word = 0u;
+    if (x < L)
+      for (uint32_t k = 0; k < 16u && y0 + k < L; ++k)
+        word |= (uint32_t)(jb[x * L + y0 + k] & 3u) << (2u * k);
+    jt[w] = word;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) f.cur[lat] = 0;
 }
 
 __global__ void kf_close(LatticeSet s, FusedScratch f) {
@@ -1384,6 +1387,13 @@ bool fused_merge_in_hbm(int L, int T) {
 }
 
 int fused_kernels_per_sweep(const FusedScratch& f) { return f.big ? 5 : 2; }
+
+hipError_t launch_fused_open(const LatticeSet& s, const FusedScratch& f, hipStream_t st) {
+  if (s.n_run <= 0) return hipSuccess;
+  const int blocks = s.N < 64 * 256 ? (s.N + 255) / 256 : 64;
+  hipLaunchKernelGGL(kf_open, dim3(blocks, s.n_run), dim3(256), 0, st, s, f);
+  return hipGetLastError();
+}
 
 hipError_t launch_fused_sweep(const LatticeSet& s, const FusedScratch& f, hipStream_t st) {
   const int part = fused_part(s, f);

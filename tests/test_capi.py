@@ -48,7 +48,8 @@ def test_descriptor_validation_on_host():
     assert tiled > lds > 10 * 32 * 32 * 2
     big = LatticeSpec(L=1024, J_h=np.ones((1, 1), np.int8), J_v=np.ones((1, 1), np.int8),
                       beta=0.5, seed=1, lattice_id=0)
-    # edge-slot layout (spins x2, couplings, labels x2, slot forest in HBM): ~8 B per site;
+    # edge-slot layout (spins, couplings, two planes of 16-bit state words, couplings
+    # packed tile by tile, slot forest in HBM): ~8 B per site;
     # multi-kernel tiled layout: ~27 B per site
     assert 1024 * 1024 * 7 < footprint([big], n_bins=1) < 1024 * 1024 * 10
     assert footprint([big], n_bins=1, engine=_hip.ENGINE_TILED) > 1024 * 1024 * 25
