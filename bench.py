@@ -63,7 +63,7 @@ ALGO_BYTES_PER_SITE_UPDATE = 4
 KAUFMAN_L1024 = {2.0: -1.7455646, 2.269: -1.4154795, 2.5: -1.1060792}
 # The reference's published numba sweep times on one core (README.md:110-111)
 README_SWEEP_MS = {32: 0.133, 64: 0.424}
-SECTIONS = ("main", "L256", "cfg2_full", "cfg1", "cfg3", "cfg4", "cfg5", "cpu_anchor")
+SECTIONS = ("main", "L256", "cut", "cfg2_full", "cfg1", "cfg3", "cfg4", "cfg5", "cpu_anchor")
 
 
 # ------------------------------------------------------------------ inputs
@@ -288,6 +288,26 @@ def measure_point(L, args, steps, warmup, plain_steps, with_cpu):
         point["cpu_baseline"] = cpu
         point["batched_vs_cpu"] = value / cpu["value"]
     return point
+
+
+def measure_cut(args):
+    """Batched scans at sides that neither 64 nor 32 divides: the edge-slot sweep on
+    tiles whose last row and column are cut (1280 lattices, 100 + 1000 sweeps)."""
+    out = {"workload": "batched uniform scans at L = 129, 200, 333: p=0.05, 40 T in "
+                       "linspace(1,3) x 32 realizations = 1280 lattices each, 100 + 1000 sweeps",
+           "unit": "site-updates/s", "points": {}}
+    for L in (129, 200, 333):
+        specs = scan_specs(L, args.p, np.linspace(1.0, 3.0, 40), 32, seed=0)
+        gpu_batched(specs, 20, 0)
+        res = gpu_batched(specs, 1100, 0)
+        point = {"value": res["site_updates"] / (res["ms"] / 1e3), "ms_per_step": res["ms"] / 1100,
+                 "groups": res["groups"]}
+        if L == 200:
+            point["roofline"] = roofline("edge-slot sweep on cut tiles: ka_tile + kb_merge",
+                                         res["sites"], res["ms"] / 1100, 1100,
+                                         "edge_slot_sweep_L200_cut_tiles.md", "traffic_bytes_per_sweep")
+        out["points"][str(L)] = point
+    return out
 
 
 def measure_cfg2_full(args):
@@ -609,6 +629,13 @@ def main():
                                                   args.second_warmup, args.second_plain_steps,
                                                   not args.no_cpu)
         note("L256")
+    if "cut" in only:
+        line["cut"] = measure_cut(args)
+        ref = line.get(f"L{args.second_L}", {}).get("value")
+        if ref:
+            for pt in line["cut"]["points"].values():
+                pt[f"vs_L{args.second_L}"] = pt["value"] / ref
+        note("cut")
     for key, fn in (("cfg2_full", measure_cfg2_full), ("cfg1", measure_cfg1),
                     ("cfg3", measure_cfg3), ("cfg4", measure_cfg4), ("cfg5", measure_cfg5)):
         if key in only:

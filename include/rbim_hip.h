@@ -118,9 +118,10 @@ typedef struct {
                                  (parts run concurrently, on at most 4 streams per
                                  batch -- the device's hardware queues -- the largest
                                  parts first on streams of their own); 0 -> default:
-                                 2 for tiled sets (L > 128), and for LDS sets that
-                                 fill the GPU more than once with a partial last
-                                 round; else 1                                    */
+                                 2 for tiled sets (L > 128) other than whole 64 x 64
+                                 tiles of the edge-slot sweep, and for LDS sets
+                                 that fill the GPU more than once with a partial
+                                 last round; else 1                               */
   int32_t series;             /* > 0: keep the per-sweep stats of the last `series`
                                  measured sweeps of every lattice in a device ring
                                  (time series for autocorrelation / equilibration

@@ -232,7 +232,7 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
   int64_t (*red)[PARTS] = sm.red;
   const int lat = run_lat(s, blockIdx.y);
   if (!lattice_active(s, lat)) return;
-  const uint32_t L = (uint32_t)s.L, N = (uint32_t)s.N;
+  const uint32_t L = (uint32_t)s.L;
   const uint32_t nt = (uint32_t)f.tiles_x, tiles = nt * nt;
   const uint32_t tile = blockIdx.x, tx = tile % nt, ty = tile / nt;
   const uint32_t x0 = tx * T, y0 = ty * T;
@@ -265,6 +265,9 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
   };
   const int64_t g = s.sweep_ctr[lat];
   const uint32_t thr = s.thr31[lat];
+  // a bond is drawn occupied when draw >> 1 < thr, i.e. draw <= 2 thr - 1
+  const bool any_bond = thr != 0u;
+  const uint32_t tlim = any_bond ? 2u * thr - 1u : 0u;
   const uint64_t seed = s.seed[lat];
   const uint32_t lid = s.lat_id[lat];
   const FusedView v = fused_view(s, f, lat);
@@ -396,10 +399,10 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
       dh1 = r2.x;
       dv1 = r2.y;
     }
-    const uint32_t oh0 = sh0 & ((dh0 >> 1) < thr ? 1u : 0u);
-    const uint32_t ov0 = sv0 & ((dv0 >> 1) < thr ? 1u : 0u);
-    const uint32_t oh1 = sh1 & ((dh1 >> 1) < thr ? 1u : 0u);
-    const uint32_t ov1 = sv1 & ((dv1 >> 1) < thr ? 1u : 0u);
+    const uint32_t oh0 = any_bond && dh0 <= tlim ? sh0 : 0u;
+    const uint32_t ov0 = any_bond && dv0 <= tlim ? sv0 : 0u;
+    const uint32_t oh1 = any_bond && dh1 <= tlim ? sh1 : 0u;
+    const uint32_t ov1 = any_bond && dv1 <= tlim ? sv1 : 0u;
     // bit 0 (the spin other lanes read as a neighbour) is rewritten unchanged
     bits[li0] = (uint8_t)((b0 & 1u) | (oh0 << 1) | (ov0 << 2) | ((dh0 & 1u) << 3));
     bits[li1] = (uint8_t)((b1 & 1u) | (oh1 << 1) | (ov1 << 2) | ((dh1 & 1u) << 3));
