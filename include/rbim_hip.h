@@ -129,9 +129,7 @@ typedef struct {
   int32_t tile;               /* tile side of the tiled engines: 32, or 64 for L >= 64
                                  (the edge-slot sweep cuts the last tile row and
                                  column when the side does not divide L); 0 -> 64
-                                 when it divides L, else 32 while the lattice's
-                                 slots of 32 x 32 tiles merge in LDS (L <= 224),
-                                 64 beyond                                        */
+                                 for L >= 64, 32 below                            */
   int32_t single_launch;      /* 1: the LDS-resident lattices (every side one of
                                  16, 32, 64, 128) run in one launch of 1024-thread
                                  workgroups, each a page of lattices of one side

@@ -328,7 +328,7 @@ int group_chunk(rbim_ctx* ctx, Group& g, int64_t n, hipStream_t stream, int64_t*
   auto sweep = [&g](hipStream_t st) {
     return g.fused ? launch_fused_sweep(g.set, g.fz, st) : launch_tiled_sweep(g.set, g.tile, st);
   };
-  const int per_sweep = g.fused ? fused_kernels_per_sweep(g.fz) : tiled_kernels_per_sweep();
+  const int per_sweep = g.fused ? fused_kernels_per_sweep(g.set, g.fz) : tiled_kernels_per_sweep();
   const int S = std::max(1, g.per_launch);
   if (n >= S && S > 1) {
     if (!g.graph) {

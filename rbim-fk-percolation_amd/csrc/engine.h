@@ -92,14 +92,13 @@ struct FusedScratch {
   uint32_t* wrapw;     // [n_lat] wrap bits of the sweep
   int32_t tile, tiles_x, nslots, big;
 };
-// Tile side for L >= 32 (pref: 32, or 64 for L >= 64; else 64 when it
-// divides L, otherwise 32 while tiles of 32 merge their slots in LDS, 64
-// beyond), the last tile row and column cut when the side does not divide L;
-// 0 for L < 32.
+// Tile side for L >= 32 (pref: 32, or 64 for L >= 64; else 64 for L >= 64,
+// 32 below), the last tile row and column cut when the side does not divide
+// L; 0 for L < 32.
 int fused_tile_size(int L, int pref = 0);
 int fused_tiles_per_side(int L, int T);
 bool fused_merge_in_hbm(int L, int T);
-int fused_kernels_per_sweep(const FusedScratch& f);
+int fused_kernels_per_sweep(const LatticeSet& s, const FusedScratch& f);
 // Before the first sweep of a run: the state words from the batch's spins.
 hipError_t launch_fused_open(const LatticeSet& s, const FusedScratch& f, hipStream_t st);
 hipError_t launch_fused_sweep(const LatticeSet& s, const FusedScratch& f, hipStream_t st);

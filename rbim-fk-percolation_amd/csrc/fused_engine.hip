@@ -202,7 +202,7 @@ __device__ __forceinline__ void troot16_4(const uint32_t* par, uint32_t (&x)[4])
 }
 
 // ------------------------------------------------------------------- KA
-// LDS of one tile pass.
+// LDS of one tile pass over a whole tile.
 template <int T, int NTH>
 struct KaShared {
   uint32_t par[T * T];
@@ -212,6 +212,33 @@ struct KaShared {
   int64_t red[NTH / 64][PARTS];
 };
 
+// The arrays of a tile pass: the whole-tile kernel's static ones, or a cut-tile
+// kernel's dynamic ones sized for the cut (cap sites).  lcnt holds the waves'
+// union queues (qw words each, a multiple of 4 and at least 192) while the
+// tile is labelled, then the cluster counters.
+template <int NTH>
+struct KaLds {
+  uint32_t* par;
+  uint32_t* lcnt;  // 16-byte aligned
+  uint8_t* bits;
+  uint8_t* hsp;
+  int64_t (*red)[PARTS];
+  uint32_t qw;
+};
+
+// Sites of LDS a cut-tile pass needs: the cut rows of the last tile column
+// (cut x T) and the packed rows of the last tile row (T x 2^(pair shift + 1)),
+// and at least 192 queue words per wave; a multiple of 4 per wave.
+template <int T>
+__host__ __device__ constexpr uint32_t cut_lds_sites(uint32_t cut, uint32_t waves) {
+  uint32_t pairs = (cut + 1u) / 2u, sh = 0u;
+  while ((1u << sh) < pairs) ++sh;
+  if (T < 64 && sh < 1u) sh = 1u;  // at least 64 / T pairs a row (T = 32)
+  uint32_t cap = (uint32_t)T << (sh + 1u);
+  if (cap < 192u * waves) cap = 192u * waves;
+  return (cap + 4u * waves - 1u) / (4u * waves) * (4u * waves);
+}
+
 // PART: 0 -- a whole tile (wx = wy = T) of a lattice of even L; 1 -- L even,
 // the last tile row and column cut (wx, wy < T; site pairs stay whole and
 // aligned); 2 -- L odd, cut as well, so the last pair of a cut row holds one
@@ -219,22 +246,20 @@ struct KaShared {
 // (two draws).
 template <int T, int NTH, bool BIG, int PART>
 __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch& f,
-                                        KaShared<T, NTH>& sm) {
+                                        const KaLds<NTH>& m, uint32_t tile) {
   constexpr int ITERS = T * T / 2 / NTH;
   constexpr uint32_t NS = 4u * T;  // edge slots per tile
-  constexpr uint32_t QW = T * T / (NTH / 64);  // union queue entries per wave (>= 192)
-  static_assert(QW >= 192, "a wave queues up to 3 unions per lane");
-  static_assert(QW % 4u == 0u, "queues are cleared with 16-byte stores");
-  uint32_t* par = sm.par;
-  uint32_t* lcnt = sm.lcnt;
-  uint8_t* bits = sm.bits;
-  uint8_t* hsp = sm.hsp;
-  int64_t (*red)[PARTS] = sm.red;
+  const uint32_t QW = m.qw;        // union queue entries per wave (a wave queues up to 192)
+  uint32_t* par = m.par;
+  uint32_t* lcnt = m.lcnt;
+  uint8_t* bits = m.bits;
+  uint8_t* hsp = m.hsp;
+  int64_t (*red)[PARTS] = m.red;
   const int lat = run_lat(s, blockIdx.y);
   if (!lattice_active(s, lat)) return;
   const uint32_t L = (uint32_t)s.L;
   const uint32_t nt = (uint32_t)f.tiles_x, tiles = nt * nt;
-  const uint32_t tile = blockIdx.x, tx = tile % nt, ty = tile / nt;
+  const uint32_t tx = tile % nt, ty = tile / nt;
   const uint32_t x0 = tx * T, y0 = ty * T;
   // the tile's valid part (the last tile row / column is cut when T does not
   // divide L); sites beyond it are absent: no loads, stores, bonds or counts
@@ -671,24 +696,48 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
   }
 }
 
+// The whole tiles of a lattice: all of them when T divides L (PART 0), else
+// all but the last tile row and column (PART 0 at even L, 2 at odd L, whose
+// site pairs can start at odd sites), the grid then covering (nt - 1)^2 tiles.
 // Eight wavefronts per SIMD: four 512-thread workgroups of 64 x 64 tiles (their
 // LDS allows four per CU) or eight of 32 x 32 tiles per CU, which caps the
-// scalar registers below what the cut-tile kernels would otherwise take (100,
+// scalar registers below what the code would otherwise take at odd L (100,
 // enough for seven).
 template <int T, int NTH, bool BIG, int PART>
 __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(8, 8))) void ka_tile(
     LatticeSet s, FusedScratch f) {
   __shared__ KaShared<T, NTH> sm;
-  if constexpr (PART == 1) {
-    // the whole tiles of a cut lattice (all but the last tile row and column)
-    // take the whole-tile code: the choice is the same for the whole workgroup
-    const uint32_t nt = (uint32_t)f.tiles_x;
-    if (blockIdx.x % nt + 1u < nt && blockIdx.x / nt + 1u < nt) {
-      ka_body<T, NTH, BIG, 0>(s, f, sm);
-      return;
-    }
-  }
-  ka_body<T, NTH, BIG, PART>(s, f, sm);
+  const KaLds<NTH> m{sm.par, sm.lcnt, sm.bits, sm.hsp, sm.red, (uint32_t)(T * T / (NTH / 64))};
+  const uint32_t nt = (uint32_t)f.tiles_x, b = blockIdx.x;
+  const uint32_t tile = gridDim.x == nt * nt ? b : (b / (nt - 1u)) * nt + b % (nt - 1u);
+  ka_body<T, NTH, BIG, PART>(s, f, m, tile);
+}
+
+// The cut tiles of a lattice T does not divide (2 nt - 1 per lattice: the last
+// tile column, then the rest of the last tile row), in narrower workgroups
+// whose LDS is sized for the cut (cap sites): most of a cut tile's wavefronts
+// would have nothing to do, and a whole tile's LDS would hold the CU's
+// workgroup slots as long as a whole tile does.
+template <int T, int NTH, bool BIG, int PART>
+__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(6, 8))) void ka_tile_cut(
+    LatticeSet s, FusedScratch f, uint32_t cap) {
+  extern __shared__ uint64_t ka_dyn[];
+  KaLds<NTH> m;
+  m.par = reinterpret_cast<uint32_t*>(ka_dyn);
+  m.lcnt = m.par + cap;
+  m.bits = reinterpret_cast<uint8_t*>(m.lcnt + cap);
+  m.hsp = m.bits + cap;
+  m.red = reinterpret_cast<int64_t(*)[PARTS]>(m.hsp + ((2u * T + 7u) & ~7u));
+  m.qw = cap / (NTH / 64);
+  const uint32_t nt = (uint32_t)f.tiles_x, c = blockIdx.x;
+  const uint32_t tile = c < nt ? c * nt + (nt - 1u) : (nt - 1u) * nt + (c - nt);
+  ka_body<T, NTH, BIG, PART>(s, f, m, tile);
+}
+
+// Dynamic LDS bytes of a cut-tile pass of cap sites.
+template <int T, int NTH>
+constexpr size_t cut_lds_bytes(uint32_t cap) {
+  return (size_t)cap * 9u + ((2u * T + 7u) & ~7u) + (size_t)(NTH / 64) * PARTS * 8u;
 }
 
 // ------------------------------------------------------------------- KB
@@ -1326,13 +1375,33 @@ __global__ void kf_close(LatticeSet s, FusedScratch f) {
 
 size_t kb_lds_bytes(int nslots) { return (size_t)nslots * (8 + 8 + 4); }
 
+// The tile passes of one sweep: every tile in one launch when T divides L,
+// else the whole tiles and the cut ones (in workgroups of half the threads) in
+// two.
+template <int T, bool BIG, int PART>
+void launch_ka(const LatticeSet& s, const FusedScratch& f, hipStream_t st) {
+  constexpr int NTH = T == 64 ? 512 : 256;
+  const int nt = f.tiles_x;
+  if constexpr (PART == 0) {
+    hipLaunchKernelGGL((ka_tile<T, NTH, BIG, 0>), dim3(nt * nt, s.n_run), dim3(NTH), 0, st, s, f);
+  } else {
+    constexpr int NTHC = NTH / 2;
+    constexpr int PW = PART == 1 ? 0 : 2;  // whole tiles: aligned pairs at even L
+    if (nt > 1)
+      hipLaunchKernelGGL((ka_tile<T, NTH, BIG, PW>), dim3((nt - 1) * (nt - 1), s.n_run), dim3(NTH), 0,
+                         st, s, f);
+    const uint32_t cap = cut_lds_sites<T>((uint32_t)(s.L - (nt - 1) * T), NTHC / 64);
+    const size_t bytes = cut_lds_bytes<T, NTHC>(cap);
+    hipLaunchKernelGGL((ka_tile_cut<T, NTHC, BIG, PART>), dim3(2 * nt - 1, s.n_run), dim3(NTHC),
+                       bytes, st, s, f, cap);
+  }
+}
+
 template <int T, int PART>
 hipError_t launch_fused_t(const LatticeSet& s, const FusedScratch& f, hipStream_t st) {
-  constexpr int NTH = T == 64 ? 512 : 256;
-  const int tiles = f.tiles_x * f.tiles_x;
   if (f.big) {
     const int nb = (f.nslots + 255) / 256;
-    hipLaunchKernelGGL((ka_tile<T, NTH, true, PART>), dim3(tiles, s.n_run), dim3(NTH), 0, st, s, f);
+    launch_ka<T, true, PART>(s, f, st);
     hipLaunchKernelGGL((kg_union<T>), dim3((f.nslots / 2 + 255) / 256, s.n_run), dim3(256), 0, st, s, f);
     hipLaunchKernelGGL(kg_fold, dim3(nb, s.n_run), dim3(256), 0, st, s, f);
     hipLaunchKernelGGL(kg_flip, dim3(nb, s.n_run), dim3(256), 0, st, s, f);
@@ -1343,7 +1412,7 @@ hipError_t launch_fused_t(const LatticeSet& s, const FusedScratch& f, hipStream_
   hipError_t e = ensure_dynamic_lds((const void*)kb_merge<T>,
                                     (int)kb_lds_bytes(RBIM_FUSED_MAX_SLOTS), configured);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((ka_tile<T, NTH, false, PART>), dim3(tiles, s.n_run), dim3(NTH), 0, st, s, f);
+  launch_ka<T, false, PART>(s, f, st);
   hipLaunchKernelGGL((kb_merge<T>), dim3(s.n_run), dim3(KB_THREADS), kb_lds_bytes(f.nslots), st, s, f);
   return hipGetLastError();
 }
@@ -1370,16 +1439,14 @@ int fused_part(const LatticeSet& s, const FusedScratch& f) {
 
 }  // namespace
 
-// Measured on 1280-lattice scans: where 64 does not divide L, tiles of 32 are
-// as fast or faster while their slots still merge in LDS (L <= 224: 1.04e11
-// site-updates/s at L = 224 against 1.00e11 on cut tiles of 64, 8.6e10 against
-// 8.0e10 at L = 200), cut tiles of 64 beyond (L = 288: 1.21e11 against 8.0e10
-// for 32 x 32 tiles merged in HBM, L = 333: 7.6e10 against 6.8e10).
+// Tiles of 64 for every L >= 64, cut where 64 does not divide L; measured on
+// 1280-lattice scans (site-updates/s, tiles of 64 against 32): L = 96 9.38e10
+// against 9.37e10, L = 129 7.61e10 / 7.46e10, L = 160 1.048e11 / 1.038e11,
+// L = 200 1.136e11 / 9.61e10, L = 224 1.209e11 / 1.056e11.
 int fused_tile_size(int L, int pref) {
   if (L < 32) return 0;
   if (pref == 32 || (pref == 64 && L >= 64)) return pref;  // cut tiles when T does not divide L
-  if (L % 64 == 0) return 64;
-  return fused_merge_in_hbm(L, 32) ? 64 : 32;
+  return L >= 64 ? 64 : 32;
 }
 
 int fused_tiles_per_side(int L, int T) { return T > 0 ? (L + T - 1) / T : 0; }
@@ -1389,7 +1456,9 @@ bool fused_merge_in_hbm(int L, int T) {
   return T > 0 && nt * nt * 4 * T > RBIM_FUSED_MAX_SLOTS;
 }
 
-int fused_kernels_per_sweep(const FusedScratch& f) { return f.big ? 5 : 2; }
+int fused_kernels_per_sweep(const LatticeSet& s, const FusedScratch& f) {
+  return (f.big ? 5 : 2) + (fused_part(s, f) != 0 ? 1 : 0);
+}
 
 hipError_t launch_fused_open(const LatticeSet& s, const FusedScratch& f, hipStream_t st) {
   if (s.n_run <= 0) return hipSuccess;
