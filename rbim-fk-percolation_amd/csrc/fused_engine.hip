@@ -243,12 +243,15 @@ __host__ __device__ constexpr uint32_t cut_lds_sites(uint32_t cut, uint32_t wave
 // the last tile row and column cut (wx, wy < T; site pairs stay whole and
 // aligned); 2 -- L odd, cut as well, so the last pair of a cut row holds one
 // site and a pair of a row starting at an odd site straddles two Philox pairs
-// (two draws).
+// (two draws); 3 -- a whole tile of a lattice of odd L: its odd rows start at
+// odd sites, and the lane map gives every wavefront rows of one parity, so
+// that only the odd-row wavefronts load site by site and draw twice.
 template <int T, int NTH, bool BIG, int PART>
 __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch& f,
                                         const KaLds<NTH>& m, uint32_t tile) {
   constexpr int ITERS = T * T / 2 / NTH;
   constexpr uint32_t NS = 4u * T;  // edge slots per tile
+  constexpr bool CUT = PART == 1 || PART == 2;  // a tile that may be cut: sites may be absent
   const uint32_t QW = m.qw;        // union queue entries per wave (a wave queues up to 192)
   uint32_t* par = m.par;
   uint32_t* lcnt = m.lcnt;
@@ -263,8 +266,8 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
   const uint32_t x0 = tx * T, y0 = ty * T;
   // the tile's valid part (the last tile row / column is cut when T does not
   // divide L); sites beyond it are absent: no loads, stores, bonds or counts
-  const uint32_t wx = PART ? min((uint32_t)T, L - x0) : (uint32_t)T;
-  const uint32_t wy = PART ? min((uint32_t)T, L - y0) : (uint32_t)T;
+  const uint32_t wx = CUT ? min((uint32_t)T, L - x0) : (uint32_t)T;
+  const uint32_t wy = CUT ? min((uint32_t)T, L - y0) : (uint32_t)T;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint64_t below = (1ull << lane) - 1ull;
   // lane map: queue position ql = it * NTH + tid holds the site pair (lx, ly0),
@@ -275,18 +278,31 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
   // lie past row wx - 1 skip every phase that works pair by pair (their sites
   // are absent, and no present site reads the local state of an absent row).
   // Slot entries and site numbers use the tile's own rows (stride T).
-  const uint32_t psh = PART ? pair_shift<T>(wy) : (uint32_t)__builtin_ctz(T / 2);
+  const uint32_t psh = CUT ? pair_shift<T>(wy) : (uint32_t)__builtin_ctz(T / 2);
   const uint32_t rsh = psh + 1u, RS = 1u << rsh;
   const uint32_t wv = (uint32_t)__builtin_amdgcn_readfirstlane(wave);
-  auto row_of = [&](uint32_t ql) -> uint32_t { return ql >> psh; };
+  // PART 3: rows permuted within blocks of twice a wavefront's rows (RPW = 128
+  // / T), a wavefront taking the block's even or its odd rows: the wavefront
+  // index's parity is its rows' parity (x0 is even, and NTH / 64 is even)
+  constexpr uint32_t RPW = 128u / T;
+  auto row_of = [&](uint32_t ql) -> uint32_t {
+    const uint32_t r = ql >> psh;
+    if constexpr (PART == 3)
+      return (r & ~(2u * RPW - 1u)) | ((r & (RPW - 1u)) << 1) | ((r / RPW) & 1u);
+    else
+      return r;
+  };
   auto col_of = [&](uint32_t ql) -> uint32_t { return (ql & ((1u << psh) - 1u)) << 1; };
-  auto li_of = [&](uint32_t ql) -> uint32_t { return PART ? (row_of(ql) << rsh) + col_of(ql) : ql << 1; };
+  auto li_of = [&](uint32_t ql) -> uint32_t {
+    return PART ? (row_of(ql) << rsh) + col_of(ql) : ql << 1;
+  };
+  const bool odd_rows = PART == 3 && (wv & 1u);  // wavefront-uniform
   // a local index as the tile-row index lx * T + ly
   auto tile_li = [&](uint32_t li) -> uint32_t {
-    return PART ? (li >> rsh) * T + (li & (RS - 1u)) : li;
+    return CUT ? (li >> rsh) * T + (li & (RS - 1u)) : li;
   };
   auto skip = [&](int it) -> bool {
-    return PART && (((uint32_t)it * NTH + wv * 64u) >> psh) >= wx;
+    return CUT && (((uint32_t)it * NTH + wv * 64u) >> psh) >= wx;
   };
   const int64_t g = s.sweep_ctr[lat];
   const uint32_t thr = s.thr31[lat];
@@ -328,8 +344,11 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
     const uint32_t lx = row_of(ql), ly0 = col_of(ql);
     const uint32_t n0 = (x0 + lx) * L + y0 + ly0;  // even L: the pair loads are aligned
     const uint32_t lt = lx * T + ly0;
-    if (PART == 0) {
+    if (PART == 0 || (PART == 3 && !odd_rows)) {
       st2[it] = *reinterpret_cast<const uint32_t*>(v.in + n0);
+      j2[it] = jt[lt >> 4];
+    } else if (PART == 3) {  // an odd row: the pair starts at an odd site
+      st2[it] = (uint32_t)v.in[n0] | (uint32_t)v.in[n0 + 1u] << 16;
       j2[it] = jt[lt >> 4];
     } else {
       // absent sites read as spin +1, no label, J = +1
@@ -353,7 +372,7 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
     const uint32_t i = right ? (uint32_t)tid : (uint32_t)tid - T;
     const uint32_t hx = right ? (x0 + wx == L ? 0u : x0 + wx) : x0 + i;
     const uint32_t hy = right ? y0 + i : (y0 + wy == L ? 0u : y0 + wy);
-    if (PART == 0 || i < (right ? wy : wx)) hw = v.in[hx * L + hy];
+    if (!CUT || i < (right ? wy : wx)) hw = v.in[hx * L + hy];
   }
   // spin bit of a site from its state word w, its edge cluster's pending flip
   // applied (wbase: the first flip word of the site's tile); the shuffle runs
@@ -393,7 +412,7 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
     const uint32_t b0 = bits[li0], b1 = bits[li1];
     // the x neighbour is in the tile below row wx-1, the y neighbour of n1 below
     // column wy-1 (of n0: n1 itself, unless n0 is a cut row's last site)
-    const bool v0 = !PART || (lx < wx && ly0 < wy), v1 = !PART || (lx < wx && ly0 + 1u < wy);
+    const bool v0 = !CUT || (lx < wx && ly0 < wy), v1 = !CUT || (lx < wx && ly0 + 1u < wy);
     const uint32_t sr0 = lx + 1u < wx ? bits[li0 + RS] & 1u : hsp[ly0];
     const uint32_t sr1 = lx + 1u < wx ? bits[li1 + RS] & 1u : hsp[ly0 + 1u];
     const uint32_t su0 = PART == 2 && ly0 + 1u >= wy ? hsp[T + lx] : b1 & 1u;
@@ -404,7 +423,7 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
     const uint32_t sh1 = v1 ? ~(b1 ^ sr1 ^ (b1 >> 4)) & 1u : 0u;
     const uint32_t sv1 = v1 ? ~(b1 ^ su1 ^ (b1 >> 5)) & 1u : 0u;
     if (meas_em) {
-      if (PART == 0) {
+      if (!CUT) {
         me += 4 - 2 * (int32_t)(sh0 + sv0 + sh1 + sv1);
         mm += 2 * (int32_t)((b0 & 1u) + (b1 & 1u)) - 2;
       } else {
@@ -417,7 +436,7 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
     // an odd site takes z, w of its pair and x, y of the next one
     const rbim_u32x4 r = rbim_sweep_draws(seed, lid, (uint32_t)g, n0 >> 1);
     uint32_t dh0 = r.x, dv0 = r.y, dh1 = r.z, dv1 = r.w;
-    if (PART == 2 && (n0 & 1u)) {
+    if ((PART == 2 && (n0 & 1u)) || odd_rows) {
       const rbim_u32x4 r2 = rbim_sweep_draws(seed, lid, (uint32_t)g, (n0 >> 1) + 1u);
       dh0 = r.z;
       dv0 = r.w;
@@ -521,7 +540,7 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
       const uint32_t la = li_of(qla), lb = li_of(qlb);
       const uint32_t ba0 = bits[la], ba1 = bits[la + 1u];
       bool va0 = true, va1 = true, vb0 = true, vb1 = true;
-      if (PART) {
+      if (CUT) {
         const uint32_t xa = row_of(qla), ya = col_of(qla), xb = row_of(qlb), yb = col_of(qlb);
         va0 = xa < wx && ya < wy, va1 = xa < wx && ya + 1u < wy;
         vb0 = xb < wx && yb < wy, vb1 = xb < wx && yb + 1u < wy;
@@ -546,8 +565,8 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
       const uint32_t ql = (uint32_t)(it * NTH + tid);
       const uint32_t li0 = li_of(ql), li1 = li0 + 1u;
       const uint32_t b0 = bits[li0], b1 = bits[li1];
-      const bool v0 = !PART || (row_of(ql) < wx && col_of(ql) < wy);
-      const bool v1 = !PART || (row_of(ql) < wx && col_of(ql) + 1u < wy);
+      const bool v0 = !CUT || (row_of(ql) < wx && col_of(ql) < wy);
+      const bool v1 = !CUT || (row_of(ql) < wx && col_of(ql) + 1u < wy);
       uint32_t r0 = li0, r1 = (b0 & 4u) && v1 ? li0 : li1;  // li1 shares li0's run when linked
       troot16_2(par, r0, r1);
       count_and_point(li0, b0, b1, r0, r1, v0, v1);
@@ -557,7 +576,7 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
   // rep slot of every edge cluster: par[root] = root | min slot << 16
   for (uint32_t k = tid; k < NS; k += NTH) {
     bool ok = true;
-    const uint32_t li = PART ? slot_li_cut<T>(k, wx, wy, rsh, ok) : slot_li<T>(k);
+    const uint32_t li = CUT ? slot_li_cut<T>(k, wx, wy, rsh, ok) : slot_li<T>(k);
     if (!ok) continue;
     const uint32_t r = par[li] & 0xFFFFu;
     atomicMin(&par[r], r | (k << 16));
@@ -576,7 +595,7 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
     const uint32_t lx = row_of(ql), ly0 = col_of(ql);
     const uint32_t li0 = li_of(ql);
     const uint32_t n0 = (x0 + lx) * L + y0 + ly0;
-    const bool vs[2] = {!PART || (lx < wx && ly0 < wy), !PART || (lx < wx && ly0 + 1u < wy)};
+    const bool vs[2] = {!CUT || (lx < wx && ly0 < wy), !CUT || (lx < wx && ly0 + 1u < wy)};
     uint32_t st2 = 0u;  // the pair's new state words
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
@@ -597,8 +616,11 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
         ik1 = max(ik1, key);
       }
     }
-    if (PART == 0 || (PART == 1 && vs[0])) {
+    if (PART == 0 || (PART == 1 && vs[0]) || (PART == 3 && !odd_rows)) {
       *reinterpret_cast<uint32_t*>(v.out + n0) = st2;
+    } else if (PART == 3) {
+      v.out[n0] = (uint16_t)(st2 & 0xFFFFu);
+      v.out[n0 + 1u] = (uint16_t)(st2 >> 16);
     } else if (PART == 2) {
       if (vs[0]) v.out[n0] = (uint16_t)(st2 & 0xFFFFu);
       if (vs[1]) v.out[n0 + 1u] = (uint16_t)(st2 >> 16);
@@ -611,7 +633,7 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
     const size_t sb = (size_t)lat * f.nslots + (size_t)tile * NS;
     for (uint32_t k = tid; k < NS; k += NTH) {
       bool ok = true;
-      const uint32_t li = PART ? slot_li_cut<T>(k, wx, wy, rsh, ok) : slot_li<T>(k);
+      const uint32_t li = CUT ? slot_li_cut<T>(k, wx, wy, rsh, ok) : slot_li<T>(k);
       if (!ok) {  // no site here: a slot of its own, never folded or counted
         f.guf[sb + k] = gpack(tile * NS + k, 0, 0);
         f.gmn[sb + k] = 0xFFFFFFFFu;
@@ -633,7 +655,7 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
     uint64_t* sl = f.slots + (size_t)lat * f.nslots + (size_t)tile * NS;
     for (uint32_t k = tid; k < NS; k += NTH) {
       bool ok = true;
-      const uint32_t li = PART ? slot_li_cut<T>(k, wx, wy, rsh, ok) : slot_li<T>(k);
+      const uint32_t li = CUT ? slot_li_cut<T>(k, wx, wy, rsh, ok) : slot_li<T>(k);
       if (!ok) {
         sl[k] = SLOT_ABSENT;
         continue;
@@ -649,7 +671,7 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
     const uint32_t i = tid < T ? (uint32_t)tid : (uint32_t)tid - T;
     uint32_t occ = tid < T ? (bits[((wx - 1u) << rsh) + i] >> 1) & 1u
                            : (bits[(i << rsh) + wy - 1u] >> 2) & 1u;
-    if (PART && i >= (tid < T ? wy : wx)) occ = 0u;
+    if (CUT && i >= (tid < T ? wy : wx)) occ = 0u;
     const uint64_t word = __ballot(occ != 0u);
     if (lane == 0) f.xbits[((size_t)lat * tiles + tile) * (2 * T / 64) + wave] = word;
   }
@@ -697,8 +719,8 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
 }
 
 // The whole tiles of a lattice: all of them when T divides L (PART 0), else
-// all but the last tile row and column (PART 0 at even L, 2 at odd L, whose
-// site pairs can start at odd sites), the grid then covering (nt - 1)^2 tiles.
+// all but the last tile row and column (PART 0 at even L, 3 at odd L, whose
+// odd rows start at odd sites), the grid then covering (nt - 1)^2 tiles.
 // Eight wavefronts per SIMD: four 512-thread workgroups of 64 x 64 tiles (their
 // LDS allows four per CU) or eight of 32 x 32 tiles per CU, which caps the
 // scalar registers below what the code would otherwise take at odd L (100,
@@ -1386,7 +1408,7 @@ void launch_ka(const LatticeSet& s, const FusedScratch& f, hipStream_t st) {
     hipLaunchKernelGGL((ka_tile<T, NTH, BIG, 0>), dim3(nt * nt, s.n_run), dim3(NTH), 0, st, s, f);
   } else {
     constexpr int NTHC = NTH / 2;
-    constexpr int PW = PART == 1 ? 0 : 2;  // whole tiles: aligned pairs at even L
+    constexpr int PW = PART == 1 ? 0 : 3;  // whole tiles of an even / odd L
     if (nt > 1)
       hipLaunchKernelGGL((ka_tile<T, NTH, BIG, PW>), dim3((nt - 1) * (nt - 1), s.n_run), dim3(NTH), 0,
                          st, s, f);
