@@ -235,10 +235,10 @@ def cpu_baseline(specs, site_updates_per_thread, per_thread):
 
 # ---------------------------------------------------------------- sections
 def uses_edge_slot(L):
-    """Whether the default tiled engine runs L on the edge-slot sweep (every
-    L >= 32, on tiles of 32 or 64 with the last row and column cut when the
-    side does not divide L); smaller L take the multi-kernel tiled sweep."""
-    return L >= 32
+    """Whether the default tiled engine runs L on the edge-slot sweep: every
+    L >= 3, on tiles of 32 or 64 with the last row and column cut when the
+    side does not divide L (below 32 the lattice is one cut tile of 32)."""
+    return L >= 3
 
 
 def measure_point(L, args, steps, warmup, plain_steps, with_cpu):
@@ -376,7 +376,8 @@ def measure_cfg1(args):
         "plain_and_batched_moments_identical": identical,
         "energy_T1": float(res_plain["observables"]["energy"]["value"][0]),
         # a launch here: one sweep of the plain path's kernels on one lattice
-        "roofline": roofline("tiled sweep (one lattice): k1_tile + k2_merge + k3_flip + k5_finalize",
+        "roofline": roofline("edge-slot sweep (one lattice, one cut tile of 32): ka_tile_cut + "
+                             "kb_merge",
                              L * L, kernel_ms / (len(specs) * total), len(specs) * total,
                              bound="launch"),
         "reference_readme_s": 5.2,

@@ -70,9 +70,9 @@ enum rbim_status {
 enum rbim_engine {
   RBIM_ENGINE_AUTO = 0,    /* LDS-resident kernel when L <= 128, else tiled  */
   RBIM_ENGINE_PLAIN = 1,   /* tiled path, sweep by sweep: the edge-slot sweep
-                              when L is a multiple of 32 (slots merged in LDS up
-                              to 6400 tile-edge slots, in HBM beyond), else the
-                              multi-kernel HBM-forest sweep                    */
+                              on tiles of 32 or 64, the last tile row and column
+                              cut where the side does not divide L (slots merged
+                              in LDS up to 6400 tile-edge slots, in HBM beyond) */
   RBIM_ENGINE_BATCHED = 2, /* LDS-resident persistent kernel (L <= 128)      */
   RBIM_ENGINE_TILED = 3    /* always the multi-kernel HBM-forest sweep        */
 };

@@ -92,9 +92,9 @@ struct FusedScratch {
   uint32_t* wrapw;     // [n_lat] wrap bits of the sweep
   int32_t tile, tiles_x, nslots, big;
 };
-// Tile side for L >= 32 (pref: 32, or 64 for L >= 64; else 64 for L >= 64,
+// Tile side for L >= 3 (pref: 32, or 64 for L >= 64; else 64 for L >= 64,
 // 32 below), the last tile row and column cut when the side does not divide
-// L; 0 for L < 32.
+// L (below 32: the lattice is one cut tile); 0 for L < 3.
 int fused_tile_size(int L, int pref = 0);
 int fused_tiles_per_side(int L, int T);
 bool fused_merge_in_hbm(int L, int T);

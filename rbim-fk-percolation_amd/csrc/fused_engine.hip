@@ -668,10 +668,13 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
   }
   if (tid < 2 * T) {
     // tid < T: +x bond of (x0+wx-1, y0+tid); else +y bond of (x0+tid-T, y0+wy-1)
+    // (a cut tile reads only the positions that hold sites: the rows and
+    // columns past the cut lie beyond its LDS)
     const uint32_t i = tid < T ? (uint32_t)tid : (uint32_t)tid - T;
-    uint32_t occ = tid < T ? (bits[((wx - 1u) << rsh) + i] >> 1) & 1u
-                           : (bits[(i << rsh) + wy - 1u] >> 2) & 1u;
-    if (CUT && i >= (tid < T ? wy : wx)) occ = 0u;
+    uint32_t occ = 0u;
+    if (!CUT || i < (tid < T ? wy : wx))
+      occ = tid < T ? (bits[((wx - 1u) << rsh) + i] >> 1) & 1u
+                    : (bits[(i << rsh) + wy - 1u] >> 2) & 1u;
     const uint64_t word = __ballot(occ != 0u);
     if (lane == 0) f.xbits[((size_t)lat * tiles + tile) * (2 * T / 64) + wave] = word;
   }
@@ -1464,9 +1467,10 @@ int fused_part(const LatticeSet& s, const FusedScratch& f) {
 // Tiles of 64 for every L >= 64, cut where 64 does not divide L; measured on
 // 1280-lattice scans (site-updates/s, tiles of 64 against 32): L = 96 9.38e10
 // against 9.37e10, L = 129 7.61e10 / 7.46e10, L = 160 1.048e11 / 1.038e11,
-// L = 200 1.136e11 / 9.61e10, L = 224 1.209e11 / 1.056e11.
+// L = 200 1.136e11 / 9.61e10, L = 224 1.209e11 / 1.056e11.  Below 32 the
+// lattice is one cut tile of 32, its own neighbour across both edges.
 int fused_tile_size(int L, int pref) {
-  if (L < 32) return 0;
+  if (L < 3) return 0;
   if (pref == 32 || (pref == 64 && L >= 64)) return pref;  // cut tiles when T does not divide L
   return L >= 64 ? 64 : 32;
 }

@@ -368,16 +368,18 @@ def test_run_past_end_is_refused(hip):
 
 @pytest.mark.parametrize("Ls", [(32, 64, 96), (128, 160, 256), (320,), (288, 384),
                                 (33, 47, 100, 129), (200,), (333,), (226, 131), (352,),
-                                (65, 127, 191), (193, 255), (257,), (130, 321)])
+                                (65, 127, 191), (193, 255), (257,), (130, 321),
+                                (3, 4, 5, 8), (16, 17, 31)])
 def test_edge_slot_sweep_equals_hbm_forest_sweep(hip, coracle, Ls):
-    """The edge-slot sweep (ENGINE_PLAIN for L >= 32) on its default tiles and
+    """The edge-slot sweep (ENGINE_PLAIN) on its default tiles and
     on tiles of 32 and of 64 forced, and the multi-kernel HBM-forest sweep
     (ENGINE_TILED), give identical trajectories, moment sums and per-sweep
     series, across split runs; and both equal the oracle.  The shapes cover
     whole tiles, tiles with a cut last row and column (every L that the side
     does not divide, in a launch of their own; odd L take two Philox calls for
     a site pair that straddles two, and cut rows are packed; cuts of 1 and 63
-    sites: 65, 193, 257 and 127, 191, 255 on tiles of 64), slots merged in LDS
+    sites: 65, 193, 257 and 127, 191, 255 on tiles of 64; below 32 the lattice is
+    one cut tile of 32, its own neighbour: 3, 4, 5, 8, 16, 17, 31), slots merged in LDS
     (up to 6400: 320 = 10 x 10 tiles of 32) and in HBM (288 = 9 x 9 and 333 =
     11 x 11 tiles of 32, 333 = 6 x 6 and 352, 384 = 6 x 6 tiles of 64)."""
     from rbim._hip import ENGINE_TILED
