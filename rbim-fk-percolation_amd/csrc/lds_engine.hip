@@ -42,27 +42,32 @@ __device__ __forceinline__ uint32_t uf_pack(uint32_t p, int wx, int wy) {
 __device__ __forceinline__ int uf_wx(uint32_t v) { return ((int)(v << 9)) >> 23; }
 __device__ __forceinline__ int uf_wy(uint32_t v) { return ((int)v) >> 23; }
 
+// Forest nodes are handles (lds_at in tile_common.h): site indices, or (BA)
+// byte offsets 4 n, which the 14-bit parent field holds while N <= 4096.
+
 // Roots of x and y in the settled forest (phase C), walked together.
+template <bool BA>
 __device__ __forceinline__ void lds_root2(const uint32_t* uf, uint32_t& x, uint32_t& y) {
-  uint32_t px = uf[x] & PMASK, py = uf[y] & PMASK;
+  uint32_t px = lds_get<BA>(uf, x) & PMASK, py = lds_get<BA>(uf, y) & PMASK;
   while (px != x || py != y) {
     x = px;
     y = py;
-    px = uf[x] & PMASK;
-    py = uf[y] & PMASK;
+    px = lds_get<BA>(uf, x) & PMASK;
+    py = lds_get<BA>(uf, y) & PMASK;
   }
 }
 
 // Roots of four sites at once (the two pairs of a lane when it owns exactly two).
+template <bool BA>
 __device__ __forceinline__ void lds_root4(const uint32_t* uf, uint32_t (&x)[4]) {
   uint32_t p[4];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) p[c] = uf[x[c]] & PMASK;
+  for (int c = 0; c < 4; ++c) p[c] = lds_get<BA>(uf, x[c]) & PMASK;
   while (p[0] != x[0] || p[1] != x[1] || p[2] != x[2] || p[3] != x[3]) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) x[c] = p[c];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) p[c] = uf[x[c]] & PMASK;
+    for (int c = 0; c < 4; ++c) p[c] = lds_get<BA>(uf, x[c]) & PMASK;
   }
 }
 
@@ -70,15 +75,16 @@ __device__ __forceinline__ void lds_root4(const uint32_t* uf, uint32_t (&x)[4]) 
 // flight at the same time), accumulating tree-path windings and halving both
 // paths.  a / b may be stale roots from a failed hook: the windings carried so
 // far stay valid because relative positions inside a tree never change.
+template <bool BA>
 __device__ __forceinline__ void find2(uint32_t* uf, uint32_t& a, int& ax, int& ay, uint32_t& b,
                                       int& bx, int& by) {
-  uint32_t va = uf[a], vb = uf[b];
+  uint32_t va = lds_get<BA>(uf, a), vb = lds_get<BA>(uf, b);
   while (true) {
     const uint32_t pa = va & PMASK, pb = vb & PMASK;
     const bool da = pa == a, db = pb == b;
     if (da && db) return;
-    const uint32_t wa = da ? va : uf[pa];
-    const uint32_t wb = db ? vb : uf[pb];
+    const uint32_t wa = da ? va : lds_get<BA>(uf, pa);
+    const uint32_t wb = db ? vb : lds_get<BA>(uf, pb);
     if (!da) {
       const uint32_t ga = wa & PMASK;
       if (ga == pa) {
@@ -88,11 +94,11 @@ __device__ __forceinline__ void find2(uint32_t* uf, uint32_t& a, int& ax, int& a
         va = wa;
       } else {
         const int hx = uf_wx(va) + uf_wx(wa), hy = uf_wy(va) + uf_wy(wa);
-        uf[a] = uf_pack(ga, hx, hy);
+        lds_at<BA>(uf, a) = uf_pack(ga, hx, hy);
         ax += hx;
         ay += hy;
         a = ga;
-        va = uf[ga];
+        va = lds_get<BA>(uf, ga);
       }
     }
     if (!db) {
@@ -104,11 +110,11 @@ __device__ __forceinline__ void find2(uint32_t* uf, uint32_t& a, int& ax, int& a
         vb = wb;
       } else {
         const int hx = uf_wx(vb) + uf_wx(wb), hy = uf_wy(vb) + uf_wy(wb);
-        uf[b] = uf_pack(gb, hx, hy);
+        lds_at<BA>(uf, b) = uf_pack(gb, hx, hy);
         bx += hx;
         by += hy;
         b = gb;
-        vb = uf[gb];
+        vb = lds_get<BA>(uf, gb);
       }
     }
   }
@@ -116,43 +122,45 @@ __device__ __forceinline__ void find2(uint32_t* uf, uint32_t& a, int& ax, int& a
 
 // Union of the occupied bond a -> b that crosses the periodic boundary
 // (cx, cy) times.  Returns wrap bits (1: x, 2: y) when the bond closes a loop.
+template <bool BA>
 __device__ __forceinline__ uint32_t lds_union(uint32_t* uf, uint32_t a, uint32_t b, int cx,
                                               int cy) {
   int ax = 0, ay = 0, bx = 0, by = 0;
   while (true) {
-    find2(uf, a, ax, ay, b, bx, by);
+    find2<BA>(uf, a, ax, ay, b, bx, by);
     if (a == b) return ((cx + bx - ax) != 0 ? 1u : 0u) | ((cy + by - ay) != 0 ? 2u : 0u);
     if (a < b) {
-      if (atomicCAS(&uf[b], b, uf_pack(a, ax - bx - cx, ay - by - cy)) == b) return 0u;
+      if (atomicCAS(&lds_at<BA>(uf, b), b, uf_pack(a, ax - bx - cx, ay - by - cy)) == b) return 0u;
     } else {
-      if (atomicCAS(&uf[a], a, uf_pack(b, bx - ax + cx, by - ay + cy)) == a) return 0u;
+      if (atomicCAS(&lds_at<BA>(uf, a), a, uf_pack(b, bx - ax + cx, by - ay + cy)) == a) return 0u;
     }
   }
 }
 
 // Winding-free forest (every word is a bare parent): roots of a and b,
 // walked together with path halving.
+template <bool BA>
 __device__ __forceinline__ void find2_plain(uint32_t* uf, uint32_t& a, uint32_t& b) {
-  uint32_t pa = uf[a], pb = uf[b];
+  uint32_t pa = lds_get<BA>(uf, a), pb = lds_get<BA>(uf, b);
   while (pa != a || pb != b) {
-    const uint32_t ga = pa != a ? uf[pa] : pa;
-    const uint32_t gb = pb != b ? uf[pb] : pb;
+    const uint32_t ga = pa != a ? lds_get<BA>(uf, pa) : pa;
+    const uint32_t gb = pb != b ? lds_get<BA>(uf, pb) : pb;
     if (pa != a) {
       if (ga == pa) {
         a = pa;
       } else {
-        uf[a] = ga;
+        lds_at<BA>(uf, a) = ga;
         a = ga;
-        pa = uf[ga];
+        pa = lds_get<BA>(uf, ga);
       }
     }
     if (pb != b) {
       if (gb == pb) {
         b = pb;
       } else {
-        uf[b] = gb;
+        lds_at<BA>(uf, b) = gb;
         b = gb;
-        pb = uf[gb];
+        pb = lds_get<BA>(uf, gb);
       }
     }
   }
@@ -161,22 +169,24 @@ __device__ __forceinline__ void find2_plain(uint32_t* uf, uint32_t& a, uint32_t&
 // Union of a bond that does not cross the periodic boundary, before any bond
 // that does: every tree path then lies inside the open L x L square, so all
 // windings are 0 and a loop closed here cannot wind.
+template <bool BA>
 __device__ __forceinline__ void lds_union_plain(uint32_t* uf, uint32_t a, uint32_t b) {
   while (true) {
-    find2_plain(uf, a, b);
+    find2_plain<BA>(uf, a, b);
     if (a == b) return;
     const uint32_t hi = max(a, b), lo = min(a, b);
-    if (atomicCAS(&uf[hi], hi, lo) == hi) return;
+    if (atomicCAS(&lds_at<BA>(uf, hi), hi, lo) == hi) return;
     a = lo;
     b = hi;
   }
 }
 
-// Cluster keys from the counter c = size << 16 | n_plus of a root.  The max key
-// is the largest cluster, ties to the smaller root; n_plus rides along.
+// Cluster keys from the counter c = size << 16 | n_plus of a root (its handle:
+// index, or byte offset when BA).  The max key is the largest cluster, ties to
+// the smaller root; n_plus rides along.
 //   wide   (any L):   size (bits 32..), ~root (16..29), n_plus (0..14)
-//   narrow (N <= 1024): size (21..31), 1023 - root (11..20), n_plus (0..10)
-template <bool NARROW>
+//   narrow (N <= 1024): size (21..31), 1023 - root index (11..20), n_plus (0..10)
+template <bool NARROW, bool BA>
 struct ClusterKey {
   using T = uint64_t;
   static __device__ __forceinline__ T make(uint32_t c, uint32_t root) {
@@ -185,11 +195,12 @@ struct ClusterKey {
   static __device__ __forceinline__ int64_t size(T k) { return (int64_t)(k >> 32); }
   static __device__ __forceinline__ int64_t n_plus(T k) { return (int64_t)(k & 0xFFFFu); }
 };
-template <>
-struct ClusterKey<true> {
+template <bool BA>
+struct ClusterKey<true, BA> {
   using T = uint32_t;
   static __device__ __forceinline__ T make(uint32_t c, uint32_t root) {
-    return ((c >> 16) << 21) | ((1023u - root) << 11) | (c & 0x7FFu);
+    // (4092 - 4 r) << 9 == (1023 - r) << 11
+    return ((c >> 16) << 21) | (BA ? (4092u - root) << 9 : (1023u - root) << 11) | (c & 0x7FFu);
   }
   static __device__ __forceinline__ int64_t size(T k) { return (int64_t)(k >> 21); }
   static __device__ __forceinline__ int64_t n_plus(T k) { return (int64_t)(k & 0x7FFu); }
@@ -241,6 +252,9 @@ __device__ __forceinline__ Pair make_pair(uint32_t q, uint32_t N, uint32_t Lr, u
 
 // LDS words of one lattice of N sites: forest, counters, state bytes.
 __host__ __device__ constexpr uint32_t lds_words(uint32_t N) { return 2u * N + ((N + 15u) & ~15u) / 4u; }
+// One lattice per workgroup at a compile-time side up to 64: static LDS, byte
+// offsets as forest handles (4 N - 4 fits the 14-bit parent field).
+__host__ __device__ constexpr bool lds_static(int P, int LC) { return P == 1 && LC > 0 && LC <= 64; }
 
 // The sweeps of P lattices in one workgroup, NT threads (whole wavefronts)
 // each: the workgroup's `page` holds lattices page * P .. page * P + P - 1 of
@@ -257,9 +271,14 @@ __device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, 
   // exactly two pairs per lane: phase C walks the four roots together
   constexpr bool MERGED = ALL && LC * LC / 2 == 2 * NT;
   static_assert(P == 1 || LC > 0, "lattices share a workgroup only at a compile-time L");
-  using Key = ClusterKey<(LC > 0 && LC <= 32)>;
+  // One lattice of a compile-time side up to 64 sits in static LDS and names
+  // its forest nodes by byte offset (handles, see lds_at): the walks then need
+  // no address arithmetic.
+  constexpr bool BA = lds_static(P, LC);
+  using Key = ClusterKey<(LC > 0 && LC <= 32), BA>;
   using KT = typename Key::T;
   extern __shared__ uint32_t smem_all[];
+  __shared__ uint32_t smem_fix[BA ? lds_words((uint32_t)(LC * LC)) : 1];
   __shared__ uint32_t red_c_all[P][NW][3];  // satisfied bonds, up spins, roots
   __shared__ KT red_k_all[P][NW][2];        // two largest cluster keys
   __shared__ uint32_t wrapflag_all[P][2];   // by sweep parity
@@ -270,7 +289,7 @@ __device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, 
   const uint32_t N = LC ? (uint32_t)(LC * LC) : (uint32_t)s.N;
   const uint32_t L = LC ? (uint32_t)LC : (uint32_t)s.L;
   const uint32_t magic = s.magic;
-  uint32_t* smem = smem_all + (size_t)sub * lds_words(N);
+  uint32_t* smem = BA ? smem_fix : smem_all + (size_t)sub * lds_words(N);
   uint32_t (&red_c)[NW][3] = red_c_all[sub];
   KT (&red_k)[NW][2] = red_k_all[sub];
   uint32_t (&wrapflag)[2] = wrapflag_all[sub];
@@ -282,6 +301,9 @@ __device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, 
   const int ri = page * P + sub;
   const bool valid = P == 1 || ri < s.n_run;
   const int lat = run_lat(s, valid ? ri : page * P);  // an idle lattice follows the page's first
+  // handle of site n, and site of handle h
+  auto H = [](uint32_t n) -> uint32_t { return BA ? n << 2 : n; };
+  auto IDX = [](uint32_t h) -> uint32_t { return BA ? h >> 2 : h; };
   const int64_t g0 = s.sweep_ctr[lat];
   const int64_t gend = min(g0 + (int64_t)sweeps, lat_total(s, lat));
   if (gend <= g0) return;  // finished (or empty) slot: its state stays as it is
@@ -328,9 +350,9 @@ __device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, 
   auto measure_site = [&](bool valid, uint32_t n, uint32_t b, uint32_t sat_h, uint32_t sat_v) {
     m_sat += popc_ballot(valid && sat_h) + popc_ballot(valid && sat_v);
     m_up += popc_ballot(valid && (b & 1u));
-    const bool root = valid && (uf[n] & PMASK) == n;
+    const bool root = valid && (lds_get<BA>(uf, H(n)) & PMASK) == H(n);
     m_root += popc_ballot(root);
-    const KT key = Key::make(cnt[n], n);
+    const KT key = Key::make(cnt[n], H(n));
     top2_merge(k1, k2, root ? key : (KT)0u, (KT)0u);
   };
   auto publish = [&]() {
@@ -447,12 +469,12 @@ __device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, 
       const uint32_t start0 = run_start(__ballot(internal && outlink), __ballot(outlink), below, chunk);
       if (p.v0) {
         sb[p.n0] = (uint8_t)((b0 & 7u) | (oh0 << 3) | (ov0 << 4) | (fl0 << 5));
-        uf[p.n0] = start0;
+        lds_at<BA>(uf, H(p.n0)) = H(start0);
         if (!compact) cnt[p.n0] = 0u;  // else B1's queue holds cnt until B2 clears it
       }
       if (p.v1) {
         sb[p.n1] = (uint8_t)((b1 & 7u) | (oh1 << 3) | (ov1 << 4) | (fl1 << 5));
-        uf[p.n1] = internal ? start0 : p.n1;
+        lds_at<BA>(uf, H(p.n1)) = internal ? H(start0) : H(p.n1);
         if (!compact) cnt[p.n1] = 0u;
       }
     }
@@ -494,15 +516,16 @@ __device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, 
       }
       if (compact) {
         // the wave's requests go through a dense queue (in cnt, cleared in B2)
-        const uint32_t qa[3] = {p.n0, p.n1, p.n1};
-        const uint32_t qb[3] = {p.nr0, p.nr1, p.nu1};
-        wave_union_queue(uf, cnt + wave * qw, req, qa, qb, below);
+        const uint32_t qa[3] = {H(p.n0), H(p.n1), H(p.n1)};
+        const uint32_t qb[3] = {H(p.nr0), H(p.nr1), H(p.nu1)};
+        wave_union_queue<BA>(uf, cnt + wave * qw, req, qa, qb, below);
       } else {
         while (__ballot(req != 0u)) {
           if (req) {
             const uint32_t k = (uint32_t)__builtin_ctz(req);
             req &= req - 1u;
-            lds_union_plain(uf, k == 0u ? p.n0 : p.n1, k == 0u ? p.nr0 : k == 1u ? p.nr1 : p.nu1);
+            lds_union_plain<BA>(uf, H(k == 0u ? p.n0 : p.n1),
+                                H(k == 0u ? p.nr0 : k == 1u ? p.nr1 : p.nu1));
           }
         }
       }
@@ -519,7 +542,7 @@ __device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, 
       const bool horiz = t >= L;
       const uint32_t a = horiz ? (L - 1u) * L + (t - L) : t * L + (L - 1u);
       if ((sb[a] >> (horiz ? 3 : 4)) & 1u)
-        wraps |= lds_union(uf, a, horiz ? t - L : t * L, horiz ? 1 : 0, horiz ? 0 : 1);
+        wraps |= lds_union<BA>(uf, H(a), H(horiz ? t - L : t * L), horiz ? 1 : 0, horiz ? 0 : 1);
     }
     wave_or_wraps(&wrapflag[g & 1], wraps);
     // sweep g-1's partials were published before the barrier after A, and its
@@ -539,10 +562,10 @@ __device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, 
       for (int it = 0; it < 2; ++it) {
         const uint32_t n0 = 2u * (uint32_t)(it * NT + tid);
         bb[it] = sb[n0];
-        rr[2 * it] = n0;
-        rr[2 * it + 1] = ((bb[it] >> 4) & 1u) ? n0 : n0 + 1u;
+        rr[2 * it] = H(n0);
+        rr[2 * it + 1] = H(((bb[it] >> 4) & 1u) ? n0 : n0 + 1u);
       }
-      lds_root4(uf, rr);
+      lds_root4<BA>(uf, rr);
     }
 #pragma unroll 1
     for (int it = 0; it < iters; ++it) {
@@ -559,18 +582,18 @@ __device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, 
           b0 = sb[p.n0];
           // n1 shares n0's run (and root) when the in-row bond n0 -> n1 is occupied
           const bool linked = ((b0 >> 4) & 1u) && p.y0 != L - 1u;
-          r0 = p.n0;
-          r1 = p.v1 && !linked ? p.n1 : p.n0;
-          lds_root2(uf, r0, r1);
+          r0 = H(p.n0);
+          r1 = H(p.v1 && !linked ? p.n1 : p.n0);
+          lds_root2<BA>(uf, r0, r1);
         }
-        ns0 = (b0 ^ (sb[r0] >> 5)) & 1u;
+        ns0 = (b0 ^ (sb[IDX(r0)] >> 5)) & 1u;
         sb[p.n0] = (uint8_t)((b0 & ~1u) | ns0);
-        if (s.canon_out) s.canon_out[p.n0] = (int32_t)r0;
+        if (s.canon_out) s.canon_out[p.n0] = (int32_t)IDX(r0);
         if (p.v1) {
           const uint32_t b1 = sb[p.n1];
-          ns1 = (b1 ^ (sb[r1] >> 5)) & 1u;
+          ns1 = (b1 ^ (sb[IDX(r1)] >> 5)) & 1u;
           sb[p.n1] = (uint8_t)((b1 & ~1u) | ns1);
-          if (s.canon_out) s.canon_out[p.n1] = (int32_t)r1;
+          if (s.canon_out) s.canon_out[p.n1] = (int32_t)IDX(r1);
         }
       }
       const uint32_t R = __builtin_amdgcn_readfirstlane(r0);
@@ -578,14 +601,14 @@ __device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, 
       const uint64_t m0 = __ballot(in0), m1 = __ballot(in1);
       const uint64_t u0 = __ballot(in0 && ns0), u1 = __ballot(in1 && ns1);
       if (__builtin_amdgcn_readfirstlane((uint32_t)lane) == (uint32_t)lane && (m0 | m1))
-        atomicAdd(&cnt[R], (uint32_t)(__popcll(m0) + __popcll(m1)) << 16 |
-                               (uint32_t)(__popcll(u0) + __popcll(u1)));
+        atomicAdd(&lds_at<BA>(cnt, R), (uint32_t)(__popcll(m0) + __popcll(m1)) << 16 |
+                                           (uint32_t)(__popcll(u0) + __popcll(u1)));
       const bool rest0 = p.v0 && !in0, rest1 = p.v1 && !in1;
       if (rest0 && rest1 && r0 == r1) {
-        atomicAdd(&cnt[r0], 0x20000u | (ns0 + ns1));
+        atomicAdd(&lds_at<BA>(cnt, r0), 0x20000u | (ns0 + ns1));
       } else {
-        if (rest0) atomicAdd(&cnt[r0], 0x10000u | ns0);
-        if (rest1) atomicAdd(&cnt[r1], 0x10000u | ns1);
+        if (rest0) atomicAdd(&lds_at<BA>(cnt, r0), 0x10000u | ns0);
+        if (rest1) atomicAdd(&lds_at<BA>(cnt, r1), 0x10000u | ns1);
       }
     }
     __syncthreads();
@@ -668,11 +691,13 @@ __global__ void init_spins_kernel(LatticeSet s, int start) {
 template <int NT, int LC>
 hipError_t launch_lds_t(const LatticeSet& s, int sweeps, hipStream_t st) {
   static std::atomic<uint64_t> configured{0};
-  hipError_t e = ensure_dynamic_lds((const void*)lds_sweep_kernel<NT, LC>,
-                                    (int)lds_bytes(RBIM_LDS_MAX_N), configured);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((lds_sweep_kernel<NT, LC>), dim3(s.n_run), dim3(NT), lds_bytes(s.N), st, s,
-                     sweeps);
+  if (!lds_static(1, LC)) {
+    hipError_t e = ensure_dynamic_lds((const void*)lds_sweep_kernel<NT, LC>,
+                                      (int)lds_bytes(RBIM_LDS_MAX_N), configured);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL((lds_sweep_kernel<NT, LC>), dim3(s.n_run), dim3(NT),
+                     lds_static(1, LC) ? 0 : lds_bytes(s.N), st, s, sweeps);
   return hipGetLastError();
 }
 
@@ -739,9 +764,9 @@ int lds_blocks_per_cu_t(int L) {
   int n = 0;
   hipError_t e;
   switch (L) {
-    case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lds_sweep_kernel<NT, 16>, NT, lds_bytes(N)); break;
-    case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lds_sweep_kernel<NT, 32>, NT, lds_bytes(N)); break;
-    case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lds_sweep_kernel<NT, 64>, NT, lds_bytes(N)); break;
+    case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lds_sweep_kernel<NT, 16>, NT, 0); break;
+    case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lds_sweep_kernel<NT, 32>, NT, 0); break;
+    case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lds_sweep_kernel<NT, 64>, NT, 0); break;
     case 128: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lds_sweep_kernel<NT, 128>, NT, lds_bytes(N)); break;
     default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lds_sweep_kernel<NT, 0>, NT, lds_bytes(N)); break;
   }

@@ -174,41 +174,53 @@ __device__ __forceinline__ uint32_t run_start(uint64_t thru, uint64_t ol, uint64
 // crosses the periodic boundary).  Two finds walked together with path
 // halving; the larger root hooks under the smaller, so every root is the
 // minimum index (= minimum site) of its tile-local cluster.
+//
+// A node is named by a handle: its index, or (BA) its byte offset 4 * index
+// from the forest's base, which then also holds parents as byte offsets.  With
+// the forest in static LDS a byte-offset walk needs no address arithmetic: the
+// parent read from one word is the address of the next, and the array's base
+// folds into the load's offset field.  Handles order like indices.
+template <bool BA>
+__device__ __forceinline__ uint32_t& lds_at(uint32_t* base, uint32_t h) {
+  if constexpr (BA) return *reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(base) + h);
+  else return base[h];
+}
+template <bool BA>
+__device__ __forceinline__ uint32_t lds_get(const uint32_t* base, uint32_t h) {
+  if constexpr (BA) return *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(base) + h);
+  else return base[h];
+}
+
 // One step of a halving walk: x has the parent px and the grandparent g.  A
 // non-root x is pointed at g and the walk moves there; at a root (px == x)
 // the walk stays.  Only non-root words are written (hooks CAS roots only).
+template <bool BA = false>
 __device__ __forceinline__ void halve_step(uint32_t* par, uint32_t& x, uint32_t px, uint32_t g) {
   const bool halve = px != x && g != px;
-  if (halve) par[x] = g;
+  if (halve) lds_at<BA>(par, x) = g;
   x = halve ? g : px;
 }
 
+template <bool BA = false>
 __device__ __forceinline__ void tfind2(uint32_t* par, uint32_t& a, uint32_t& b) {
-  uint32_t pa = par[a], pb = par[b];
+  uint32_t pa = lds_get<BA>(par, a), pb = lds_get<BA>(par, b);
   while (pa != a || pb != b) {
     // a root re-reads its own word; a hook made meanwhile shows as a new parent
-    const uint32_t ga = par[pa], gb = par[pb];
-    halve_step(par, a, pa, ga);
-    halve_step(par, b, pb, gb);
-    pa = par[a];
-    pb = par[b];
+    const uint32_t ga = lds_get<BA>(par, pa), gb = lds_get<BA>(par, pb);
+    halve_step<BA>(par, a, pa, ga);
+    halve_step<BA>(par, b, pb, gb);
+    pa = lds_get<BA>(par, a);
+    pb = lds_get<BA>(par, b);
   }
 }
 
-// Unions of one wavefront, compacted: every lane holds up to three requests
-// (bit k of req: the pair (a_k, b_k)); they are written densely into the
-// wave's queue q (>= 192 entries, packed a | b << 16) and processed 64 at a
-// time, so a round has no idle lanes waiting on a neighbour's second request.
-__device__ __forceinline__ void wave_union_queue(uint32_t* par, uint32_t* q, uint32_t req,
-                                                 const uint32_t (&a)[3], const uint32_t (&b)[3],
-                                                 uint64_t below);
-
+template <bool BA = false>
 __device__ __forceinline__ void tunion(uint32_t* par, uint32_t a, uint32_t b) {
   while (true) {
-    tfind2(par, a, b);
+    tfind2<BA>(par, a, b);
     if (a == b) return;
     const uint32_t hi = max(a, b), lo = min(a, b);
-    if (atomicCAS(&par[hi], hi, lo) == hi) return;
+    if (atomicCAS(&lds_at<BA>(par, hi), hi, lo) == hi) return;
     a = lo;
     b = hi;
   }
@@ -225,6 +237,12 @@ __device__ __forceinline__ void troot2(const uint32_t* par, uint32_t& x, uint32_
   }
 }
 
+// Unions of one wavefront, compacted: every lane holds up to three requests
+// (bit k of req: the pair (a_k, b_k) of handles); they are written densely
+// into the wave's queue q (>= 192 entries, packed a | b << 16) and processed
+// 64 at a time, so a round has no idle lanes waiting on a neighbour's second
+// request.
+template <bool BA = false>
 __device__ __forceinline__ void wave_union_queue(uint32_t* par, uint32_t* q, uint32_t req,
                                                  const uint32_t (&a)[3], const uint32_t (&b)[3],
                                                  uint64_t below) {
@@ -241,7 +259,7 @@ __device__ __forceinline__ void wave_union_queue(uint32_t* par, uint32_t* q, uin
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   for (uint32_t i = threadIdx.x & 63u; i < total; i += 64u) {
     const uint32_t e = q[i];
-    tunion(par, e & 0xFFFFu, e >> 16);
+    tunion<BA>(par, e & 0xFFFFu, e >> 16);
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
