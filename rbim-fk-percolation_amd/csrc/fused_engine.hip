@@ -679,14 +679,9 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
     if (lane == 0) f.xbits[((size_t)lat * tiles + tile) * (2 * T / 64) + wave] = word;
   }
   if (meas_cl) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const uint32_t o1 = __shfl_xor(ik1, off), o2 = __shfl_xor(ik2, off);
-      incl += __shfl_xor(incl, off);
-      ik2 = max(min(ik1, o1), max(ik2, o2));
-      ik1 = max(ik1, o1);
-    }
-    if (lane == 0) {
+    wave_top2(ik1, ik2);
+    incl = wave_sum63(incl);
+    if (lane == 63) {
       red[wave][2] = incl;
       red[wave][3] = ik1;
       red[wave][4] = ik2;

@@ -206,13 +206,6 @@ struct ClusterKey<true, BA> {
   static __device__ __forceinline__ int64_t n_plus(T k) { return (int64_t)(k & 0x7FFu); }
 };
 
-template <typename K>
-__device__ __forceinline__ void top2_merge(K& k1, K& k2, K o1, K o2) {
-  const K hi = max(k1, o1);
-  const K lo = max(min(k1, o1), max(k2, o2));
-  k1 = hi;
-  k2 = lo;
-}
 
 __device__ __forceinline__ uint32_t popc_ballot(bool pred) {
   return (uint32_t)__popcll(__ballot(pred));
@@ -356,13 +349,8 @@ __device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, 
     top2_merge(k1, k2, root ? key : (KT)0u, (KT)0u);
   };
   auto publish = [&]() {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const KT o1 = __shfl_xor(k1, off);
-      const KT o2 = __shfl_xor(k2, off);
-      top2_merge(k1, k2, o1, o2);
-    }
-    if (lane == 0) {
+    wave_top2(k1, k2);
+    if (lane == 63) {
       red_c[wave][0] = m_sat;
       red_c[wave][1] = m_up;
       red_c[wave][2] = m_root;

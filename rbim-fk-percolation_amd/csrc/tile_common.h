@@ -123,6 +123,58 @@ __device__ __forceinline__ void top2_insert(TileMeas& a, uint64_t k, uint32_t p)
   }
 }
 
+// Top-2 of two key pairs (0 = none), keys distinct or equal.
+template <typename K>
+__device__ __forceinline__ void top2_merge(K& k1, K& k2, K o1, K o2) {
+  const K hi = max(k1, o1);
+  const K lo = max(min(k1, o1), max(k2, o2));
+  k1 = hi;
+  k2 = lo;
+}
+
+// Data-parallel-primitive moves within a wavefront (gfx9 DPP): the value of
+// another lane, 0 where that lane lies outside the pattern (and in the rows
+// the row mask leaves out).
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xF, true);
+}
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ uint64_t dpp_key(uint64_t v) {
+  return (uint64_t)dpp_u32<CTRL, ROWS>((uint32_t)v) |
+         (uint64_t)dpp_u32<CTRL, ROWS>((uint32_t)(v >> 32)) << 32;
+}
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ uint32_t dpp_key(uint32_t v) {
+  return dpp_u32<CTRL, ROWS>(v);
+}
+constexpr int DPP_ROW_SHR1 = 0x111, DPP_ROW_SHR2 = 0x112, DPP_ROW_SHR4 = 0x114,
+              DPP_ROW_SHR8 = 0x118, DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143;
+
+// Top-2 keys of a wavefront (0 = none) into lane 63: shifts within each row of
+// 16 lanes, then row 0 -> 1 and 2 -> 3, then row 1 -> 2, 3.
+template <typename K>
+__device__ __forceinline__ void wave_top2(K& k1, K& k2) {
+  top2_merge(k1, k2, dpp_key<DPP_ROW_SHR1>(k1), dpp_key<DPP_ROW_SHR1>(k2));
+  top2_merge(k1, k2, dpp_key<DPP_ROW_SHR2>(k1), dpp_key<DPP_ROW_SHR2>(k2));
+  top2_merge(k1, k2, dpp_key<DPP_ROW_SHR4>(k1), dpp_key<DPP_ROW_SHR4>(k2));
+  top2_merge(k1, k2, dpp_key<DPP_ROW_SHR8>(k1), dpp_key<DPP_ROW_SHR8>(k2));
+  top2_merge(k1, k2, dpp_key<DPP_ROW_BCAST15, 0xA>(k1), dpp_key<DPP_ROW_BCAST15, 0xA>(k2));
+  top2_merge(k1, k2, dpp_key<DPP_ROW_BCAST31, 0xC>(k1), dpp_key<DPP_ROW_BCAST31, 0xC>(k2));
+}
+
+// Sum over a wavefront into lane 63.
+template <typename T>
+__device__ __forceinline__ T wave_sum63(T v) {
+  v += (T)dpp_u32<DPP_ROW_SHR1>((uint32_t)v);
+  v += (T)dpp_u32<DPP_ROW_SHR2>((uint32_t)v);
+  v += (T)dpp_u32<DPP_ROW_SHR4>((uint32_t)v);
+  v += (T)dpp_u32<DPP_ROW_SHR8>((uint32_t)v);
+  v += (T)dpp_u32<DPP_ROW_BCAST15, 0xA>((uint32_t)v);
+  v += (T)dpp_u32<DPP_ROW_BCAST31, 0xC>((uint32_t)v);
+  return v;
+}
+
 // Workgroup reduction of the lanes' partials in two halves, so the cluster
 // part (read before the sweep overwrites the roots) and the spin part (summed
 // while the sweep reads the spins) need not be live at the same time: each
@@ -147,12 +199,9 @@ __device__ __forceinline__ void top2_insert(TileMeas& a, uint64_t k, uint32_t p)
 }
 
 [[maybe_unused]] __device__ void wave_spins(int32_t e, int32_t m, int64_t (*red)[PARTS]) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    e += __shfl_xor(e, off);
-    m += __shfl_xor(m, off);
-  }
-  if ((threadIdx.x & 63) == 0) {
+  e = wave_sum63(e);
+  m = wave_sum63(m);
+  if ((threadIdx.x & 63) == 63) {
     red[threadIdx.x >> 6][0] = e;
     red[threadIdx.x >> 6][1] = m;
   }
