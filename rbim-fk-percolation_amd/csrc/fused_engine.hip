@@ -174,17 +174,17 @@ __device__ __forceinline__ void load_halo(const LatticeSet& s, const FusedScratc
   hsp[tid] = (uint8_t)true_spin<T>(v.in[hn], ntile, fb);
 }
 
-// Roots of x and y in the settled tile forest, walked together, reading
-// parents through a 16-bit mask (a root's word may already carry its rep slot
-// above bit 16).
+// Roots of x and y in the settled tile forest (nodes named by byte-offset
+// handles, lds_at), walked together, reading parents through a 16-bit mask (a
+// root's word may already carry its rep slot above bit 16).
 [[maybe_unused]] __device__ __forceinline__ void troot16_2(const uint32_t* par, uint32_t& x,
                                                           uint32_t& y) {
-  uint32_t px = par[x] & 0xFFFFu, py = par[y] & 0xFFFFu;
+  uint32_t px = lds_get<true>(par, x) & 0xFFFFu, py = lds_get<true>(par, y) & 0xFFFFu;
   while (px != x || py != y) {
     x = px;
     y = py;
-    px = par[x] & 0xFFFFu;
-    py = par[y] & 0xFFFFu;
+    px = lds_get<true>(par, x) & 0xFFFFu;
+    py = lds_get<true>(par, y) & 0xFFFFu;
   }
 }
 
@@ -192,12 +192,12 @@ __device__ __forceinline__ void load_halo(const LatticeSet& s, const FusedScratc
 __device__ __forceinline__ void troot16_4(const uint32_t* par, uint32_t (&x)[4]) {
   uint32_t p[4];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) p[c] = par[x[c]] & 0xFFFFu;
+  for (int c = 0; c < 4; ++c) p[c] = lds_get<true>(par, x[c]) & 0xFFFFu;
   while (p[0] != x[0] || p[1] != x[1] || p[2] != x[2] || p[3] != x[3]) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) x[c] = p[c];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) p[c] = par[x[c]] & 0xFFFFu;
+    for (int c = 0; c < 4; ++c) p[c] = lds_get<true>(par, x[c]) & 0xFFFFu;
   }
 }
 
@@ -297,6 +297,11 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
     return PART ? (row_of(ql) << rsh) + col_of(ql) : ql << 1;
   };
   const bool odd_rows = PART == 3 && (wv & 1u);  // wavefront-uniform
+  // The tile forest names its nodes by handles, byte offsets 4 li (lds_at): a
+  // parent read is the address of the next step.  H: handle of local index li,
+  // IDX: local index of handle h; lcnt is read and counted at handles as well.
+  auto H = [](uint32_t li) -> uint32_t { return li << 2; };
+  auto IDX = [](uint32_t h) -> uint32_t { return h >> 2; };
   // a local index as the tile-row index lx * T + ly
   auto tile_li = [&](uint32_t li) -> uint32_t {
     return CUT ? (li >> rsh) * T + (li & (RS - 1u)) : li;
@@ -457,8 +462,8 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
     // never leaves its row)
     const uint32_t chunk = PART ? li0 - 2u * (uint32_t)lane : 2u * (uint32_t)(it * NTH + wave * 64);
     const uint32_t start0 = run_start(__ballot(internal && outlink), __ballot(outlink), below, chunk);
-    par[li0] = start0;
-    par[li1] = internal ? start0 : li1;
+    par[li0] = H(start0);
+    par[li1] = internal ? H(start0) : H(li1);
   }
   if (meas_em) wave_spins(me, mm, red);
   __syncthreads();
@@ -494,9 +499,9 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
     }
     req |= ((b1 >> 2) & 1u & ((lane == 63 && ly0 + 2u < wy) ? 1u : 0u)) << 2;
     // the wave's queue lives in lcnt, which is cleared before it is used
-    const uint32_t qa[3] = {li0, li1, li1};
-    const uint32_t qb[3] = {li0 + RS, li1 + RS, li1 + 1u};
-    wave_union_queue(par, lcnt + wave * QW, req, qa, qb, below);
+    const uint32_t qa[3] = {H(li0), H(li1), H(li1)};
+    const uint32_t qb[3] = {H(li0 + RS), H(li1 + RS), H(li1 + 1u)};
+    wave_union_queue<true>(par, lcnt + wave * QW, req, qa, qb, below);
   }
   {  // this wave's queue, cleared with 16-byte stores (QW is a multiple of 4)
     uint4* q4 = reinterpret_cast<uint4*>(lcnt + wave * QW);
@@ -516,20 +521,20 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
     const uint64_t m0 = __ballot(in0), m1 = __ballot(in1);
     const uint64_t u0 = __ballot(in0 && (b0 & 1u)), u1 = __ballot(in1 && (b1 & 1u));
     if (lane == 0)
-      atomicAdd(&lcnt[R], (uint32_t)(__popcll(m0) + __popcll(m1)) << 16 |
+      atomicAdd(&lds_at<true>(lcnt, R), (uint32_t)(__popcll(m0) + __popcll(m1)) << 16 |
                               (uint32_t)(__popcll(u0) + __popcll(u1)));
     const bool o0 = v0 && !in0, o1 = v1 && !in1;
     if (o0 && o1 && r0 == r1) {
-      atomicAdd(&lcnt[r0], 0x20000u | ((b0 & 1u) + (b1 & 1u)));
+      atomicAdd(&lds_at<true>(lcnt, r0), 0x20000u | ((b0 & 1u) + (b1 & 1u)));
     } else {
-      if (o0) atomicAdd(&lcnt[r0], 0x10000u | (b0 & 1u));
-      if (o1) atomicAdd(&lcnt[r1], 0x10000u | (b1 & 1u));
+      if (o0) atomicAdd(&lds_at<true>(lcnt, r0), 0x10000u | (b0 & 1u));
+      if (o1) atomicAdd(&lds_at<true>(lcnt, r1), 0x10000u | (b1 & 1u));
     }
     // every site now points straight at its root; a root's word becomes
     // root | rep << 16 with rep = "none" until an edge slot claims it (the
     // walks read parents through a 16-bit mask)
-    par[li0] = r0 != li0 ? r0 : li0 | (NONE16 << 16);
-    par[li1] = r1 != li1 ? r1 : li1 | (NONE16 << 16);
+    par[li0] = r0 != H(li0) ? r0 : H(li0) | (NONE16 << 16);
+    par[li1] = r1 != H(li1) ? r1 : H(li1) | (NONE16 << 16);
   };
   if constexpr (ITERS % 2 == 0) {
     // two pairs per step: four root walks in flight together
@@ -545,7 +550,7 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
         va0 = xa < wx && ya < wy, va1 = xa < wx && ya + 1u < wy;
         vb0 = xb < wx && yb < wy, vb1 = xb < wx && yb + 1u < wy;
         if (skip(it + 1)) {  // the second pair's wavefront is past the cut
-          uint32_t r0 = la, r1 = (ba0 & 4u) && va1 ? la : la + 1u;
+          uint32_t r0 = H(la), r1 = H((ba0 & 4u) && va1 ? la : la + 1u);
           troot16_2(par, r0, r1);
           count_and_point(la, ba0, ba1, r0, r1, va0, va1);
           continue;
@@ -553,7 +558,8 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
       }
       const uint32_t bb0 = bits[lb], bb1 = bits[lb + 1u];
       // li1 shares li0's run when the in-row bond is occupied
-      uint32_t r[4] = {la, (ba0 & 4u) && va1 ? la : la + 1u, lb, (bb0 & 4u) && vb1 ? lb : lb + 1u};
+      uint32_t r[4] = {H(la), H((ba0 & 4u) && va1 ? la : la + 1u), H(lb),
+                       H((bb0 & 4u) && vb1 ? lb : lb + 1u)};
       troot16_4(par, r);
       count_and_point(la, ba0, ba1, r[0], r[1], va0, va1);
       count_and_point(lb, bb0, bb1, r[2], r[3], vb0, vb1);
@@ -567,19 +573,19 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
       const uint32_t b0 = bits[li0], b1 = bits[li1];
       const bool v0 = !CUT || (row_of(ql) < wx && col_of(ql) < wy);
       const bool v1 = !CUT || (row_of(ql) < wx && col_of(ql) + 1u < wy);
-      uint32_t r0 = li0, r1 = (b0 & 4u) && v1 ? li0 : li1;  // li1 shares li0's run when linked
+      uint32_t r0 = H(li0), r1 = H((b0 & 4u) && v1 ? li0 : li1);  // li1 shares li0's run when linked
       troot16_2(par, r0, r1);
       count_and_point(li0, b0, b1, r0, r1, v0, v1);
     }
   }
   __syncthreads();
-  // rep slot of every edge cluster: par[root] = root | min slot << 16
+  // rep slot of every edge cluster: the root's word = root | min slot << 16
   for (uint32_t k = tid; k < NS; k += NTH) {
     bool ok = true;
     const uint32_t li = CUT ? slot_li_cut<T>(k, wx, wy, rsh, ok) : slot_li<T>(k);
     if (!ok) continue;
     const uint32_t r = par[li] & 0xFFFFu;
-    atomicMin(&par[r], r | (k << 16));
+    atomicMin(&lds_at<true>(par, r), r | (k << 16));
   }
   __syncthreads();
 
@@ -601,16 +607,16 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
     for (int j = 0; j < 2; ++j) {
       const uint32_t li = li0 + (uint32_t)j;
       const uint32_t r = par[li] & 0xFFFFu;
-      const uint32_t rp = par[r];
+      const uint32_t rp = lds_get<true>(par, r);
       const uint32_t rep = rp >> 16;
-      const uint32_t br = bits[r];
+      const uint32_t br = bits[IDX(r)];
       const uint32_t fl = (br >> 3) & 1u;
       const uint32_t sbit = bits[li] & 1u;
       const bool edge = rep != NONE16;
       const uint32_t ns = edge ? sbit : sbit ^ fl;
       st2 |= ((ns ? ST_SPIN : 0u) | (edge ? ST_EDGE | rep : 0u)) << (16 * j);
-      if (meas_cl && !edge && r == li && vs[j]) {  // an interior cluster, at its root
-        const uint32_t key = ((lcnt[r] >> 16) << 12) | (0xFFFu - li);
+      if (meas_cl && !edge && r == H(li) && vs[j]) {  // an interior cluster, at its root
+        const uint32_t key = ((lds_get<true>(lcnt, r) >> 16) << 12) | (0xFFFu - li);
         ++incl;
         ik2 = max(ik2, min(ik1, key));
         ik1 = max(ik1, key);
@@ -640,13 +646,13 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
         continue;
       }
       const uint32_t r = par[li] & 0xFFFFu;
-      const uint32_t rep = par[r] >> 16;
+      const uint32_t rep = lds_get<true>(par, r) >> 16;
       f.guf[sb + k] = gpack(tile * NS + rep, 0, 0);
       if (rep == k) {
-        const uint32_t c = lcnt[r];
+        const uint32_t c = lds_get<true>(lcnt, r);
         f.gcnt[sb + k] = ((uint64_t)(c >> 16) << 32) | (c & 0xFFFFu);
-        const uint32_t rt = tile_li(r);
-        f.gmn[sb + k] = (((x0 + rt / T) * L + y0 + rt % T) << 1) | ((bits[r] >> 3) & 1u);
+        const uint32_t rt = tile_li(IDX(r));
+        f.gmn[sb + k] = (((x0 + rt / T) * L + y0 + rt % T) << 1) | ((bits[IDX(r)] >> 3) & 1u);
       } else {
         f.gmn[sb + k] = 0xFFFFFFFFu;
       }
@@ -661,9 +667,9 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
         continue;
       }
       const uint32_t r = par[li] & 0xFFFFu;
-      const uint32_t rep = par[r] >> 16;
-      const uint32_t c = lcnt[r];
-      sl[k] = slot_entry(rep, tile_li(r), (bits[r] >> 3) & 1u, c >> 16, c & 0xFFFFu);
+      const uint32_t rep = lds_get<true>(par, r) >> 16;
+      const uint32_t c = lds_get<true>(lcnt, r);
+      sl[k] = slot_entry(rep, tile_li(IDX(r)), (bits[IDX(r)] >> 3) & 1u, c >> 16, c & 0xFFFFu);
     }
   }
   if (tid < 2 * T) {
