@@ -29,7 +29,9 @@
 //                          minimum site and that site's flip bit.
 //              The occupation of the 2T bonds leaving the tile (+x, +y) goes out
 //              as a bitmask.
-//   KB lattice one workgroup per lattice.  Union-find over the lattice's edge
+//   KB lattice one workgroup per lattice (kb_merge; kb_merge_c packs a slot
+//              into one 64-bit word while the lattice has at most 4096 slots and
+//              65536 sites).  Union-find over the lattice's edge
 //              slots in LDS: slots start under their rep (winding 0: same
 //              tile), every occupied tile-crossing bond unites its two slots
 //              with its periodic winding; a bond that finds one root closes a
@@ -1100,6 +1102,231 @@ __global__ __launch_bounds__(KB_THREADS) void kb_merge(LatticeSet s, FusedScratc
 }
 
 // ------------------------------------------- slot merge in HBM (large L)
+// ------------------------------------------------------------ KB, compact
+// The same merge for lattices of at most 4096 slots and 65536 sites (every
+// L <= 256), in one 64-bit LDS word per slot (32 KB at 4096 slots against 80
+// KB for kb_merge, so four to five workgroups share a CU instead of two):
+//   non-root  parent (bits 0..11) | wx (12..27) | wy (28..43), the int16
+//             windings of the tree path to the parent
+//   root      itself (0..11) | mn (12..28): its cluster's minimum site << 1 |
+//             that site's flip bit | size (29..45) | n_plus (46..62); size 0
+//             marks a position of a cut tile that holds no site
+// Roots hook the one with the larger mn under the smaller, so every root
+// already holds its cluster's minimum site and flip bit; the reps that end up
+// below a root add their counts to its word.  A winding is that of a path in
+// the spanning forest of the hooking bonds, a simple path, which crosses each
+// of the L boundary bonds of a direction at most once: |w| <= L fits 16 bits.
+constexpr uint32_t KC_P = 0xFFFu;
+__device__ __forceinline__ uint64_t kc_link(uint32_t p, int wx, int wy) {
+  return (uint64_t)p | ((uint64_t)(uint16_t)(int16_t)wx << 12) |
+         ((uint64_t)(uint16_t)(int16_t)wy << 28);
+}
+__device__ __forceinline__ uint64_t kc_root(uint32_t k, uint32_t mn, uint32_t size, uint32_t np) {
+  return (uint64_t)k | ((uint64_t)mn << 12) | ((uint64_t)size << 29) | ((uint64_t)np << 46);
+}
+__device__ __forceinline__ uint32_t kc_par(uint64_t v) { return (uint32_t)v & KC_P; }
+__device__ __forceinline__ int kc_wx(uint64_t v) { return (int)(int16_t)(uint16_t)(v >> 12); }
+__device__ __forceinline__ int kc_wy(uint64_t v) { return (int)(int16_t)(uint16_t)(v >> 28); }
+__device__ __forceinline__ uint32_t kc_mn(uint64_t v) { return (uint32_t)(v >> 12) & 0x1FFFFu; }
+__device__ __forceinline__ uint32_t kc_size(uint64_t v) { return (uint32_t)(v >> 29) & 0x1FFFFu; }
+__device__ __forceinline__ uint32_t kc_np(uint64_t v) { return (uint32_t)(v >> 46) & 0x1FFFFu; }
+
+// kfind2 on compact words; va / vb end as the roots' words.
+__device__ __forceinline__ void kc_find2(uint64_t* uf, uint32_t& a, uint64_t& va, int& ax, int& ay,
+                                         uint32_t& b, uint64_t& vb, int& bx, int& by) {
+  va = uf[a];
+  vb = uf[b];
+  while (true) {
+    const uint32_t pa = kc_par(va), pb = kc_par(vb);
+    const bool da = pa == a, db = pb == b;
+    if (da && db) return;
+    const uint64_t wa = da ? va : uf[pa];
+    const uint64_t wb = db ? vb : uf[pb];
+    if (!da) {
+      const uint32_t ga = kc_par(wa);
+      if (ga == pa) {
+        ax += kc_wx(va);
+        ay += kc_wy(va);
+        a = pa;
+        va = wa;
+      } else {
+        const int hx = kc_wx(va) + kc_wx(wa), hy = kc_wy(va) + kc_wy(wa);
+        uf[a] = kc_link(ga, hx, hy);
+        ax += hx;
+        ay += hy;
+        a = ga;
+        va = uf[ga];
+      }
+    }
+    if (!db) {
+      const uint32_t gb = kc_par(wb);
+      if (gb == pb) {
+        bx += kc_wx(vb);
+        by += kc_wy(vb);
+        b = pb;
+        vb = wb;
+      } else {
+        const int hx = kc_wx(vb) + kc_wx(wb), hy = kc_wy(vb) + kc_wy(wb);
+        uf[b] = kc_link(gb, hx, hy);
+        bx += hx;
+        by += hy;
+        b = gb;
+        vb = uf[gb];
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t kc_rootof(const uint64_t* uf, uint32_t x) {
+  uint32_t p = kc_par(uf[x]);
+  while (p != x) {
+    x = p;
+    p = kc_par(uf[x]);
+  }
+  return x;
+}
+
+// Occupied bond a -> b crossing the periodic boundary (cx, cy) times; wrap
+// bits when it closes a loop.  A root's word changes only when it is hooked,
+// so a CAS against the word its find read fails exactly when it was.
+__device__ uint32_t kc_union(uint64_t* uf, uint32_t a, uint32_t b, int cx, int cy) {
+  int ax = 0, ay = 0, bx = 0, by = 0;
+  while (true) {
+    uint64_t va, vb;
+    kc_find2(uf, a, va, ax, ay, b, vb, bx, by);
+    if (a == b) return ((cx + bx - ax) != 0 ? 1u : 0u) | ((cy + by - ay) != 0 ? 2u : 0u);
+    if (kc_mn(va) < kc_mn(vb)) {
+      if (atomicCAS((unsigned long long*)&uf[b], (unsigned long long)vb,
+                    (unsigned long long)kc_link(a, ax - bx - cx, ay - by - cy)) == vb)
+        return 0u;
+    } else {
+      if (atomicCAS((unsigned long long*)&uf[a], (unsigned long long)va,
+                    (unsigned long long)kc_link(b, bx - ax + cx, by - ay + cy)) == va)
+        return 0u;
+    }
+  }
+}
+
+template <int T, int NTH>
+__global__ __launch_bounds__(NTH) void kb_merge_c(LatticeSet s, FusedScratch f) {
+  constexpr uint32_t NS = 4u * T;
+  constexpr int WORK = NTH - 64;  // the last wavefront folds the previous sweep first
+  extern __shared__ uint64_t kbsm[];  // one word per slot; the reduction rows at the end
+  const int lat = run_lat(s, blockIdx.x);
+  if (!lattice_active(s, lat)) return;
+  const uint32_t ns = (uint32_t)f.nslots, nt = (uint32_t)f.tiles_x, L = (uint32_t)s.L;
+  uint64_t* uf = kbsm;
+  const int tid = threadIdx.x;
+  const int64_t g = s.sweep_ctr[lat];
+  const uint64_t* ent = f.slots + (size_t)lat * ns;
+  const bool worker = tid < WORK;
+  if (tid == WORK && g >= 1 && f.fin[lat] == g - 1) {
+    if (g - 1 >= lat_therm(s, lat)) fused_finalize(s, f, lat, g - 1);
+    f.fin[lat] = g;
+  }
+  for (uint32_t k = tid; worker && k < ns; k += WORK) {
+    const uint64_t e = ent[k];
+    const uint32_t t = k / NS, loc = k % NS, rep = (uint32_t)(e & 0xFFFu);
+    if (e & SLOT_ABSENT) {
+      uf[k] = kc_root(k, 0u, 0u, 0u);
+    } else if (rep == loc) {
+      const uint32_t r = (uint32_t)(e >> 12) & 0xFFFu, fl = (uint32_t)(e >> 24) & 1u;
+      const uint32_t size = (uint32_t)(e >> 25) & 0x1FFFu, np = (uint32_t)(e >> 38) & 0x1FFFu;
+      const uint32_t site = ((t % nt) * T + r / T) * L + (t / nt) * T + r % T;
+      uf[k] = kc_root(k, (site << 1) | fl, size, np);
+    } else {
+      uf[k] = kc_link(t * NS + rep, 0, 0);
+    }
+  }
+  __syncthreads();
+  // tile-crossing bonds, as in kb_merge
+  const uint32_t nb = (ns / NS) * 2u * T;
+  uint32_t wraps = 0u;
+  for (uint32_t b = tid; worker && b < nb; b += WORK) {
+    const uint32_t t = b / (2u * T), i = b % (2u * T);
+    const uint64_t word = f.xbits[((size_t)lat * (ns / NS) + t) * (2 * T / 64) + i / 64];
+    if (!((word >> (i % 64)) & 1ull)) continue;
+    const uint32_t tx = t % nt, ty = t / nt;
+    if (i < (uint32_t)T) {
+      const uint32_t tr = ty * nt + (tx + 1u == nt ? 0u : tx + 1u);
+      wraps |= kc_union(uf, t * NS + T + i, tr * NS + i, tx + 1u == nt ? 1 : 0, 0);
+    } else {
+      const uint32_t j = i - T;
+      const uint32_t tu = (ty + 1u == nt ? 0u : ty + 1u) * nt + tx;
+      wraps |= kc_union(uf, t * NS + 3u * T + j, tu * NS + 2u * T + j, 0, ty + 1u == nt ? 1 : 0);
+    }
+  }
+  const uint64_t wmx = __ballot((wraps & 1u) != 0u), wmy = __ballot((wraps & 2u) != 0u);
+  const uint32_t wave_wraps = (wmx ? 1u : 0u) | (wmy ? 2u : 0u);
+  __syncthreads();
+  // reps below a root add their counts to its word and point straight at it
+  // (the windings are no longer read); the root word's parent bits never
+  // change, so walks meanwhile still end there
+  for (uint32_t k = tid; worker && k < ns; k += WORK) {
+    const uint64_t e = ent[k];
+    if ((e & SLOT_ABSENT) || (uint32_t)(e & 0xFFFu) != k % NS) continue;
+    const uint32_t r = kc_rootof(uf, k);
+    if (r != k) {
+      const uint64_t size = (e >> 25) & 0x1FFFu, np = (e >> 38) & 0x1FFFu;
+      atomicAdd((unsigned long long*)&uf[r], (unsigned long long)((size << 29) | (np << 46)));
+      uf[k] = kc_link(r, 0, 0);
+    }
+  }
+  __syncthreads();
+  // flip bits per slot; merged clusters measured at their roots
+  const bool meas_cl = g >= lat_therm(s, lat);
+  TileMeas ms = {0, 0, 0, 0ull, 0ull, 0u, 0u};
+  uint32_t* fbo = f.flipbits + (size_t)lat * (ns / 32u);
+  for (uint32_t k0 = 0; worker && k0 < ns; k0 += WORK) {
+    const uint32_t k = k0 + tid;
+    uint32_t fl = 0u;
+    if (k < ns) {
+      const uint32_t r = kc_rootof(uf, k);
+      const uint64_t w = uf[r];
+      const uint32_t m = kc_mn(w), size = kc_size(w);
+      fl = m & 1u;
+      if (meas_cl && r == k && size != 0u) {  // (absent slots are no cluster)
+        const uint32_t np = kc_np(w);
+        ++ms.ncl;
+        top2_insert(ms, ((uint64_t)size << 32) | (uint64_t)(0xFFFFFFFFu - (m >> 1)),
+                    fl ? size - np : np);
+      }
+    }
+    const uint64_t word = __ballot(fl != 0u);
+    if ((tid & 63) == 0 && k < ns) {
+      fbo[k / 32] = (uint32_t)word;
+      if (k + 32 < ns) fbo[k / 32 + 1] = (uint32_t)(word >> 32);
+    }
+  }
+  __syncthreads();  // the slot words are dead: their first rows take the reduction
+  int64_t (*red)[PARTS] = reinterpret_cast<int64_t(*)[PARTS]>(kbsm);
+  if (meas_cl) wave_clusters(ms, red);
+  if ((tid & 63) == 0) red[tid >> 6][0] = wave_wraps;
+  __syncthreads();
+  if (tid == 0) {
+    if (meas_cl) {
+      TileMeas a = {0, 0, 0, 0ull, 0ull, 0u, 0u};
+      int64_t ncl = 0;
+      uint32_t wb = 0u;
+      for (int w = 0; w < NTH / 64; ++w) {
+        wb |= (uint32_t)red[w][0];
+        ncl += red[w][2];
+        top2_insert(a, (uint64_t)red[w][3], (uint32_t)red[w][5]);
+        top2_insert(a, (uint64_t)red[w][4], (uint32_t)red[w][6]);
+      }
+      int64_t* o = f.bst + ((size_t)(g & 1) * s.n_lat + lat) * 6;
+      o[0] = ncl;
+      o[1] = (int64_t)a.k1;
+      o[2] = (int64_t)a.k2;
+      o[3] = a.p1;
+      o[4] = a.p2;
+      o[5] = wb;
+    }
+    f.cur[lat] ^= 1;
+    s.sweep_ctr[lat] = g + 1;
+  }
+}
+
 // The same merge as kb_merge for lattices whose slots do not fit one
 // workgroup's LDS: the slot forest (parent | wx | wy), the rep counts and the
 // (min site << 1 | flip) words live in HBM, one thread per bond / slot.
@@ -1401,6 +1628,19 @@ __global__ void kf_close(LatticeSet s, FusedScratch f) {
 
 size_t kb_lds_bytes(int nslots) { return (size_t)nslots * (8 + 8 + 4); }
 
+// The compact merge: workgroup size, LDS (one word per slot, at least the
+// reduction rows), and whether a lattice fits it (slot indices in 12 bits,
+// sites in 16).  Measured on 1280-lattice scans: 512 threads at 4096 slots (L
+// = 200, 256: four workgroups of 32 KB per CU; 256 threads 5 % and 1024
+// threads 9 % slower at L = 256), 256 at 2304 (L = 129: 5 % ahead of 512).
+int kc_threads(int nslots) { return nslots > 3072 ? 512 : 256; }
+size_t kc_lds_bytes(int nslots, int threads) {
+  return std::max((size_t)nslots * 8, (size_t)(threads / 64) * PARTS * 8);
+}
+bool kc_fits(const LatticeSet& s, const FusedScratch& f) {
+  return f.nslots <= (int)KC_P + 1 && s.N <= 65536;
+}
+
 // The tile passes of one sweep: every tile in one launch when T divides L,
 // else the whole tiles and the cut ones (in workgroups of half the threads) in
 // two.
@@ -1432,6 +1672,16 @@ hipError_t launch_fused_t(const LatticeSet& s, const FusedScratch& f, hipStream_
     hipLaunchKernelGGL(kg_fold, dim3(nb, s.n_run), dim3(256), 0, st, s, f);
     hipLaunchKernelGGL(kg_flip, dim3(nb, s.n_run), dim3(256), 0, st, s, f);
     hipLaunchKernelGGL(kg_sweep_end, dim3(s.n_run), dim3(KG_END_THREADS), 0, st, s, f, nb);
+    return hipGetLastError();
+  }
+  if (kc_fits(s, f)) {  // compact slot words
+    launch_ka<T, false, PART>(s, f, st);
+    if (kc_threads(f.nslots) == 512)
+      hipLaunchKernelGGL((kb_merge_c<T, 512>), dim3(s.n_run), dim3(512), kc_lds_bytes(f.nslots, 512),
+                         st, s, f);
+    else
+      hipLaunchKernelGGL((kb_merge_c<T, 256>), dim3(s.n_run), dim3(256), kc_lds_bytes(f.nslots, 256),
+                         st, s, f);
     return hipGetLastError();
   }
   static std::atomic<uint64_t> configured{0};
