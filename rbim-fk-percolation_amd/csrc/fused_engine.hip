@@ -29,17 +29,17 @@
 //                          minimum site and that site's flip bit.
 //              The occupation of the 2T bonds leaving the tile (+x, +y) goes out
 //              as a bitmask.
-//   KB lattice one workgroup per lattice (kb_merge; kb_merge_c packs a slot
-//              into one 64-bit word while the lattice has at most 4096 slots and
-//              65536 sites).  Union-find over the lattice's edge
+//   KB lattice one workgroup per lattice, one 64-bit LDS word per slot.
+//              Union-find over the lattice's edge
 //              slots in LDS: slots start under their rep (winding 0: same
 //              tile), every occupied tile-crossing bond unites its two slots
 //              with its periodic winding; a bond that finds one root closes a
 //              loop and c + w(b) - w(a) decides wrap_x / wrap_y exactly
-//              (_unionfind.py:101-124).  Each merged cluster sums its reps'
-//              sizes and n_plus and takes the smallest (min site << 1 | flip):
-//              its flip bit is the flip bit of its minimum site, the rule of
-//              every engine here.  Flip bits go out per slot; the merged
+//              (_unionfind.py:101-124).  Roots hook by minimum site, so a
+//              merged cluster's root is the rep holding its minimum site and
+//              the cluster flips by that site's flip bit, the rule of every
+//              engine here; it sums its reps' sizes and n_plus.  Flip bits go
+//              out per slot; the merged
 //              clusters are measured; the previous sweep is folded into its
 //              bin (E, M from this sweep's KA; interior clusters from the
 //              previous KA; edge clusters and wraps from the previous KB).
@@ -768,90 +768,6 @@ constexpr size_t cut_lds_bytes(uint32_t cap) {
   return (size_t)cap * 9u + ((2u * T + 7u) & ~7u) + (size_t)(NTH / 64) * PARTS * 8u;
 }
 
-// ------------------------------------------------------------------- KB
-__device__ __forceinline__ uint64_t kpack(uint32_t p, int wx, int wy) {
-  return (uint64_t)p | ((uint64_t)(uint16_t)(int16_t)wx << 32) |
-         ((uint64_t)(uint16_t)(int16_t)wy << 48);
-}
-__device__ __forceinline__ uint32_t kpar(uint64_t v) { return (uint32_t)v; }
-__device__ __forceinline__ int kwx(uint64_t v) { return (int)(int16_t)(uint16_t)(v >> 32); }
-__device__ __forceinline__ int kwy(uint64_t v) { return (int)(int16_t)(uint16_t)(v >> 48); }
-
-// Roots of a and b in the slot forest with the windings of the paths to them,
-// walked together (both chains' loads in flight at once) with path halving;
-// stores only touch non-root words, which hooks never target.
-__device__ __forceinline__ void kfind2(uint64_t* uf, uint32_t& a, int& ax, int& ay, uint32_t& b,
-                                       int& bx, int& by) {
-  uint64_t va = uf[a], vb = uf[b];
-  while (true) {
-    const uint32_t pa = kpar(va), pb = kpar(vb);
-    const bool da = pa == a, db = pb == b;
-    if (da && db) return;
-    const uint64_t wa = da ? va : uf[pa];
-    const uint64_t wb = db ? vb : uf[pb];
-    if (!da) {
-      const uint32_t ga = kpar(wa);
-      if (ga == pa) {
-        ax += kwx(va);
-        ay += kwy(va);
-        a = pa;
-        va = wa;
-      } else {
-        const int hx = kwx(va) + kwx(wa), hy = kwy(va) + kwy(wa);
-        uf[a] = kpack(ga, hx, hy);
-        ax += hx;
-        ay += hy;
-        a = ga;
-        va = uf[ga];
-      }
-    }
-    if (!db) {
-      const uint32_t gb = kpar(wb);
-      if (gb == pb) {
-        bx += kwx(vb);
-        by += kwy(vb);
-        b = pb;
-        vb = wb;
-      } else {
-        const int hx = kwx(vb) + kwx(wb), hy = kwy(vb) + kwy(wb);
-        uf[b] = kpack(gb, hx, hy);
-        bx += hx;
-        by += hy;
-        b = gb;
-        vb = uf[gb];
-      }
-    }
-  }
-}
-
-__device__ __forceinline__ uint32_t kroot(const uint64_t* uf, uint32_t x) {
-  uint32_t p = kpar(uf[x]);
-  while (p != x) {
-    x = p;
-    p = kpar(uf[x]);
-  }
-  return x;
-}
-
-// Occupied bond a -> b crossing the periodic boundary (cx, cy) times; wrap
-// bits (1: x, 2: y) when it closes a loop.
-__device__ uint32_t kunion(uint64_t* uf, uint32_t a, uint32_t b, int cx, int cy) {
-  int ax = 0, ay = 0, bx = 0, by = 0;
-  while (true) {
-    kfind2(uf, a, ax, ay, b, bx, by);
-    if (a == b) return ((cx + bx - ax) != 0 ? 1u : 0u) | ((cy + by - ay) != 0 ? 2u : 0u);
-    if (a < b) {
-      if (atomicCAS((unsigned long long*)&uf[b], (unsigned long long)kpack(b, 0, 0),
-                    (unsigned long long)kpack(a, ax - bx - cx, ay - by - cy)) == kpack(b, 0, 0))
-        return 0u;
-    } else {
-      if (atomicCAS((unsigned long long*)&uf[a], (unsigned long long)kpack(a, 0, 0),
-                    (unsigned long long)kpack(b, bx - ax + cx, by - ay + cy)) == kpack(a, 0, 0))
-        return 0u;
-    }
-  }
-}
-
 // Fold measured sweep gm of lattice lat into its bin: E, M from part_em,
 // interior clusters from part_int[gm & 1], edge clusters and wraps from
 // bst[gm & 1].  Single thread.
@@ -970,166 +886,37 @@ __device__ void fused_finalize_block(const LatticeSet& s, const FusedScratch& f,
   }
 }
 
-constexpr int KB_THREADS = 1024;
-// The last wavefront of a merge workgroup folds the previous sweep into its
-// bin (one thread, a few tiles per lattice) while the others merge.
-constexpr int KB_WORK = KB_THREADS - 64;
-
-template <int T>
-__global__ __launch_bounds__(KB_THREADS) void kb_merge(LatticeSet s, FusedScratch f) {
-  constexpr uint32_t NS = 4u * T;
-  extern __shared__ uint64_t kbsm[];
-  __shared__ int64_t red[KB_THREADS / 64][PARTS];
-  __shared__ uint32_t wrapbits;
-  const int lat = run_lat(s, blockIdx.x);
-  if (!lattice_active(s, lat)) return;
-  const uint32_t ns = (uint32_t)f.nslots, nt = (uint32_t)f.tiles_x, L = (uint32_t)s.L;
-  uint64_t* uf = kbsm;                                 // parent | wx:16 | wy:16
-  uint64_t* cnt = kbsm + ns;                           // size << 32 | n_plus, at reps
-  uint32_t* mn = reinterpret_cast<uint32_t*>(kbsm + 2 * ns);  // min site << 1 | flip, at reps
-  const int tid = threadIdx.x;
-  const int64_t g = s.sweep_ctr[lat];
-  const uint64_t* ent = f.slots + (size_t)lat * ns;
-  const bool worker = tid < KB_WORK;
-  if (tid == 0) wrapbits = 0u;
-  // sweep g-1 is complete (its E, M came from this sweep's KA; its interior
-  // clusters, edge clusters and wraps sit in the other parity's partials)
-  if (tid == KB_WORK && g >= 1 && f.fin[lat] == g - 1) {
-    if (g - 1 >= lat_therm(s, lat)) fused_finalize(s, f, lat, g - 1);
-    f.fin[lat] = g;
-  }
-  for (uint32_t k = tid; worker && k < ns; k += KB_WORK) {
-    const uint64_t e = ent[k];
-    if (e & SLOT_ABSENT) {  // a position of a cut tile without a site
-      uf[k] = kpack(k, 0, 0);
-      mn[k] = 0xFFFFFFFFu;
-      continue;
-    }
-    const uint32_t t = k / NS, loc = k % NS, rep = (uint32_t)(e & 0xFFFu);
-    uf[k] = kpack(t * NS + rep, 0, 0);
-    if (rep == loc) {
-      const uint32_t r = (uint32_t)(e >> 12) & 0xFFFu, fl = (uint32_t)(e >> 24) & 1u;
-      const uint32_t size = (uint32_t)(e >> 25) & 0x1FFFu, np = (uint32_t)(e >> 38) & 0x1FFFu;
-      const uint32_t site = ((t % nt) * T + r / T) * L + (t / nt) * T + r % T;
-      cnt[k] = ((uint64_t)size << 32) | np;
-      mn[k] = (site << 1) | fl;
-    } else {
-      mn[k] = 0xFFFFFFFFu;
-    }
-  }
-  __syncthreads();
-  // tile-crossing bonds: tid-th bond of tile t: i < T is the +x bond of slot
-  // (side 1, i) into the right neighbour's (side 0, i); else the +y bond of
-  // (side 3, i-T) into the upper neighbour's (side 2, i-T)
-  const uint32_t nb = (ns / NS) * 2u * T;
-  uint32_t wraps = 0u;
-  for (uint32_t b = tid; worker && b < nb; b += KB_WORK) {
-    const uint32_t t = b / (2u * T), i = b % (2u * T);
-    const uint64_t word = f.xbits[((size_t)lat * (ns / NS) + t) * (2 * T / 64) + i / 64];
-    if (!((word >> (i % 64)) & 1ull)) continue;
-    const uint32_t tx = t % nt, ty = t / nt;
-    if (i < (uint32_t)T) {
-      const uint32_t tr = ty * nt + (tx + 1u == nt ? 0u : tx + 1u);
-      wraps |= kunion(uf, t * NS + T + i, tr * NS + i, tx + 1u == nt ? 1 : 0, 0);
-    } else {
-      const uint32_t j = i - T;
-      const uint32_t tu = (ty + 1u == nt ? 0u : ty + 1u) * nt + tx;
-      wraps |= kunion(uf, t * NS + 3u * T + j, tu * NS + 2u * T + j, 0, ty + 1u == nt ? 1 : 0);
-    }
-  }
-  wave_or_wraps(&wrapbits, wraps);
-  __syncthreads();
-  // reps fold into their roots and are pointed straight at them (the forest is
-  // settled and its windings are no longer read); per-lane LDS atomics, as
-  // the wavefront sums of kg_fold cost more here than they save
-  for (uint32_t k = tid; worker && k < ns; k += KB_WORK) {
-    if (mn[k] == 0xFFFFFFFFu) continue;
-    const uint32_t r = kroot(uf, k);
-    if (r != k) {
-      atomicAdd((unsigned long long*)&cnt[r], (unsigned long long)cnt[k]);
-      atomicMin(&mn[r], mn[k]);
-      uf[k] = kpack(r, 0, 0);
-    }
-  }
-  __syncthreads();
-  // flip bits per slot; merged clusters measured at their roots
-  const bool meas_cl = g >= lat_therm(s, lat);
-  TileMeas ms = {0, 0, 0, 0ull, 0ull, 0u, 0u};
-  uint32_t* fbo = f.flipbits + (size_t)lat * (ns / 32u);
-  for (uint32_t k0 = 0; worker && k0 < ns; k0 += KB_WORK) {
-    const uint32_t k = k0 + tid;
-    uint32_t fl = 0u;
-    if (k < ns) {
-      const uint32_t r = kroot(uf, k);
-      const uint32_t m = mn[r];
-      fl = m & 1u;
-      if (meas_cl && r == k && m != 0xFFFFFFFFu) {  // (absent slots are no cluster)
-        const uint64_t c = cnt[r];
-        const uint32_t size = (uint32_t)(c >> 32), np = (uint32_t)c;
-        ++ms.ncl;
-        top2_insert(ms, ((uint64_t)size << 32) | (uint64_t)(0xFFFFFFFFu - (m >> 1)),
-                    fl ? size - np : np);
-      }
-    }
-    const uint64_t word = __ballot(fl != 0u);
-    if ((tid & 63) == 0 && k < ns) {
-      fbo[k / 32] = (uint32_t)word;
-      if (k + 32 < ns) fbo[k / 32 + 1] = (uint32_t)(word >> 32);
-    }
-  }
-  if (meas_cl) wave_clusters(ms, red);
-  __syncthreads();
-  if (tid == 0) {
-    if (meas_cl) {
-      TileMeas a = {0, 0, 0, 0ull, 0ull, 0u, 0u};
-      int64_t ncl = 0;
-      for (int w = 0; w < KB_THREADS / 64; ++w) {
-        ncl += red[w][2];
-        top2_insert(a, (uint64_t)red[w][3], (uint32_t)red[w][5]);
-        top2_insert(a, (uint64_t)red[w][4], (uint32_t)red[w][6]);
-      }
-      int64_t* o = f.bst + ((size_t)(g & 1) * s.n_lat + lat) * 6;
-      o[0] = ncl;
-      o[1] = (int64_t)a.k1;
-      o[2] = (int64_t)a.k2;
-      o[3] = a.p1;
-      o[4] = a.p2;
-      o[5] = wrapbits;
-    }
-    f.cur[lat] ^= 1;
-    s.sweep_ctr[lat] = g + 1;
-  }
-}
-
-// ------------------------------------------- slot merge in HBM (large L)
-// ------------------------------------------------------------ KB, compact
-// The same merge for lattices of at most 4096 slots and 65536 sites (every
-// L <= 256), in one 64-bit LDS word per slot (32 KB at 4096 slots against 80
-// KB for kb_merge, so four to five workgroups share a CU instead of two):
-//   non-root  parent (bits 0..11) | wx (12..27) | wy (28..43), the int16
+// ------------------------------------------------------------------- KB
+// One workgroup per lattice merges its edge slots in LDS, one 64-bit word per
+// slot (at most 8192 slots and 131071 sites; 32 KB at L = 256's 4096 slots,
+// so four workgroups share a CU):
+//   non-root  parent (bits 0..12) | wx (13..28) | wy (29..44), the int16
 //             windings of the tree path to the parent
-//   root      itself (0..11) | mn (12..28): its cluster's minimum site << 1 |
-//             that site's flip bit | size (29..45) | n_plus (46..62); size 0
-//             marks a position of a cut tile that holds no site
-// Roots hook the one with the larger mn under the smaller, so every root
-// already holds its cluster's minimum site and flip bit; the reps that end up
-// below a root add their counts to its word.  A winding is that of a path in
-// the spanning forest of the hooking bonds, a simple path, which crosses each
-// of the L boundary bonds of a direction at most once: |w| <= L fits 16 bits.
-constexpr uint32_t KC_P = 0xFFFu;
+//   root      itself (0..12) | its cluster's minimum site (13..29) | size
+//             (30..46) | n_plus before the flip (47..63); size 0 marks a
+//             position of a cut tile that holds no site
+// Roots hook the one with the larger minimum site under the smaller, so every
+// root is the rep holding its cluster's minimum site, whose flip bit (the
+// cluster's) stays in the root's slot entry; the reps that end up below a root
+// add their counts to its word.  A winding is that of a path in the spanning
+// forest of the hooking bonds, a simple path, which crosses each of the L
+// boundary bonds of a direction at most once: |w| <= L fits 16 bits.
+constexpr uint32_t KC_P = 0x1FFFu;
+// every lattice merged in LDS fits: at most 6400 slots, L <= 320 at T = 64
+static_assert(RBIM_FUSED_MAX_SLOTS <= (int)KC_P + 1 && 320 * 320 < (1 << 17), "slot word fields");
 __device__ __forceinline__ uint64_t kc_link(uint32_t p, int wx, int wy) {
-  return (uint64_t)p | ((uint64_t)(uint16_t)(int16_t)wx << 12) |
-         ((uint64_t)(uint16_t)(int16_t)wy << 28);
+  return (uint64_t)p | ((uint64_t)(uint16_t)(int16_t)wx << 13) |
+         ((uint64_t)(uint16_t)(int16_t)wy << 29);
 }
-__device__ __forceinline__ uint64_t kc_root(uint32_t k, uint32_t mn, uint32_t size, uint32_t np) {
-  return (uint64_t)k | ((uint64_t)mn << 12) | ((uint64_t)size << 29) | ((uint64_t)np << 46);
+__device__ __forceinline__ uint64_t kc_root(uint32_t k, uint32_t site, uint32_t size, uint32_t np) {
+  return (uint64_t)k | ((uint64_t)site << 13) | ((uint64_t)size << 30) | ((uint64_t)np << 47);
 }
 __device__ __forceinline__ uint32_t kc_par(uint64_t v) { return (uint32_t)v & KC_P; }
-__device__ __forceinline__ int kc_wx(uint64_t v) { return (int)(int16_t)(uint16_t)(v >> 12); }
-__device__ __forceinline__ int kc_wy(uint64_t v) { return (int)(int16_t)(uint16_t)(v >> 28); }
-__device__ __forceinline__ uint32_t kc_mn(uint64_t v) { return (uint32_t)(v >> 12) & 0x1FFFFu; }
-__device__ __forceinline__ uint32_t kc_size(uint64_t v) { return (uint32_t)(v >> 29) & 0x1FFFFu; }
-__device__ __forceinline__ uint32_t kc_np(uint64_t v) { return (uint32_t)(v >> 46) & 0x1FFFFu; }
+__device__ __forceinline__ int kc_wx(uint64_t v) { return (int)(int16_t)(uint16_t)(v >> 13); }
+__device__ __forceinline__ int kc_wy(uint64_t v) { return (int)(int16_t)(uint16_t)(v >> 29); }
+__device__ __forceinline__ uint32_t kc_site(uint64_t v) { return (uint32_t)(v >> 13) & 0x1FFFFu; }
+__device__ __forceinline__ uint32_t kc_size(uint64_t v) { return (uint32_t)(v >> 30) & 0x1FFFFu; }
+__device__ __forceinline__ uint32_t kc_np(uint64_t v) { return (uint32_t)(v >> 47) & 0x1FFFFu; }
 
 // kfind2 on compact words; va / vb end as the roots' words.
 __device__ __forceinline__ void kc_find2(uint64_t* uf, uint32_t& a, uint64_t& va, int& ax, int& ay,
@@ -1195,7 +982,7 @@ __device__ uint32_t kc_union(uint64_t* uf, uint32_t a, uint32_t b, int cx, int c
     uint64_t va, vb;
     kc_find2(uf, a, va, ax, ay, b, vb, bx, by);
     if (a == b) return ((cx + bx - ax) != 0 ? 1u : 0u) | ((cy + by - ay) != 0 ? 2u : 0u);
-    if (kc_mn(va) < kc_mn(vb)) {
+    if (kc_site(va) < kc_site(vb)) {
       if (atomicCAS((unsigned long long*)&uf[b], (unsigned long long)vb,
                     (unsigned long long)kc_link(a, ax - bx - cx, ay - by - cy)) == vb)
         return 0u;
@@ -1208,9 +995,11 @@ __device__ uint32_t kc_union(uint64_t* uf, uint32_t a, uint32_t b, int cx, int c
 }
 
 template <int T, int NTH>
-__global__ __launch_bounds__(NTH) void kb_merge_c(LatticeSet s, FusedScratch f) {
+__global__ __launch_bounds__(NTH) void kb_merge(LatticeSet s, FusedScratch f) {
   constexpr uint32_t NS = 4u * T;
-  constexpr int WORK = NTH - 64;  // the last wavefront folds the previous sweep first
+  // the last wavefront folds the previous sweep into its bin (one thread, a
+  // few tiles per lattice) while the others merge
+  constexpr int WORK = NTH - 64;
   extern __shared__ uint64_t kbsm[];  // one word per slot; the reduction rows at the end
   const int lat = run_lat(s, blockIdx.x);
   if (!lattice_active(s, lat)) return;
@@ -1230,10 +1019,10 @@ __global__ __launch_bounds__(NTH) void kb_merge_c(LatticeSet s, FusedScratch f) 
     if (e & SLOT_ABSENT) {
       uf[k] = kc_root(k, 0u, 0u, 0u);
     } else if (rep == loc) {
-      const uint32_t r = (uint32_t)(e >> 12) & 0xFFFu, fl = (uint32_t)(e >> 24) & 1u;
+      const uint32_t r = (uint32_t)(e >> 12) & 0xFFFu;
       const uint32_t size = (uint32_t)(e >> 25) & 0x1FFFu, np = (uint32_t)(e >> 38) & 0x1FFFu;
       const uint32_t site = ((t % nt) * T + r / T) * L + (t / nt) * T + r % T;
-      uf[k] = kc_root(k, (site << 1) | fl, size, np);
+      uf[k] = kc_root(k, site, size, np);
     } else {
       uf[k] = kc_link(t * NS + rep, 0, 0);
     }
@@ -1268,7 +1057,7 @@ __global__ __launch_bounds__(NTH) void kb_merge_c(LatticeSet s, FusedScratch f) 
     const uint32_t r = kc_rootof(uf, k);
     if (r != k) {
       const uint64_t size = (e >> 25) & 0x1FFFu, np = (e >> 38) & 0x1FFFu;
-      atomicAdd((unsigned long long*)&uf[r], (unsigned long long)((size << 29) | (np << 46)));
+      atomicAdd((unsigned long long*)&uf[r], (unsigned long long)((size << 30) | (np << 47)));
       uf[k] = kc_link(r, 0, 0);
     }
   }
@@ -1283,12 +1072,12 @@ __global__ __launch_bounds__(NTH) void kb_merge_c(LatticeSet s, FusedScratch f) 
     if (k < ns) {
       const uint32_t r = kc_rootof(uf, k);
       const uint64_t w = uf[r];
-      const uint32_t m = kc_mn(w), size = kc_size(w);
-      fl = m & 1u;
+      const uint32_t size = kc_size(w);
+      fl = (uint32_t)(ent[r] >> 24) & 1u;  // (0 at an absent position)
       if (meas_cl && r == k && size != 0u) {  // (absent slots are no cluster)
         const uint32_t np = kc_np(w);
         ++ms.ncl;
-        top2_insert(ms, ((uint64_t)size << 32) | (uint64_t)(0xFFFFFFFFu - (m >> 1)),
+        top2_insert(ms, ((uint64_t)size << 32) | (uint64_t)(0xFFFFFFFFu - kc_site(w)),
                     fl ? size - np : np);
       }
     }
@@ -1626,19 +1415,13 @@ __global__ void kf_close(LatticeSet s, FusedScratch f) {
   f.cur[lat] = 0;
 }
 
-size_t kb_lds_bytes(int nslots) { return (size_t)nslots * (8 + 8 + 4); }
-
-// The compact merge: workgroup size, LDS (one word per slot, at least the
-// reduction rows), and whether a lattice fits it (slot indices in 12 bits,
-// sites in 16).  Measured on 1280-lattice scans: 512 threads at 4096 slots (L
-// = 200, 256: four workgroups of 32 KB per CU; 256 threads 5 % and 1024
+// The slot merge: workgroup size and LDS (one word per slot, at least the
+// reduction rows).  Measured on 1280-lattice scans: 512 threads at 4096 slots
+// (L = 200, 256: four workgroups of 32 KB per CU; 256 threads 5 % and 1024
 // threads 9 % slower at L = 256), 256 at 2304 (L = 129: 5 % ahead of 512).
-int kc_threads(int nslots) { return nslots > 3072 ? 512 : 256; }
-size_t kc_lds_bytes(int nslots, int threads) {
+int kb_threads(int nslots) { return nslots > 3072 ? 512 : 256; }
+size_t kb_lds_bytes(int nslots, int threads) {
   return std::max((size_t)nslots * 8, (size_t)(threads / 64) * PARTS * 8);
-}
-bool kc_fits(const LatticeSet& s, const FusedScratch& f) {
-  return f.nslots <= (int)KC_P + 1 && s.N <= 65536;
 }
 
 // The tile passes of one sweep: every tile in one launch when T divides L,
@@ -1674,22 +1457,13 @@ hipError_t launch_fused_t(const LatticeSet& s, const FusedScratch& f, hipStream_
     hipLaunchKernelGGL(kg_sweep_end, dim3(s.n_run), dim3(KG_END_THREADS), 0, st, s, f, nb);
     return hipGetLastError();
   }
-  if (kc_fits(s, f)) {  // compact slot words
-    launch_ka<T, false, PART>(s, f, st);
-    if (kc_threads(f.nslots) == 512)
-      hipLaunchKernelGGL((kb_merge_c<T, 512>), dim3(s.n_run), dim3(512), kc_lds_bytes(f.nslots, 512),
-                         st, s, f);
-    else
-      hipLaunchKernelGGL((kb_merge_c<T, 256>), dim3(s.n_run), dim3(256), kc_lds_bytes(f.nslots, 256),
-                         st, s, f);
-    return hipGetLastError();
-  }
-  static std::atomic<uint64_t> configured{0};
-  hipError_t e = ensure_dynamic_lds((const void*)kb_merge<T>,
-                                    (int)kb_lds_bytes(RBIM_FUSED_MAX_SLOTS), configured);
-  if (e != hipSuccess) return e;
   launch_ka<T, false, PART>(s, f, st);
-  hipLaunchKernelGGL((kb_merge<T>), dim3(s.n_run), dim3(KB_THREADS), kb_lds_bytes(f.nslots), st, s, f);
+  if (kb_threads(f.nslots) == 512)
+    hipLaunchKernelGGL((kb_merge<T, 512>), dim3(s.n_run), dim3(512), kb_lds_bytes(f.nslots, 512), st,
+                       s, f);
+  else
+    hipLaunchKernelGGL((kb_merge<T, 256>), dim3(s.n_run), dim3(256), kb_lds_bytes(f.nslots, 256), st,
+                       s, f);
   return hipGetLastError();
 }
 
