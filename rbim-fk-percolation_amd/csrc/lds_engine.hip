@@ -32,6 +32,13 @@
 #include "engine.h"
 #include "tile_common.h"
 
+#ifndef RBIM_UNROLL_A
+#define RBIM_UNROLL_A 1
+#endif
+#ifndef RBIM_UNROLL_B
+#define RBIM_UNROLL_B 1
+#endif
+
 namespace {
 
 constexpr uint32_t PMASK = 0x3FFFu;  // parent field
@@ -236,10 +243,20 @@ __device__ __forceinline__ Pair make_pair(uint32_t q, uint32_t N, uint32_t Lr, u
     p.x1 = edge ? p.x0 + 1u : p.x0;
     p.y1 = edge ? 0u : p.y0 + 1u;
   }
-  p.nr0 = (p.x0 == L - 1u) ? p.y0 : p.n0 + L;
-  p.nu0 = (p.y0 == L - 1u) ? p.n0 - p.y0 : p.n0 + 1u;
-  p.nr1 = (p.x1 == L - 1u) ? p.y1 : p.n1 + L;
-  p.nu1 = (p.y1 == L - 1u) ? p.n1 - p.y1 : p.n1 + 1u;
+  if constexpr (LC > 0 && (LC & (LC - 1)) == 0) {
+    // a power of two: the periodic neighbours by masks (x + 1 wraps with n + L
+    // mod N, y + 1 within the row's bits)
+    constexpr uint32_t NM = (uint32_t)(LC * LC) - 1u, YM = (uint32_t)LC - 1u;
+    p.nr0 = (p.n0 + L) & NM;
+    p.nu0 = (p.n0 & ~YM) | ((p.n0 + 1u) & YM);
+    p.nr1 = (p.n1 + L) & NM;
+    p.nu1 = (p.n1 & ~YM) | ((p.n1 + 1u) & YM);
+  } else {
+    p.nr0 = (p.x0 == L - 1u) ? p.y0 : p.n0 + L;
+    p.nu0 = (p.y0 == L - 1u) ? p.n0 - p.y0 : p.n0 + 1u;
+    p.nr1 = (p.x1 == L - 1u) ? p.y1 : p.n1 + L;
+    p.nu1 = (p.y1 == L - 1u) ? p.n1 - p.y1 : p.n1 + 1u;
+  }
   return p;
 }
 
@@ -421,7 +438,7 @@ __device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, 
   bool pending = false;  // sweep g-1 still to be measured (block-uniform)
   for (int64_t g = g0; g < gend; ++g) {
     // ------------------------------------------------ A (+ measure g-1)
-#pragma unroll 1
+#pragma unroll RBIM_UNROLL_A
     for (int it = 0; it < iters; ++it) {
       const uint32_t q = (uint32_t)(it * NT + tid);
       const uint32_t chunk = 2u * (uint32_t)(it * NT + wave * 64);
@@ -472,7 +489,7 @@ __device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, 
     // Bonds that do not cross the periodic boundary: x bonds below row L-1 and
     // the in-row y bond that leaves the wave's chunk (lane 63).  The forest
     // has no windings yet, so plain parent chasing suffices.
-#pragma unroll 1
+#pragma unroll RBIM_UNROLL_B
     for (int it = 0; it < iters; ++it) {
       const uint32_t q = (uint32_t)(it * NT + tid);
       const Pair p = make_pair<LC, ALL>(q, N, L, magic);
@@ -555,8 +572,8 @@ __device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, 
       }
       lds_root4<BA>(uf, rr);
     }
-#pragma unroll 1
-    for (int it = 0; it < iters; ++it) {
+    // flip and count of the lane's pair of pass it
+    auto c_pass = [&](int it) {
       const uint32_t q = (uint32_t)(it * NT + tid);
       const Pair p = make_pair<LC, ALL>(q, N, L, magic);
       uint32_t r0 = 0u, r1 = 0u, ns0 = 0u, ns1 = 0u;
@@ -598,6 +615,13 @@ __device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, 
         if (rest0) atomicAdd(&lds_at<BA>(cnt, r0), 0x10000u | ns0);
         if (rest1) atomicAdd(&lds_at<BA>(cnt, r1), 0x10000u | ns1);
       }
+        };
+    if constexpr (MERGED) {
+      c_pass(0);
+      c_pass(1);
+    } else {
+#pragma unroll 1
+      for (int it = 0; it < iters; ++it) c_pass(it);
     }
     __syncthreads();
     pending = g >= n_therm;
