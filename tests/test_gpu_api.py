@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 import rbim
+from rbim import _hip
 from rbim.clusters import activate_bonds, cluster_statistics, label_and_wrap, label_clusters
 from rbim.clusters import spans_along, wraps_along
 from rbim.lattice import build_lattice, couplings_from_disorder
@@ -139,6 +140,42 @@ def test_sw_sweep_updates_in_place_and_reports():
     assert sw_sweep(spins, lattice, J, 0.5, rng, measure=False) is None
     quiet = sw_sweep(spins, lattice, J, 0.5, rng, measure_percolation=False)
     assert quiet.wraps_x is None and quiet.wraps_y is None
+
+
+@pytest.mark.parametrize("engine", [_hip.ENGINE_PLAIN, _hip.ENGINE_BATCHED])
+def test_sw_sweep_bit_exact_against_the_oracle(coracle, engine):
+    """The shim's one sweep replayed on the oracle: sw_sweep draws its Philox key
+    from the caller's generator and runs lattice id 0, sweep 0, so the oracle's
+    activation with that key, and its sweep given those bits and flip bits,
+    must give the same spins and every field of the SweepResult."""
+    from rbim.clusters import activation_threshold
+    from rbim.lattice import split_couplings
+
+    for L, T, p, seed in ((12, 2.3, 0.1, 1), (16, 1.5, 0.05, 2), (33, 2.27, 0.1, 3),
+                          (64, 3.5, 0.0, 4), (40, 1.2, 0.12, 5)):
+        lattice = build_lattice(L)
+        rng = np.random.default_rng(seed)
+        J_h = np.where(rng.random((L, L)) < p, -1, 1).astype(np.int8)
+        J_v = np.where(rng.random((L, L)) < p, -1, 1).astype(np.int8)
+        J = couplings_from_disorder(J_h, J_v)
+        spins = random_spins(lattice.n_sites, rng)
+        replay = np.random.default_rng(seed + 1000)
+        key = int(replay.integers(0, 2**64, dtype=np.uint64))
+        before = spins.copy()
+        res = sw_sweep(spins, lattice, J, 1.0 / T, np.random.default_rng(seed + 1000),
+                       engine=engine)
+        jh, jv = split_couplings(J)
+        active, flips = coracle.activate(before, jh, jv, activation_threshold(1.0 / T), key, 0, 0)
+        want, st, _, _ = coracle.sweep_given_bits(before, jh, jv, active, flips)
+        case = f"L={L} T={T} engine={engine}"
+        assert np.array_equal(spins, want.astype(np.int64)), case
+        assert (res.energy, res.magnetization, res.n_clusters) == \
+            (st["energy"], st["magnetization"], st["n_clusters"]), case
+        assert (res.largest_size, res.largest_abs_magnetization) == \
+            (st["largest_size"], st["largest_abs_mag"]), case
+        assert (res.second_size, res.second_abs_magnetization) == \
+            (st["second_size"], st["second_abs_mag"]), case
+        assert (res.wraps_x, res.wraps_y) == (st["wrap_x"], st["wrap_y"]), case
 
 
 # ---------------------------------------------------- test_physics.py
