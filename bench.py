@@ -250,7 +250,9 @@ def measure_point(L, args, steps, warmup, plain_steps, with_cpu):
                       args.streams, args.tile)
     value = res["site_updates"] / (res["ms"] / 1e3)
     n_lat = len(specs)
-    lds = L <= 128 and args.engine != 1
+    from rbim.engine import auto_uses_lds
+
+    lds = (args.engine == 2 and L <= 128) or (args.engine == 0 and auto_uses_lds(L, len(specs)))
     launches = max(1, res["launches"])
     default_shape = (args.engine, args.sweeps_per_launch, args.threads, args.tile) == (0, 0, 0, 0)
     if lds:
@@ -396,9 +398,10 @@ def measure_cfg1(args):
 
 
 def measure_cfg3(args):
-    """The ragged scan as one launch per L (each on its own stream) and as one
-    launch over every L (lds_ragged_kernel: a page of lattices of one L per
-    1024-thread workgroup)."""
+    """The ragged scan as one launch sequence per L (each on its own stream, each
+    side on the auto engine's choice) and as one launch over every L on the LDS
+    kernel (lds_ragged_kernel: a page of lattices of one L per 1024-thread
+    workgroup)."""
     specs = workload_specs("ragged")
     out = {
         "workload": "ragged scan: L in {16,32,64,128} x p in {0,0.05,0.10,0.12} x 24 T in "
@@ -407,8 +410,11 @@ def measure_cfg3(args):
     }
     by_L = {L: sum(s.L * s.L for s in specs if s.L == L) for L in (16, 32, 64, 128)}
     for single in (False, True):
-        gpu_batched(specs, 20, 0, single_launch=single)  # load the kernels
-        res = gpu_batched(specs, 11000, 0, therm_fraction=11, single_launch=single)
+        # the single launch holds every side on the LDS kernel (L = 128 too),
+        # the default picks each side's engine (the edge-slot sweep at L = 128)
+        engine = 2 if single else 0
+        gpu_batched(specs, 20, 0, engine=engine, single_launch=single)  # load the kernels
+        res = gpu_batched(specs, 11000, 0, engine=engine, therm_fraction=11, single_launch=single)
         launches = max(1, res["launches"])
         sites = res["sites"]
         item = {
@@ -418,8 +424,10 @@ def measure_cfg3(args):
                                   sites * 11000 / launches, res["ms"] / launches, launches,
                                   "lds_ragged_kernel_ragged_scan.md")
                          if single else
-                         roofline("lds_sweep_kernel (one launch sequence per L, concurrent)",
-                                  sites * 11000, res["ms"], 1, "lds_sweep_kernel_ragged_scan.md")),
+                         roofline("one launch sequence per L, concurrent: edge-slot sweep "
+                                  "(ka_tile + kb_merge) for L = 128, lds_sweep_kernel below",
+                                  sites * 11000, res["ms"], 1,
+                                  "edge_slot_sweep_and_lds_sweep_kernel_ragged_scan.md")),
         }
         out["single_launch" if single else "per_L_streams"] = item
     out["site_share"] = {str(L): by_L[L] / sum(by_L.values()) for L in by_L}

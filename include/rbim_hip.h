@@ -68,7 +68,11 @@ enum rbim_status {
 };
 
 enum rbim_engine {
-  RBIM_ENGINE_AUTO = 0,    /* LDS-resident kernel when L <= 128, else tiled  */
+  RBIM_ENGINE_AUTO = 0,    /* a batch's lattices of side L: the LDS-resident
+                              kernel when L <= 64, or L <= 128 and they hold
+                              fewer than 2^23 sites; else as PLAIN.  A slot
+                              pool: the LDS kernel when L <= 128.  The
+                              one-lattice entry points: as PLAIN              */
   RBIM_ENGINE_PLAIN = 1,   /* tiled path, sweep by sweep: the edge-slot sweep
                               on tiles of 32 or 64, the last tile row and column
                               cut where the side does not divide L (slots merged

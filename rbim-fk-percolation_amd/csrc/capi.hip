@@ -986,8 +986,27 @@ static int check_desc(rbim_ctx* ctx, const rbim_batch_desc* d) {
   return RBIM_OK;
 }
 
-static bool use_lds(const rbim_batch_desc* d, int L) {
-  return d->engine == RBIM_ENGINE_BATCHED || (d->engine == RBIM_ENGINE_AUTO && L <= RBIM_LDS_MAX_L);
+// The auto engine's choice for a set of `count` lattices of side L: the
+// LDS-resident kernel up to L = 64, and up to L = 128 while the set holds
+// fewer than 2^23 sites; the edge-slot sweep above.  Measured on scans of 1280
+// lattices (site-updates/s, LDS kernel against edge-slot sweep): L = 128
+// 1.20e11 / 1.50e11, 112 9.6e10 / 9.7e10, 96 9.4e10 / 1.12e11, 80 8.2e10 /
+// 9.5e10, 64 1.23e11 / 1.06e11, 48 8.2e10 / 5.7e10; and of 256: L = 128
+// 1.10e11 / 8.4e10 (the edge-slot sweep's two launches per sweep against the
+// LDS kernel's hundreds of sweeps per launch), L = 64 6.3e10 / 3.7e10.
+static bool auto_lds(int L, int64_t count) {
+  return L <= 64 || (L <= RBIM_LDS_MAX_L && count * (int64_t)L * L < (int64_t(1) << 23));
+}
+
+static bool use_lds(const rbim_batch_desc* d, int L, int64_t count) {
+  return d->engine == RBIM_ENGINE_BATCHED || (d->engine == RBIM_ENGINE_AUTO && auto_lds(L, count));
+}
+
+// Lattices of each side in a batch.
+static std::map<int, int64_t> side_counts(const rbim_batch_desc* d) {
+  std::map<int, int64_t> n;
+  for (int i = 0; i < d->n_lattices; ++i) n[d->L[i]]++;
+  return n;
 }
 
 static bool use_fused(const rbim_batch_desc* d, int L, bool lds) {
@@ -998,7 +1017,9 @@ int rbim_batch_footprint(const rbim_batch_desc* d, uint64_t* bytes) {
   int rc = check_desc(nullptr, d);
   if (rc != RBIM_OK) return rc;
   std::map<std::pair<int, int>, int> counts;
-  for (int i = 0; i < d->n_lattices; ++i) counts[{d->L[i], use_lds(d, d->L[i]) ? 1 : 0}]++;
+  const std::map<int, int64_t> sides = side_counts(d);
+  for (int i = 0; i < d->n_lattices; ++i)
+    counts[{d->L[i], use_lds(d, d->L[i], sides.at(d->L[i])) ? 1 : 0}]++;
   uint64_t b = 0;
   for (auto& kv : counts) {
     const bool lds = kv.first.second != 0;
@@ -1041,9 +1062,10 @@ int rbim_batch_create(rbim_ctx* ctx, const rbim_batch_desc* d, rbim_batch** out)
     return set_err(ctx, RBIM_E_HIP, "stream/event creation failed: %s", hipGetErrorString(e));
   }
   std::map<std::pair<int, int>, int> gid;
+  const std::map<int, int64_t> sides = side_counts(d);
   for (int i = 0; i < d->n_lattices; ++i) {
     const int L = d->L[i];
-    const int lds = use_lds(d, L) ? 1 : 0;
+    const int lds = use_lds(d, L, sides.at(L)) ? 1 : 0;
     auto key = std::make_pair(L, lds);
     if (!gid.count(key)) {
       gid[key] = (int)b->groups.size();

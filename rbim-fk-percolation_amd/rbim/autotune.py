@@ -4,10 +4,11 @@ For a lattice size L it times short runs of a synthetic batch (p = 0.05
 disorder, temperatures across [1, 3], hot start) over the candidate shapes and
 keeps the fastest:
 
-* LDS-resident kernel (L <= 128): workgroup size, i.e. wavefronts per lattice
-  (64 ... 1024 threads), and sweeps per launch;
-* tiled kernels (L > 128): tile side (32 or 64, the last tile row and column
-  cut when it does not divide L) and sweeps per captured hipGraph.
+* LDS-resident kernel (where the auto engine picks it: L <= 64, or L <= 128 on
+  sets of fewer than 2^23 sites): workgroup size, i.e. wavefronts per
+  lattice (64 ... 1024 threads), and sweeps per launch;
+* tiled kernels (otherwise): tile side (32 or 64, the last tile row and
+  column cut when it does not divide L) and sweeps per captured hipGraph.
 
 Results go to a JSON cache keyed by GPU architecture, engine kind (LDS or
 tiled), L and a lattice-count bucket (``$RBIM_AUTOTUNE_CACHE``, default ``~/.cache/rbim-hip/autotune.json``);
@@ -48,15 +49,19 @@ def _bucket(n_lattices: int) -> int:
     return b
 
 
-def _kind(L: int, engine: int = 0) -> Optional[str]:
-    """Engine a tuned shape applies to: "lds" (the LDS-resident kernel, L <= 128),
-    "tiled" (the tiled sweeps the auto engine picks above), or None for an
-    engine choice the tuner never timed."""
+def _kind(L: int, engine: int = 0, n_lattices: int = 1) -> Optional[str]:
+    """Engine a tuned shape applies to: "lds" (the LDS-resident kernel the auto
+    engine picks for n_lattices of side L, or BATCHED), "tiled" (the edge-slot
+    sweep it picks otherwise, or PLAIN), or None for an engine choice the tuner
+    never timed."""
     from rbim import _hip
+    from rbim.engine import auto_uses_lds
 
-    if L <= 128:
-        return "lds" if engine in (_hip.ENGINE_AUTO, _hip.ENGINE_BATCHED) else None
-    return "tiled" if engine in (_hip.ENGINE_AUTO, _hip.ENGINE_PLAIN) else None
+    if engine == _hip.ENGINE_AUTO:
+        return "lds" if auto_uses_lds(L, n_lattices) else "tiled"
+    if engine == _hip.ENGINE_BATCHED and L <= 128:
+        return "lds"
+    return "tiled" if engine == _hip.ENGINE_PLAIN else None
 
 
 def _key(L: int, n_lattices: int, kind: str, arch: str = "gfx950") -> str:
@@ -74,7 +79,7 @@ def _load() -> Dict[str, dict]:
 def lookup(L: int, n_lattices: int, engine: int = 0) -> Optional[dict]:
     """Tuned {"threads", "sweeps_per_launch", "tile"} for this shape on this
     engine choice, or None."""
-    kind = _kind(L, engine)
+    kind = _kind(L, engine, n_lattices)
     if kind is None:
         return None
     with _lock:
@@ -84,7 +89,7 @@ def lookup(L: int, n_lattices: int, engine: int = 0) -> Optional[dict]:
 def _store(L: int, n_lattices: int, entry: dict) -> None:
     with _lock:
         data = _load()
-        data[_key(L, n_lattices, _kind(L))] = entry
+        data[_key(L, n_lattices, _kind(L, 0, n_lattices))] = entry
         os.makedirs(os.path.dirname(cache_path()), exist_ok=True)
         tmp = cache_path() + ".tmp"
         with open(tmp, "w") as fh:
@@ -112,7 +117,7 @@ def tune(L: int, n_lattices: int = 1280, sweeps: int = 2000, store: bool = True)
 
     specs = _synthetic(L, n_lattices)
     sites = L * L * n_lattices
-    lds = L <= 128
+    lds = _kind(L, 0, n_lattices) == "lds"  # the engine the auto choice runs
     if lds:
         shapes = [(t, k, 0) for t in LDS_THREADS for k in LDS_SWEEPS if k <= sweeps]
     else:

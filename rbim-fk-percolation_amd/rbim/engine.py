@@ -303,6 +303,20 @@ class Batch:
             pass
 
 
+# The auto engine's choice for a set of lattices of one side (the C library's
+# rule, restated for the autotuner and callers that want to know): the LDS
+# kernel up to L = 64, and up to L = 128 while the set holds fewer than 2^23
+# sites; the edge-slot sweep above.
+AUTO_LDS_ALWAYS_L = 64
+AUTO_LDS_MAX_SITES = 1 << 23
+
+
+def auto_uses_lds(L: int, n_lattices: int) -> bool:
+    """Whether a batch's ``n_lattices`` lattices of side ``L`` run on the
+    LDS-resident kernel under ENGINE_AUTO."""
+    return L <= AUTO_LDS_ALWAYS_L or (L <= 128 and n_lattices * L * L < AUTO_LDS_MAX_SITES)
+
+
 def footprint(lattices: Sequence[LatticeSpec], *, n_bins: int, engine: int = _hip.ENGINE_AUTO,
               series: int = 0, tile: int = 0) -> int:
     """Device bytes a batch of these lattices would hold (admission control)."""
