@@ -279,6 +279,23 @@ def test_allocation_stays_within_footprint(engine):
     assert 0 < info.device_bytes <= need
 
 
+def test_auto_engine_choice_follows_the_stated_rule():
+    """ENGINE_AUTO puts a side on the LDS kernel up to L = 64, and up to L = 128
+    while its lattices hold fewer than 2^23 sites; the edge-slot sweep otherwise.
+    rbim.engine.auto_uses_lds states the rule for the autotuner and bench.py:
+    the batch's own grouping must agree with it on both sides of each limit."""
+    from rbim.engine import Batch, LatticeSpec, auto_uses_lds
+
+    for L, n in ((64, 2100), (80, 8), (80, 1400), (128, 8), (128, 511), (128, 512), (129, 4)):
+        J = np.ones((L, L), np.int8)
+        specs = [LatticeSpec(L=L, J_h=J, J_v=J, beta=0.4, seed=1, lattice_id=i) for i in range(n)]
+        with Batch(specs, n_therm=0, n_bins=1, sweeps_per_bin=1) as batch:
+            info = batch.info()
+        lds = auto_uses_lds(L, n)
+        assert (info.lds_groups > 0, info.tiled_groups > 0) == (lds, not lds), (L, n)
+    assert auto_uses_lds(128, 511) and not auto_uses_lds(128, 512) and not auto_uses_lds(129, 1)
+
+
 def test_descriptor_validation():
     from rbim.engine import Batch, LatticeSpec
 
