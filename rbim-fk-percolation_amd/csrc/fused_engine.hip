@@ -203,6 +203,14 @@ __device__ __forceinline__ void troot16_4(const uint32_t* par, uint32_t (&x)[4])
   }
 }
 
+// The HBM slot forest's root words: the slot itself (low 32 bits, as in any
+// word: gpar) and its cluster's minimum site in the upper half, where a
+// non-root word keeps its windings; roots hook by that site (kg_union).
+__device__ __forceinline__ uint64_t groot_word(uint32_t k, uint32_t site) {
+  return (uint64_t)k | ((uint64_t)site << 32);
+}
+__device__ __forceinline__ uint32_t groot_site(uint64_t v) { return (uint32_t)(v >> 32); }
+
 // ------------------------------------------------------------------- KA
 // LDS of one tile pass over a whole tile.
 template <int T, int NTH>
@@ -637,7 +645,8 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
   // edge slots for KB, and the occupation of the bonds leaving the tile
   if (BIG) {
     // the slot forest in HBM starts here: every slot under its rep, reps
-    // carrying their cluster's counts and (min site << 1 | flip)
+    // roots that carry their cluster's minimum site in the word's upper half,
+    // their counts, and (min site << 1 | flip)
     const size_t sb = (size_t)lat * f.nslots + (size_t)tile * NS;
     for (uint32_t k = tid; k < NS; k += NTH) {
       bool ok = true;
@@ -649,13 +658,15 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
       }
       const uint32_t r = par[li] & 0xFFFFu;
       const uint32_t rep = lds_get<true>(par, r) >> 16;
-      f.guf[sb + k] = gpack(tile * NS + rep, 0, 0);
       if (rep == k) {
         const uint32_t c = lds_get<true>(lcnt, r);
         f.gcnt[sb + k] = ((uint64_t)(c >> 16) << 32) | (c & 0xFFFFu);
         const uint32_t rt = tile_li(IDX(r));
-        f.gmn[sb + k] = (((x0 + rt / T) * L + y0 + rt % T) << 1) | ((bits[IDX(r)] >> 3) & 1u);
+        const uint32_t site = (x0 + rt / T) * L + y0 + rt % T;
+        f.guf[sb + k] = groot_word(tile * NS + k, site);
+        f.gmn[sb + k] = (site << 1) | ((bits[IDX(r)] >> 3) & 1u);
       } else {
+        f.guf[sb + k] = gpack(tile * NS + rep, 0, 0);
         f.gmn[sb + k] = 0xFFFFFFFFu;
       }
     }
@@ -1117,8 +1128,57 @@ __global__ __launch_bounds__(NTH) void kb_merge(LatticeSet s, FusedScratch f) {
 }
 
 // The same merge as kb_merge for lattices whose slots do not fit one
-// workgroup's LDS: the slot forest (parent | wx | wy), the rep counts and the
-// (min site << 1 | flip) words live in HBM, one thread per bond / slot.
+// workgroup's LDS: the slot forest (parent | wx | wy, roots: the slot | its
+// cluster's minimum site), the rep counts and the (min site << 1 | flip)
+// words live in HBM, one thread per bond / slot.  Roots hook by minimum site,
+// so a merged cluster's root is the rep holding its minimum site, whose flip
+// bit is the cluster's: the fold adds counts only.
+
+// gfind2 that also returns the roots' words (va, vb).
+__device__ __forceinline__ void gfind2w(uint64_t* uf, uint32_t& a, uint64_t& va, int& ax, int& ay,
+                                        uint32_t& b, uint64_t& vb, int& bx, int& by) {
+  va = gload(&uf[a]);
+  vb = gload(&uf[b]);
+  while (true) {
+    const uint32_t pa = gpar(va), pb = gpar(vb);
+    const bool da = pa == a, db = pb == b;
+    if (da && db) return;
+    const uint64_t wa = da ? va : gload(&uf[pa]);
+    const uint64_t wb = db ? vb : gload(&uf[pb]);
+    if (!da) {
+      const uint32_t ga = gpar(wa);
+      if (ga == pa) {
+        ax += gwx(va);
+        ay += gwy(va);
+        a = pa;
+        va = wa;
+      } else {
+        const int hx = gwx(va) + gwx(wa), hy = gwy(va) + gwy(wa);
+        gstore(&uf[a], gpack(ga, hx, hy));
+        ax += hx;
+        ay += hy;
+        a = ga;
+        va = gload(&uf[ga]);
+      }
+    }
+    if (!db) {
+      const uint32_t gb = gpar(wb);
+      if (gb == pb) {
+        bx += gwx(vb);
+        by += gwy(vb);
+        b = pb;
+        vb = wb;
+      } else {
+        const int hx = gwx(vb) + gwx(wb), hy = gwy(vb) + gwy(wb);
+        gstore(&uf[b], gpack(gb, hx, hy));
+        bx += hx;
+        by += hy;
+        b = gb;
+        vb = gload(&uf[gb]);
+      }
+    }
+  }
+}
 template <int T>
 __global__ __launch_bounds__(256) void kg_union(LatticeSet s, FusedScratch f) {
   constexpr uint32_t NS = 4u * T;
@@ -1147,25 +1207,28 @@ __global__ __launch_bounds__(256) void kg_union(LatticeSet s, FusedScratch f) {
   int ax = 0, ay = 0, bx = 0, by = 0;
   uint32_t w = 0u;
   while (true) {
-    gfind2(uf, a, ax, ay, c, bx, by);
+    uint64_t va, vc;
+    gfind2w(uf, a, va, ax, ay, c, vc, bx, by);
     if (a == c) {
       w = ((cx + bx - ax) != 0 ? 1u : 0u) | ((cy + by - ay) != 0 ? 2u : 0u);
       break;
     }
-    if (a < c) {
-      if (atomicCAS((unsigned long long*)&uf[c], (unsigned long long)gpack(c, 0, 0),
-                    (unsigned long long)gpack(a, ax - bx - cx, ay - by - cy)) == gpack(c, 0, 0))
+    // a root's word changes only when it is hooked: a CAS against the word
+    // the find read fails exactly when it was
+    if (groot_site(va) < groot_site(vc)) {
+      if (atomicCAS((unsigned long long*)&uf[c], (unsigned long long)vc,
+                    (unsigned long long)gpack(a, ax - bx - cx, ay - by - cy)) == vc)
         break;
     } else {
-      if (atomicCAS((unsigned long long*)&uf[a], (unsigned long long)gpack(a, 0, 0),
-                    (unsigned long long)gpack(c, bx - ax + cx, by - ay + cy)) == gpack(a, 0, 0))
+      if (atomicCAS((unsigned long long*)&uf[a], (unsigned long long)va,
+                    (unsigned long long)gpack(c, bx - ax + cx, by - ay + cy)) == va)
         break;
     }
   }
   wave_or_wraps(&f.wrapw[lat], w);
 }
 
-// Reps fold their counts and (min site << 1 | flip) into their roots.
+// Reps fold their counts into their roots.
 __global__ __launch_bounds__(256) void kg_fold(LatticeSet s, FusedScratch f) {
   const int lat = run_lat(s, blockIdx.y);
   if (!lattice_active(s, lat)) return;
@@ -1180,29 +1243,19 @@ __global__ __launch_bounds__(256) void kg_fold(LatticeSet s, FusedScratch f) {
   fold = fold && r != k;
   const uint64_t c = fold ? f.gcnt[base + k] : 0ull;
   // The folding reps of a wavefront under the first one's root (below the
-  // transition nearly all: one giant cluster) add their counts and take the
-  // minimum of their (site << 1 | flip) words by shuffles, and one lane makes
-  // the two atomics -- per-lane atomics on one address serialise in the L2.
+  // transition nearly all: one giant cluster) add their counts by a wavefront
+  // sum, and one lane makes the atomic -- per-lane atomics on one address
+  // serialise in the L2.
   const uint64_t fm = __ballot(fold);
   if (fm == 0ull) return;
   const int first = __ffsll((long long)fm) - 1;
   const uint32_t R = (uint32_t)__shfl((int)r, first);
   const bool in = fold && r == R;
   unsigned long long cs = in ? c : 0ull;
-  uint32_t mm = in ? m : 0xFFFFFFFFu;
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    cs += __shfl_xor(cs, off);
-    mm = min(mm, (uint32_t)__shfl_xor((int)mm, off));
-  }
-  if ((int)__lane_id() == first) {
-    atomicAdd((unsigned long long*)&f.gcnt[base + R], cs);
-    atomicMin(&f.gmn[base + R], mm);
-  }
-  if (fold && !in) {
-    atomicAdd((unsigned long long*)&f.gcnt[base + r], (unsigned long long)c);
-    atomicMin(&f.gmn[base + r], m);
-  }
+  for (int off = 32; off > 0; off >>= 1) cs += __shfl_xor(cs, off);
+  if ((int)__lane_id() == first) atomicAdd((unsigned long long*)&f.gcnt[base + R], cs);
+  if (fold && !in) atomicAdd((unsigned long long*)&f.gcnt[base + r], (unsigned long long)c);
 }
 
 // Flip bit of every slot's cluster; merged clusters measured at their roots
@@ -1217,12 +1270,17 @@ __global__ __launch_bounds__(256) void kg_flip(LatticeSet s, FusedScratch f) {
   const bool valid = k < ns;
   const size_t base = (size_t)lat * ns;
   const bool meas_cl = s.sweep_ctr[lat] >= lat_therm(s, lat);
-  uint32_t r = 0u, m = 0u;
+  // only reps walk: a site (and a halo read) names its edge cluster's rep slot,
+  // so the other slots' flip bits are never read (they go out as 0)
+  uint32_t r = k, m = 0xFFFFFFFFu;
   if (valid) {
-    r = groot(f.guf + base, k);
-    m = f.gmn[base + r];
+    m = f.gmn[base + k];
+    if (m != 0xFFFFFFFFu) {
+      r = groot(f.guf + base, k);
+      if (r != k) m = f.gmn[base + r];
+    }
   }
-  const uint32_t fl = m & 1u;
+  const uint32_t fl = m != 0xFFFFFFFFu ? m & 1u : 0u;
   const uint64_t word = __ballot(fl != 0u);
   if ((threadIdx.x & 63) == 0 && valid) {
     uint32_t* fbo = f.flipbits + (size_t)lat * (ns / 32u) + k / 32u;
