@@ -1039,22 +1039,44 @@ __global__ __launch_bounds__(NTH) void kb_merge(LatticeSet s, FusedScratch f) {
     }
   }
   __syncthreads();
-  // tile-crossing bonds, as in kb_merge
+  // tile-crossing bonds: bond i of tile t (i < T: the +x bond of slot (side 1,
+  // i) into the right neighbour's (side 0, i); else the +y bond of (side 3,
+  // i - T) into the upper neighbour's (side 2, i - T)).  nb and WORK are
+  // multiples of 64, so every lane of a working wavefront holds a bond, lane l
+  // - 1 the bond b - 1.  An occupied bond whose two slots have the same reps
+  // (from the tile pass's entries) as the occupied bond before it on the same
+  // edge is skipped: with that bond (or the first kept one of the run) it
+  // closes a loop through the two tiles' own clusters that crosses the edge
+  // once each way -- winding 0, no new connection.
   const uint32_t nb = (ns / NS) * 2u * T;
   uint32_t wraps = 0u;
+  const int lane = tid & 63;
   for (uint32_t b = tid; worker && b < nb; b += WORK) {
     const uint32_t t = b / (2u * T), i = b % (2u * T);
     const uint64_t word = f.xbits[((size_t)lat * (ns / NS) + t) * (2 * T / 64) + i / 64];
-    if (!((word >> (i % 64)) & 1ull)) continue;
+    const bool occ = (word >> (i % 64)) & 1ull;
     const uint32_t tx = t % nt, ty = t / nt;
+    uint32_t a, c;
+    int cx = 0, cy = 0;
     if (i < (uint32_t)T) {
-      const uint32_t tr = ty * nt + (tx + 1u == nt ? 0u : tx + 1u);
-      wraps |= kc_union(uf, t * NS + T + i, tr * NS + i, tx + 1u == nt ? 1 : 0, 0);
+      a = t * NS + T + i;
+      c = (ty * nt + (tx + 1u == nt ? 0u : tx + 1u)) * NS + i;
+      cx = tx + 1u == nt ? 1 : 0;
     } else {
       const uint32_t j = i - T;
-      const uint32_t tu = (ty + 1u == nt ? 0u : ty + 1u) * nt + tx;
-      wraps |= kc_union(uf, t * NS + 3u * T + j, tu * NS + 2u * T + j, 0, ty + 1u == nt ? 1 : 0);
+      a = t * NS + 3u * T + j;
+      c = ((ty + 1u == nt ? 0u : ty + 1u) * nt + tx) * NS + 2u * T + j;
+      cy = ty + 1u == nt ? 1 : 0;
     }
+    uint32_t ra = 0u, rc = 0u;
+    if (occ) {
+      ra = (uint32_t)(ent[a] & 0xFFFu);  // reps within the slots' own tiles
+      rc = (uint32_t)(ent[c] & 0xFFFu);
+    }
+    const int pocc = __shfl_up(occ ? 1 : 0, 1);
+    const uint32_t pra = (uint32_t)__shfl_up((int)ra, 1), prc = (uint32_t)__shfl_up((int)rc, 1);
+    if (!occ || (lane != 0 && (i % T) != 0u && pocc && pra == ra && prc == rc)) continue;
+    wraps |= kc_union(uf, a, c, cx, cy);
   }
   const uint64_t wmx = __ballot((wraps & 1u) != 0u), wmy = __ballot((wraps & 2u) != 0u);
   const uint32_t wave_wraps = (wmx ? 1u : 0u) | (wmy ? 2u : 0u);
@@ -1186,10 +1208,13 @@ __global__ __launch_bounds__(256) void kg_union(LatticeSet s, FusedScratch f) {
   if (!lattice_active(s, lat)) return;
   const uint32_t ns = (uint32_t)f.nslots, nt = (uint32_t)f.tiles_x;
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= ns / 2u) return;  // 2T bonds leave each tile: ns / 2 per lattice
+  // 2T bonds leave each tile: ns / 2 per lattice, a multiple of 64, so whole
+  // wavefronts leave here and lane l - 1 holds bond i - 1 of the same tile edge
+  // unless i is a multiple of T (a wavefront's first lane among them)
+  if (b >= ns / 2u) return;
   const uint32_t t = b / (2u * T), i = b % (2u * T);
   const uint64_t word = f.xbits[((size_t)lat * (ns / NS) + t) * (2 * T / 64) + i / 64];
-  if (!((word >> (i % 64)) & 1ull)) return;
+  const bool occ = (word >> (i % 64)) & 1ull;
   const uint32_t tx = t % nt, ty = t / nt;
   uint32_t a, c;
   int cx = 0, cy = 0;
@@ -1204,6 +1229,22 @@ __global__ __launch_bounds__(256) void kg_union(LatticeSet s, FusedScratch f) {
     cy = ty + 1u == nt ? 1 : 0;
   }
   uint64_t* uf = f.guf + (size_t)lat * ns;
+  // An occupied bond whose two slots have the same reps as the occupied bond
+  // before it along the edge is skipped: with that bond (or the first kept one
+  // of the run) it closes a loop through the two tiles' own clusters, which
+  // crosses the edge once each way -- winding 0, no new connection.  A slot's
+  // rep is itself (it carries a min-site word) or its parent as the tile pass
+  // wrote it: a slot that is no rep is a leaf, whose word only its own bond's
+  // find changes, after this read.
+  const uint32_t* mn = f.gmn + (size_t)lat * ns;
+  uint32_t ra = 0u, rc = 0u;
+  if (occ) {
+    ra = mn[a] != 0xFFFFFFFFu ? a : gpar(gload(&uf[a]));
+    rc = mn[c] != 0xFFFFFFFFu ? c : gpar(gload(&uf[c]));
+  }
+  const int pocc = __shfl_up(occ ? 1 : 0, 1);
+  const uint32_t pra = (uint32_t)__shfl_up((int)ra, 1), prc = (uint32_t)__shfl_up((int)rc, 1);
+  if (!occ || ((i % T) != 0u && pocc && pra == ra && prc == rc)) return;
   int ax = 0, ay = 0, bx = 0, by = 0;
   uint32_t w = 0u;
   while (true) {
