@@ -1076,7 +1076,8 @@ __global__ __launch_bounds__(NTH) void kb_merge(LatticeSet s, FusedScratch f) {
     const int pocc = __shfl_up(occ ? 1 : 0, 1);
     const uint32_t pra = (uint32_t)__shfl_up((int)ra, 1), prc = (uint32_t)__shfl_up((int)rc, 1);
     if (!occ || (lane != 0 && (i % T) != 0u && pocc && pra == ra && prc == rc)) continue;
-    wraps |= kc_union(uf, a, c, cx, cy);
+    // the finds start at the reps (a slot sits under its rep with winding 0)
+    wraps |= kc_union(uf, (a / NS) * NS + ra, (c / NS) * NS + rc, cx, cy);
   }
   const uint64_t wmx = __ballot((wraps & 1u) != 0u), wmy = __ballot((wraps & 2u) != 0u);
   const uint32_t wave_wraps = (wmx ? 1u : 0u) | (wmy ? 2u : 0u);
@@ -1245,6 +1246,9 @@ __global__ __launch_bounds__(256) void kg_union(LatticeSet s, FusedScratch f) {
   const int pocc = __shfl_up(occ ? 1 : 0, 1);
   const uint32_t pra = (uint32_t)__shfl_up((int)ra, 1), prc = (uint32_t)__shfl_up((int)rc, 1);
   if (!occ || ((i % T) != 0u && pocc && pra == ra && prc == rc)) return;
+  // the finds start at the reps (a slot sits under its rep with winding 0)
+  a = ra;
+  c = rc;
   int ax = 0, ay = 0, bx = 0, by = 0;
   uint32_t w = 0u;
   while (true) {
