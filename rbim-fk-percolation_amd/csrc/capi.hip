@@ -158,19 +158,25 @@ size_t group_bytes(int L, int n_lat, bool lds, bool fused, int n_bins, int serie
   return b;
 }
 
+// LDS kernel workgroup size for a full batch of side L.  Four wavefronts up to
+// 64 x 64: the kernel holds five per SIMD, so a CU keeps five such lattices at
+// once (four at L = 64 by LDS), where eight-wavefront workgroups fit two and
+// leave a 1280-lattice batch in three rounds (1280 lattices, 256 against 512
+// threads: L = 40 9.4e10 / 7.2e10, 48 1.08e11 / 8.2e10, 56 1.10e11 / 8.9e10,
+// 64 1.37e11 / 1.22e11 site-updates/s).
 int default_threads(int L) {
   const int N = L * L;
   if (N <= 256) return 64;
-  if (N <= 1024) return 256;
-  if (N <= 4096) return 512;
+  if (N <= 4096) return 256;
   return 1024;
 }
 
 // Workgroup sizes of a batch's LDS groups (threads not given): the per-L
 // default, doubled -- fewer pairs per lane, more wavefronts in flight, up to
-// one pair per lane -- while the whole batch would give the SIMDs fewer than
-// four wavefronts each.  Lattices up to 32 x 32 only (beyond, the larger
-// workgroups measured no faster).
+// one pair per lane and 1024 threads -- while the whole batch would give the
+// SIMDs fewer than four wavefronts each, for lattices up to 64 x 64 (128
+// lattices, 512 against 256 threads: L = 48 2.1e10 / 1.4e10, 64 3.2e10 /
+// 2.1e10; beyond, the larger workgroups measured no faster).
 void default_lds_threads(std::vector<Group>& groups, int n_cu) {
   const long target = 16l * (n_cu > 0 ? n_cu : 256);  // wavefronts
   long waves = 0;
@@ -183,7 +189,7 @@ void default_lds_threads(std::vector<Group>& groups, int n_cu) {
   while (waves < target && grew) {
     grew = false;
     for (Group& g : groups)
-      if (g.lds && g.N <= 1024 && 2 * g.threads <= g.N / 2) {
+      if (g.lds && g.N <= 4096 && 2 * g.threads <= g.N / 2 && 2 * g.threads <= 1024) {
         waves += (long)g.idx.size() * (g.threads / 64);
         g.threads *= 2;
         grew = true;
