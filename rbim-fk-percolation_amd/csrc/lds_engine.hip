@@ -717,7 +717,9 @@ template <int NT>
 hipError_t launch_lds_l(const LatticeSet& s, int sweeps, hipStream_t st) {
   switch (s.L) {
     case 16: return launch_lds_t<NT, 16>(s, sweeps, st);
+    case 24: return launch_lds_t<NT, 24>(s, sweeps, st);
     case 32: return launch_lds_t<NT, 32>(s, sweeps, st);
+    case 48: return launch_lds_t<NT, 48>(s, sweeps, st);
     case 64: return launch_lds_t<NT, 64>(s, sweeps, st);
     case 128: return launch_lds_t<NT, 128>(s, sweeps, st);
     default: return launch_lds_t<NT, 0>(s, sweeps, st);
@@ -777,7 +779,9 @@ int lds_blocks_per_cu_t(int L) {
   hipError_t e;
   switch (L) {
     case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lds_sweep_kernel<NT, 16>, NT, 0); break;
+    case 24: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lds_sweep_kernel<NT, 24>, NT, 0); break;
     case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lds_sweep_kernel<NT, 32>, NT, 0); break;
+    case 48: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lds_sweep_kernel<NT, 48>, NT, 0); break;
     case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lds_sweep_kernel<NT, 64>, NT, 0); break;
     case 128: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lds_sweep_kernel<NT, 128>, NT, lds_bytes(N)); break;
     default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lds_sweep_kernel<NT, 0>, NT, lds_bytes(N)); break;

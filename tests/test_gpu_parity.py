@@ -171,7 +171,7 @@ def _specs(Ls, seed=0, p=0.1, temps=(1.4, 2.27, 3.2)):
 def test_trajectories_bit_exact(hip, coracle, engine):
     from rbim.engine import Batch
 
-    specs = _specs((3, 5, 16, 17, 32, 64))
+    specs = _specs((3, 5, 16, 17, 24, 32, 48, 64))
     n_therm, spb, n_bins = 10, 9, 5
     total = n_therm + spb * n_bins
     with Batch(specs, n_therm=n_therm, n_bins=n_bins, sweeps_per_bin=spb, engine=engine,
