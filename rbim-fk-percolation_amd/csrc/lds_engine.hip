@@ -523,7 +523,7 @@ __device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, 
         // the wave's requests go through a dense queue (in cnt, cleared in B2)
         const uint32_t qa[3] = {H(p.n0), H(p.n1), H(p.n1)};
         const uint32_t qb[3] = {H(p.nr0), H(p.nr1), H(p.nu1)};
-        wave_union_queue<BA>(uf, cnt + wave * qw, req, qa, qb, below);
+        wave_union_queue<BA, true>(uf, cnt + wave * qw, req, qa, qb, below);
       } else {
         while (__ballot(req != 0u)) {
           if (req) {
@@ -542,6 +542,14 @@ __device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, 
     // under one root closes a loop whose winding c + w(b) - w(a) decides
     // wrap_x / wrap_y.
     uint32_t wraps = 0u;
+    // When one wavefront holds all of B2 (2L <= 64), the other wavefronts of
+    // the lattice wait at the next barrier for it alone: it issues ahead of
+    // the other workgroups' wavefronts on its SIMD meanwhile, above the B1
+    // unions too (the wavefront's own issue priority; at L=32 1.310e11 ->
+    // 1.341e11 site-updates/s, +1.6 % at L = 16 and 24; for two B2
+    // wavefronts, L = 48 / 64, +1.2 % / -0.8 %).
+    const bool b2_first = 2u * L <= 64u && wave == 0;
+    if (b2_first) __builtin_amdgcn_s_setprio(2);
     for (uint32_t t = tid; t < 2u * L; t += NT) {
       // t < L: y bond of row t across y = L-1; t >= L: x bond of row L-1 at y = t-L
       const bool horiz = t >= L;
@@ -550,6 +558,7 @@ __device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, 
         wraps |= lds_union<BA>(uf, H(a), H(horiz ? t - L : t * L), horiz ? 1 : 0, horiz ? 0 : 1);
     }
     wave_or_wraps(&wrapflag[g & 1], wraps);
+    if (b2_first) __builtin_amdgcn_s_setprio(0);
     // sweep g-1's partials were published before the barrier after A, and its
     // wrap slot is not the one B2 of sweep g sets
     if (pending && tid == fold_tid) finalize(g - 1);
@@ -560,6 +569,11 @@ __device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, 
     // Counts go to cnt[root] by LDS atomics; the sites of the first lane's
     // cluster (at low temperature usually most of the wave, one giant cluster)
     // are summed by ballots and added by a single atomic.
+    // The root walks are dependent LDS round trips: with one walk per site
+    // pair in flight they go ahead of other wavefronts' arithmetic (+3.5 % at
+    // L = 24 and 48); with four (MERGED) the walks overlap by themselves (-0.5 %
+    // at L=32 with the priority).
+    if constexpr (!MERGED) __builtin_amdgcn_s_setprio(1);
     uint32_t rr[4] = {0u, 0u, 0u, 0u};  // MERGED: roots of n0, n1 of both pairs
     uint32_t bb[2] = {0u, 0u};          // MERGED: state bytes of both n0
     if constexpr (MERGED) {  // even L: n1 is linked to n0 by the in-row bond bit
@@ -623,6 +637,7 @@ __device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, 
 #pragma unroll 1
       for (int it = 0; it < iters; ++it) c_pass(it);
     }
+    if constexpr (!MERGED) __builtin_amdgcn_s_setprio(0);
     __syncthreads();
     pending = g >= n_therm;
     // an unmeasured sweep's wrap bits are dropped; its slot is reused two

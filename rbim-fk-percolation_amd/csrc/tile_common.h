@@ -290,8 +290,9 @@ __device__ __forceinline__ void troot2(const uint32_t* par, uint32_t& x, uint32_
 // (bit k of req: the pair (a_k, b_k) of handles); they are written densely
 // into the wave's queue q (>= 192 entries, packed a | b << 16) and processed
 // 64 at a time, so a round has no idle lanes waiting on a neighbour's second
-// request.
-template <bool BA = false>
+// request.  PRIO: the wavefront works its queue at issue priority 1 (its
+// dependent LDS round trips go ahead of other wavefronts' arithmetic).
+template <bool BA = false, bool PRIO = false>
 __device__ __forceinline__ void wave_union_queue(uint32_t* par, uint32_t* q, uint32_t req,
                                                  const uint32_t (&a)[3], const uint32_t (&b)[3],
                                                  uint64_t below) {
@@ -306,10 +307,12 @@ __device__ __forceinline__ void wave_union_queue(uint32_t* par, uint32_t* q, uin
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (PRIO && total != 0u) __builtin_amdgcn_s_setprio(1);
   for (uint32_t i = threadIdx.x & 63u; i < total; i += 64u) {
     const uint32_t e = q[i];
     tunion<BA>(par, e & 0xFFFFu, e >> 16);
   }
+  if (PRIO && total != 0u) __builtin_amdgcn_s_setprio(0);
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
