@@ -554,7 +554,25 @@ __device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, 
       // t < L: y bond of row t across y = L-1; t >= L: x bond of row L-1 at y = t-L
       const bool horiz = t >= L;
       const uint32_t a = horiz ? (L - 1u) * L + (t - L) : t * L + (L - 1u);
-      if ((sb[a] >> (horiz ? 3 : 4)) & 1u)
+      uint32_t occ = (sb[a] >> (horiz ? 3 : 4)) & 1u;
+      // B1's plaquette skip along the boundary: the y bond of row i (i = t)
+      // is skipped when the y bond of row i - 1 and the x bonds of (i - 1,
+      // L - 1) and (i - 1, 0) are occupied; the x bond at column i (i = t - L)
+      // when the x bond at column i - 1 and the in-row bonds (L - 1, i - 1) ->
+      // (L - 1, i) and (0, i - 1) -> (0, i) are.  The four bonds close a loop
+      // that crosses the boundary once each way (winding 0); row 0 and column
+      // 0 are never skipped, so by induction along the boundary a skipped
+      // bond's ends stay joined through kept bonds with the same crossings.
+      const uint32_t i = horiz ? t - L : t;
+      if (occ && i != 0u) {
+        // the partner bond's site, and the site of the plaquette's far bond
+        const uint32_t p0 = horiz ? (L - 1u) * L + i - 1u : (i - 1u) * L + L - 1u;
+        const uint32_t q0 = horiz ? i - 1u : (i - 1u) * L;
+        const uint32_t bp = sb[p0];
+        const uint32_t sh = horiz ? 3u : 4u, oh = horiz ? 4u : 3u;  // along / across
+        occ &= ~((bp >> sh) & (bp >> oh) & (sb[q0] >> oh)) & 1u;
+      }
+      if (occ)
         wraps |= lds_union<BA>(uf, H(a), H(horiz ? t - L : t * L), horiz ? 1 : 0, horiz ? 0 : 1);
     }
     wave_or_wraps(&wrapflag[g & 1], wraps);
