@@ -435,6 +435,11 @@ __device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, 
     }
   };
 
+  // MERGED: the run starts of the lane's four sites, kept from phase A (the
+  // sites' parents then), where B1's unions and C's root walks start: a site
+  // and its run start have one root, and halving from the run start compresses
+  // the path every site of the run shares
+  uint32_t rs[4] = {0u, 0u, 0u, 0u};
   bool pending = false;  // sweep g-1 still to be measured (block-uniform)
   for (int64_t g = g0; g < gend; ++g) {
     // ------------------------------------------------ A (+ measure g-1)
@@ -472,6 +477,16 @@ __device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, 
       const bool internal = p.v1 && ov0 && p.y0 != L - 1u;
       const bool outlink = p.v1 && ov1 && p.y1 != L - 1u;
       const uint32_t start0 = run_start(__ballot(internal && outlink), __ballot(outlink), below, chunk);
+      if constexpr (MERGED) {
+        const uint32_t s1 = internal ? start0 : p.n1;
+        if (it == 0) {
+          rs[0] = start0;
+          rs[1] = s1;
+        } else {
+          rs[2] = start0;
+          rs[3] = s1;
+        }
+      }
       if (p.v0) {
         sb[p.n0] = (uint8_t)((b0 & 7u) | (oh0 << 3) | (ov0 << 4) | (fl0 << 5));
         lds_at<BA>(uf, H(p.n0)) = H(start0);
@@ -521,8 +536,22 @@ __device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, 
       }
       if (compact) {
         // the wave's requests go through a dense queue (in cnt, cleared in B2)
-        const uint32_t qa[3] = {H(p.n0), H(p.n1), H(p.n1)};
-        const uint32_t qb[3] = {H(p.nr0), H(p.nr1), H(p.nu1)};
+        uint32_t qa[3] = {H(p.n0), H(p.n1), H(p.n1)};
+        uint32_t qb[3] = {H(p.nr0), H(p.nr1), H(p.nu1)};
+        if constexpr (MERGED) {
+          // from the run starts: the lane's own, and those of n0 + L, n1 + L
+          // from the lane L / 2 above when it is in this wavefront (x bonds
+          // from row L - 1 are B2's; its lanes ask for none here)
+          const uint32_t s0 = it == 0 ? rs[0] : rs[2], s1 = it == 0 ? rs[1] : rs[3];
+          const int up = lane + (int)(L / 2u);
+          const uint32_t u0 = (uint32_t)__shfl((int)s0, up & 63), u1 = (uint32_t)__shfl((int)s1, up & 63);
+          qa[0] = H(s0);
+          qa[1] = qa[2] = H(s1);
+          if (up < 64) {
+            qb[0] = H(u0);
+            qb[1] = H(u1);
+          }
+        }
         wave_union_queue<BA, true>(uf, cnt + wave * qw, req, qa, qb, below);
       } else {
         while (__ballot(req != 0u)) {
@@ -599,8 +628,8 @@ __device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, 
       for (int it = 0; it < 2; ++it) {
         const uint32_t n0 = 2u * (uint32_t)(it * NT + tid);
         bb[it] = sb[n0];
-        rr[2 * it] = H(n0);
-        rr[2 * it + 1] = H(((bb[it] >> 4) & 1u) ? n0 : n0 + 1u);
+        rr[2 * it] = H(rs[2 * it]);  // n1's run start is n0's when they are linked
+        rr[2 * it + 1] = H(rs[2 * it + 1]);
       }
       lds_root4<BA>(uf, rr);
     }
