@@ -415,8 +415,16 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
   }
   __syncthreads();
 
-  // activation (+ E, M of the starting state) and run pre-labels
+  // activation (+ E, M of the starting state) and run pre-labels.  In a whole
+  // tile the run starts of each pair's two sites stay in registers (RSK): the
+  // unions and root walks below start there, one LDS round trip shorter (a
+  // site and its run start have one root, and halving from the run start
+  // compresses the path its whole run shares); a cut tile's pass has no
+  // registers to spare for them
+  constexpr bool RSK = !CUT;
+  constexpr int UB = RSK ? ITERS : 1;  // unrolled where the registers are indexed
   int32_t me = 0, mm = 0;
+  uint32_t rsa[ITERS], rsb[ITERS];
 #pragma unroll
   for (int it = 0; it < ITERS; ++it) {
     if (skip(it)) continue;
@@ -472,8 +480,10 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
     // never leaves its row)
     const uint32_t chunk = PART ? li0 - 2u * (uint32_t)lane : 2u * (uint32_t)(it * NTH + wave * 64);
     const uint32_t start0 = run_start(__ballot(internal && outlink), __ballot(outlink), below, chunk);
+    rsa[it] = start0;
+    rsb[it] = internal ? start0 : li1;
     par[li0] = H(start0);
-    par[li1] = internal ? H(start0) : H(li1);
+    par[li1] = rsb[it] << 2;
   }
   if (meas_em) wave_spins(me, mm, red);
   __syncthreads();
@@ -490,7 +500,7 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
 
   // remaining in-tile bonds: x bonds (plaquette skip) and the y bond leaving a
   // wave's chunk
-#pragma unroll 1
+#pragma unroll UB
   for (int it = 0; it < ITERS; ++it) {
     if (skip(it)) continue;
     const uint32_t ql = (uint32_t)(it * NTH + tid);
@@ -511,8 +521,23 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
     // the wave's queue lives in lcnt, which is cleared before it is used; the
     // unions run at issue priority 1 (L=256 +1.4 %, L=200 +2.8 %; the root
     // walks below gain nothing from it)
-    const uint32_t qa[3] = {H(li0), H(li1), H(li1)};
-    const uint32_t qb[3] = {H(li0 + RS), H(li1 + RS), H(li1 + 1u)};
+    uint32_t qa[3] = {H(li0), H(li1), H(li1)};
+    uint32_t qb[3] = {H(li0 + RS), H(li1 + RS), H(li1 + 1u)};
+    if constexpr (RSK) {
+      qa[0] = H(rsa[it]);
+      qa[1] = qa[2] = H(rsb[it]);
+    }
+    if constexpr (PART == 0) {
+      // the row above is the lanes RS / 2 up, in this wavefront for its first
+      // rows: their run starts
+      const int up = lane + (int)(RS >> 1);
+      const uint32_t u0 = (uint32_t)__shfl((int)rsa[it], up & 63);
+      const uint32_t u1 = (uint32_t)__shfl((int)rsb[it], up & 63);
+      if (up < 64) {
+        qb[0] = H(u0);
+        qb[1] = H(u1);
+      }
+    }
     wave_union_queue<true, true>(par, lcnt + wave * QW, req, qa, qb, below);
   }
   {  // this wave's queue, cleared with 16-byte stores (QW is a multiple of 4)
@@ -549,8 +574,9 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
     par[li1] = r1 != H(li1) ? r1 : H(li1) | (NONE16 << 16);
   };
   if constexpr (ITERS % 2 == 0) {
-    // two pairs per step: four root walks in flight together
-#pragma unroll 1
+    // two pairs per step: four root walks in flight together (from the run
+    // starts in a whole tile)
+#pragma unroll UB
     for (int it = 0; it < ITERS; it += 2) {
       if (skip(it)) continue;  // and so is it + 1
       const uint32_t qla = (uint32_t)(it * NTH + tid), qlb = qla + NTH;
@@ -572,6 +598,12 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
       // li1 shares li0's run when the in-row bond is occupied
       uint32_t r[4] = {H(la), H((ba0 & 4u) && va1 ? la : la + 1u), H(lb),
                        H((bb0 & 4u) && vb1 ? lb : lb + 1u)};
+      if constexpr (RSK) {
+        r[0] = H(rsa[it]);
+        r[1] = H(rsb[it]);
+        r[2] = H(rsa[it + 1]);
+        r[3] = H(rsb[it + 1]);
+      }
       troot16_4(par, r);
       count_and_point(la, ba0, ba1, r[0], r[1], va0, va1);
       count_and_point(lb, bb0, bb1, r[2], r[3], vb0, vb1);
