@@ -421,7 +421,7 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
   // site and its run start have one root, and halving from the run start
   // compresses the path its whole run shares); a cut tile's pass has no
   // registers to spare for them
-  constexpr bool RSK = !CUT;
+  constexpr bool RSK = !CUT && ITERS % 2 == 0;  // (the four-root walk below keeps the roots)
   constexpr int UB = RSK ? ITERS : 1;  // unrolled where the registers are indexed
   int32_t me = 0, mm = 0;
   uint32_t rsa[ITERS], rsb[ITERS];
@@ -605,6 +605,12 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
         r[3] = H(rsb[it + 1]);
       }
       troot16_4(par, r);
+      if constexpr (RSK) {  // the registers now keep the sites' roots (handles)
+        rsa[it] = r[0];
+        rsb[it] = r[1];
+        rsa[it + 1] = r[2];
+        rsb[it + 1] = r[3];
+      }
       count_and_point(la, ba0, ba1, r[0], r[1], va0, va1);
       count_and_point(lb, bb0, bb1, r[2], r[3], vb0, vb1);
     }
@@ -650,7 +656,7 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const uint32_t li = li0 + (uint32_t)j;
-      const uint32_t r = par[li] & 0xFFFFu;
+      const uint32_t r = RSK ? (j == 0 ? rsa[it] : rsb[it]) : par[li] & 0xFFFFu;
       const uint32_t rp = lds_get<true>(par, r);
       const uint32_t rep = rp >> 16;
       const uint32_t br = bits[IDX(r)];
