@@ -126,10 +126,21 @@ def gpu_batched(specs, K, W, engine=0, sweeps_per_launch=0, threads=0, streams=0
         info = batch.info()
         acc = batch.moments()
     sites = sum(s.L * s.L for s in specs)
-    return {"ms": ms, "wall_s": wall, "site_updates": sites * K, "sites": sites,
+    return {"ms": ms, "wall_s": wall, "site_updates": sites * K, "sites": sites, "acc": acc,
             "launches": int(info.kernel_launches), "threads": int(info.threads),
             "sweeps_per_launch": int(info.sweeps_per_launch), "groups": int(info.n_groups),
             "checksum": int(np.bitwise_xor.reduce(acc.reshape(-1) * np.uint64(0x9E3779B97F4A7C15)))}
+
+
+def plain_matches(spec, K, acc):
+    """One lattice of a batched run rerun alone on the plain path (same sweeps,
+    schedule and key): are its bin sums (all 20 slots) identical?"""
+    from rbim.engine import Batch
+
+    n_therm, n_bins, spb = run_schedule(K)
+    with Batch([spec], n_therm=n_therm, n_bins=n_bins, sweeps_per_bin=spb, engine=1) as batch:
+        batch.run(K)
+        return bool(np.array_equal(batch.moments()[0], acc))
 
 
 def gpu_plain(spec, K):
@@ -268,6 +279,9 @@ def measure_point(L, args, steps, warmup, plain_steps, with_cpu):
         roof = roofline(kernel, res["sites"], res["ms"] / steps, steps, profile,
                         "traffic_bytes_per_sweep")
     plain = gpu_plain(specs[len(specs) // 2], plain_steps)
+    # bit-exactness of the headline pair: the middle lattice's whole batched run
+    # rerun alone on the plain path
+    same = plain_matches(specs[len(specs) // 2], steps, res["acc"][len(specs) // 2])
     point = {
         "L": L,
         "value": value,
@@ -284,6 +298,7 @@ def measure_point(L, args, steps, warmup, plain_steps, with_cpu):
         "plain": {"value": plain, "unit": "site-updates/s", "L": L,
                   "sample": f"one lattice, {plain_steps} sweeps, tiled kernels in hipGraphs"},
         "batched_vs_plain": value / plain,
+        "plain_moments_identical": same,
     }
     if with_cpu:
         cpu = cpu_baseline(specs, args.cpu_site_updates, args.cpu_per_thread)
@@ -310,7 +325,7 @@ def measure_cut(args):
                                          1100, "edge_slot_sweep_L200_cut_tiles.md",
                                          "traffic_bytes_per_sweep", bound="valu")
             point["roofline"]["bound_basis"] = (
-                "the tile pass's measured VALUBusy on whole tiles (98.7 % at L=256, "
+                "the tile pass's measured VALUBusy on whole tiles (101 % at L=256, "
                 "edge_slot_sweep_L256_batched.md); this item's own counters are not recorded")
         out["points"][str(L)] = point
     return out
@@ -639,6 +654,7 @@ def main():
             "roofline": main_pt["roofline"],
             "plain": main_pt["plain"],
             "batched_vs_plain": main_pt["batched_vs_plain"],
+            "plain_moments_identical": main_pt["plain_moments_identical"],
         })
         if "cpu_baseline" in main_pt:
             line["cpu_baseline"] = main_pt["cpu_baseline"]
