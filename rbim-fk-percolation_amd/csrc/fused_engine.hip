@@ -527,15 +527,13 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
       qa[0] = H(rsa[it]);
       qa[1] = qa[2] = H(rsb[it]);
     }
-    if constexpr (PART == 0) {
-      // the row above is the lanes RS / 2 up, in this wavefront for its first
-      // rows: their run starts
-      const int up = lane + (int)(RS >> 1);
-      const uint32_t u0 = (uint32_t)__shfl((int)rsa[it], up & 63);
-      const uint32_t u1 = (uint32_t)__shfl((int)rsb[it], up & 63);
-      if (up < 64) {
-        qb[0] = H(u0);
-        qb[1] = H(u1);
+    if constexpr (RSK) {
+      // the row above from its parent words, read here ahead of the dependent
+      // finds (any parent word names an ancestor; the forest has no rep bits
+      // yet)
+      if (lx + 1u < wx) {
+        qb[0] = par[li0 + RS];
+        qb[1] = par[li1 + RS];
       }
     }
     wave_union_queue<true, true>(par, lcnt + wave * QW, req, qa, qb, below);

@@ -539,18 +539,15 @@ __device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, 
         uint32_t qa[3] = {H(p.n0), H(p.n1), H(p.n1)};
         uint32_t qb[3] = {H(p.nr0), H(p.nr1), H(p.nu1)};
         if constexpr (MERGED) {
-          // from the run starts: the lane's own, and those of n0 + L, n1 + L
-          // from the lane L / 2 above when it is in this wavefront (x bonds
-          // from row L - 1 are B2's; its lanes ask for none here)
+          // from the run starts: the lane's own, and for n0 + L, n1 + L their
+          // parent words, read here ahead of the dependent finds (any parent
+          // word names an ancestor, and B1's forest carries no windings; the
+          // reads of the lanes that ask for no union go unused)
           const uint32_t s0 = it == 0 ? rs[0] : rs[2], s1 = it == 0 ? rs[1] : rs[3];
-          const int up = lane + (int)(L / 2u);
-          const uint32_t u0 = (uint32_t)__shfl((int)s0, up & 63), u1 = (uint32_t)__shfl((int)s1, up & 63);
           qa[0] = H(s0);
           qa[1] = qa[2] = H(s1);
-          if (up < 64) {
-            qb[0] = H(u0);
-            qb[1] = H(u1);
-          }
+          qb[0] = lds_get<BA>(uf, H(p.nr0));
+          qb[1] = lds_get<BA>(uf, H(p.nr1));
         }
         wave_union_queue<BA, true>(uf, cnt + wave * qw, req, qa, qb, below);
       } else {
