@@ -499,7 +499,10 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
   }
 
   // remaining in-tile bonds: x bonds (plaquette skip) and the y bond leaving a
-  // wave's chunk
+  // wave's chunk, through the wave's queue in lcnt (cleared before the counts
+  // use it), worked when the next pass could overflow it and after the last
+  uint32_t* wq = lcnt + wave * QW;
+  uint32_t qn = 0u;
 #pragma unroll UB
   for (int it = 0; it < ITERS; ++it) {
     if (skip(it)) continue;
@@ -518,8 +521,7 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
       req = h0 | (h1 << 1);
     }
     req |= ((b1 >> 2) & 1u & ((lane == 63 && ly0 + 2u < wy) ? 1u : 0u)) << 2;
-    // the wave's queue lives in lcnt, which is cleared before it is used; the
-    // unions run at issue priority 1 (L=256 +1.4 %, L=200 +2.8 %; the root
+    // the unions run at issue priority 1 (L=256 +1.4 %, L=200 +2.8 %; the root
     // walks below gain nothing from it)
     uint32_t qa[3] = {H(li0), H(li1), H(li1)};
     uint32_t qb[3] = {H(li0 + RS), H(li1 + RS), H(li1 + 1u)};
@@ -536,8 +538,13 @@ __device__ __forceinline__ void ka_body(const LatticeSet& s, const FusedScratch&
         qb[1] = par[li1 + RS];
       }
     }
-    wave_union_queue<true, true>(par, lcnt + wave * QW, req, qa, qb, below);
+    if (qn + WAVE_QUEUE_PASS > QW) {
+      wave_queue_run<true, true>(par, wq, qn);
+      qn = 0u;
+    }
+    qn = wave_queue_put(wq, qn, req, qa, qb, below);
   }
+  wave_queue_run<true, true>(par, wq, qn);
   {  // this wave's queue, cleared with 16-byte stores (QW is a multiple of 4)
     uint4* q4 = reinterpret_cast<uint4*>(lcnt + wave * QW);
     for (uint32_t i = lane; i < QW / 4u; i += 64u) q4[i] = make_uint4(0u, 0u, 0u, 0u);

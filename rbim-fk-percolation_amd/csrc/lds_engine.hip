@@ -503,7 +503,9 @@ __device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, 
     // ----------------------------------------------------------- B1
     // Bonds that do not cross the periodic boundary: x bonds below row L-1 and
     // the in-row y bond that leaves the wave's chunk (lane 63).  The forest
-    // has no windings yet, so plain parent chasing suffices.
+    // has no windings yet, so plain parent chasing suffices.  With compact
+    // queues the passes fill one queue per wave (in cnt, cleared in B2).
+    uint32_t qn = 0u;
 #pragma unroll RBIM_UNROLL_B
     for (int it = 0; it < iters; ++it) {
       const uint32_t q = (uint32_t)(it * NT + tid);
@@ -549,7 +551,12 @@ __device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, 
           qb[0] = lds_get<BA>(uf, H(p.nr0));
           qb[1] = lds_get<BA>(uf, H(p.nr1));
         }
-        wave_union_queue<BA, true>(uf, cnt + wave * qw, req, qa, qb, below);
+        // worked when the next pass could overflow it, and after the last
+        if (qn + WAVE_QUEUE_PASS > qw) {
+          wave_queue_run<BA, true>(uf, cnt + wave * qw, qn);
+          qn = 0u;
+        }
+        qn = wave_queue_put(cnt + wave * qw, qn, req, qa, qb, below);
       } else {
         while (__ballot(req != 0u)) {
           if (req) {
@@ -561,6 +568,7 @@ __device__ __forceinline__ void lds_sweeps(const LatticeSet& s, int32_t sweeps, 
         }
       }
     }
+    if (compact) wave_queue_run<BA, true>(uf, cnt + wave * qw, qn);
     __syncthreads();
     // ----------------------------------------------------------- B2
     // The at most 2L bonds across the periodic boundary, with windings: x

@@ -286,36 +286,47 @@ __device__ __forceinline__ void troot2(const uint32_t* par, uint32_t& x, uint32_
   }
 }
 
-// Unions of one wavefront, compacted: every lane holds up to three requests
-// (bit k of req: the pair (a_k, b_k) of handles); they are written densely
-// into the wave's queue q (>= 192 entries, packed a | b << 16) and processed
-// 64 at a time, so a round has no idle lanes waiting on a neighbour's second
-// request.  PRIO: the wavefront works its queue at issue priority 1 (its
-// dependent LDS round trips go ahead of other wavefronts' arithmetic).
-template <bool BA = false, bool PRIO = false>
-__device__ __forceinline__ void wave_union_queue(uint32_t* par, uint32_t* q, uint32_t req,
-                                                 const uint32_t (&a)[3], const uint32_t (&b)[3],
-                                                 uint64_t below) {
+// Unions of one wavefront through a dense queue q (packed a | b << 16): every
+// lane holds up to three requests (bit k of req: the pair (a_k, b_k) of
+// handles), appended at the queue's length n (wavefront-uniform), which the
+// call returns grown.  The queue is then worked 64 entries at a time, so a
+// round has no idle lanes waiting on a neighbour's second request -- and,
+// filled over several passes of a wavefront's pairs, fewer rounds than one
+// queue per pass (the requests depend on the occupation bits only, never on
+// the forest).  A pass adds at most 129 (two per lane, three for lane 63).
+__device__ __forceinline__ uint32_t wave_queue_put(uint32_t* q, uint32_t n, uint32_t req,
+                                                   const uint32_t (&a)[3], const uint32_t (&b)[3],
+                                                   uint64_t below) {
   const uint32_t c = (uint32_t)__popc(req);
   const uint64_t c0 = __ballot(c & 1u), c1 = __ballot(c & 2u);
-  uint32_t pos = (uint32_t)(__popcll(c0 & below) + 2 * __popcll(c1 & below));
-  const uint32_t total = (uint32_t)(__popcll(c0) + 2 * __popcll(c1));
+  uint32_t pos = n + (uint32_t)(__popcll(c0 & below) + 2 * __popcll(c1 & below));
 #pragma unroll
   for (int k = 0; k < 3; ++k)
     if ((req >> k) & 1u) q[pos++] = a[k] | (b[k] << 16);
-  // the queue is read by other lanes of this wavefront only
+  return n + (uint32_t)(__popcll(c0) + 2 * __popcll(c1));
+}
+
+// The unions of entries [0, n) of the wave's queue; afterwards the queue may
+// be filled again.  PRIO: the wavefront works its queue at issue priority 1
+// (its dependent LDS round trips go ahead of other wavefronts' arithmetic).
+template <bool BA = false, bool PRIO = false>
+__device__ __forceinline__ void wave_queue_run(uint32_t* par, const uint32_t* q, uint32_t n) {
+  // the queue is written and read by the lanes of this wavefront only
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  if (PRIO && total != 0u) __builtin_amdgcn_s_setprio(1);
-  for (uint32_t i = threadIdx.x & 63u; i < total; i += 64u) {
+  if (PRIO && n != 0u) __builtin_amdgcn_s_setprio(1);
+  for (uint32_t i = threadIdx.x & 63u; i < n; i += 64u) {
     const uint32_t e = q[i];
     tunion<BA>(par, e & 0xFFFFu, e >> 16);
   }
-  if (PRIO && total != 0u) __builtin_amdgcn_s_setprio(0);
+  if (PRIO && n != 0u) __builtin_amdgcn_s_setprio(0);
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+
+// Most requests a pass of wave_queue_put adds.
+constexpr uint32_t WAVE_QUEUE_PASS = 129u;
 
 }  // namespace
